@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Electron-moves/s of 128-electron DMC on one MI355X (FP64 and mixed precision).
+
+Workload: the seeded 128-electron molecule ae32 (32 Z=4 all-electron atoms, 64+64 electrons,
+416 spline basis functions, Slater-Jastrow), DMC with timestep 0.01 and 8192 walkers resident
+in HBM.  A step is one DMC step of the whole population: 128 single-electron moves per walker,
+the local energy, the weight update (and branching every fifth step).
+
+Prints one JSON line: the headline value is the mixed-precision rate when the mixed path is
+available, otherwise FP64; both are reported, with the roofline of the dominant kernel
+(the electron-move sweep) and a CPU baseline timed on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15}
+HBM = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flops_per_move(nb, ns, ne, na, nshell, n_ee, n_ei, p_acc):
+    """Algorithmic flops of one electron move: dense LCAO (5 components), Sherman-Morrison,
+    ratio + gradient, Jastrow pairs (old and new position), distances and splines."""
+    return (10.0 * nb * ns + 4.0 * ns * ns * p_acc + 10.0 * ns + 2 * (ne - 1) * (12 + 30 * n_ee)
+            + 2 * na * (12 + 30 * n_ei) + 10 * na + 60 * nshell)
+
+
+def bytes_per_move(ns, ne, s_inv, p_acc):
+    """Algorithmic HBM bytes of one move with the inverse held on chip for the whole sweep:
+    MO-cache row read (gradients) and write on acceptance, position and inverse traffic
+    amortised over the spin channel's moves (load + store of the inverse once per sweep)."""
+    return 3 * ns * s_inv + 5 * ns * s_inv * p_acc + 2.0 * ns * ns * s_inv / ns + 24
+
+
+def run_gpu(deck, mol, precision, nw, steps, warmup, tau):
+    from mainline_amd import engine as E
+    from mainline_amd import synth
+    eng = E.Engine(deck, precision, seed=20261204)
+    pos = synth.random_configs(mol, nw, seed=5)
+    eng.set_walkers(pos)
+    log("[%s] warmup %d steps" % (precision, warmup))
+    eng.run(E.method("DMC", nw, nblock=1, nstep=max(1, warmup), timestep=tau))
+    log("[%s] timed %d steps" % (precision, steps))
+    t0 = time.perf_counter()
+    res = eng.run(E.method("DMC", nw, nblock=1, nstep=steps, timestep=tau))
+    dt = time.perf_counter() - t0
+    kt = eng.kernel_times()
+    moves = nw * mol.nelec * steps
+    return dict(moves_per_s=moves / dt, seconds=dt, ms_per_step=1e3 * dt / steps, energy=float(res["eblock"][0, 0]),
+                acceptance=float(res["acceptance"]), kernels={k: [v[0], int(v[1])] for k, v in kt.items()})
+
+
+def cpu_baseline(deck, mol, seconds_target):
+    """The oracle's FP64 CPU engine (same algorithm, threads over walkers) on a bounded sample."""
+    from oracle import oracle as O
+    from mainline_amd import synth
+    threads = min(16, os.cpu_count() or 1)
+    orc = O.Oracle(deck)
+    nw = max(threads, 16)
+    pos = synth.random_configs(mol, nw, seed=6)
+    t0 = time.perf_counter()
+    orc.run(pos, "DMC", nblock=1, nstep=1, timestep=0.01, seed=3, nthreads=threads)
+    t1 = time.perf_counter() - t0
+    nstep = max(1, int(seconds_target / max(t1, 1e-3)))
+    t0 = time.perf_counter()
+    orc.run(pos, "DMC", nblock=1, nstep=nstep, timestep=0.01, seed=4, nthreads=threads)
+    dt = time.perf_counter() - t0
+    moves = nw * mol.nelec * nstep
+    return dict(value=moves / dt, unit="electron-moves/s", cores=threads, kind="port",
+                sample="oracle FP64 DMC, %d walkers x %d steps of the 128-electron workload (%.1f s)" % (nw, nstep, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--walkers", type=int, default=8192)
+    ap.add_argument("--precision", default="auto", help="fp64, mixed_fp32, mixed_bf16 or auto")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the walker engine runs on a single MI355X in this build; "
+              "multi-GPU sharding is described in DESIGN.md and not run here.", file=sys.stderr)
+        sys.exit(2)
+    from mainline_amd import synth
+    mol = synth.ae32()
+    tmp = tempfile.mkdtemp(prefix="qwbench_")
+    deck = mol.write(tmp)
+    tau = 0.01
+    results = {}
+    precs = ["fp64", "mixed_bf16"] if a.precision == "auto" else [a.precision]
+    for p in precs:
+        try:
+            results[p] = run_gpu(deck, mol, p, a.walkers, a.steps, a.warmup, tau)
+            log("[%s] %.3e electron-moves/s, %.2f ms/step" % (p, results[p]["moves_per_s"], results[p]["ms_per_step"]))
+        except Exception as ex:  # the mixed path may be absent in this build
+            if p == "fp64" or a.precision != "auto":
+                raise
+            log("[%s] unavailable: %s" % (p, ex))
+    head = "mixed_bf16" if "mixed_bf16" in results else "fp64"
+    R = results[head]
+    ns, nb, ne, na = mol.nup, mol.nbasis, mol.nelec, mol.natoms
+    p_acc = R["acceptance"] if R["acceptance"] > 0 else 0.99
+    kname = "k_sweep" if "k_sweep" in R["kernels"] else next(iter(R["kernels"]))
+    avg_ms = R["kernels"][kname][0]
+    moves_per_launch = a.walkers * ns
+    f_em = flops_per_move(nb, ns, ne, na, 5 * na, 5, 4, p_acc)
+    dt = "f64" if head == "fp64" else "bf16"
+    achieved = f_em * moves_per_launch / (avg_ms * 1e-3)
+    roof = {"bound": "mfma" if head != "fp64" else "valu-fp64", "achieved": achieved / 1e12,
+            "peak": PEAK[dt] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[dt], "traffic": None,
+            "kernel": kname, "avg_ms": avg_ms, "flops_per_move": f_em, "moves_per_launch": moves_per_launch}
+    cpu = None if a.no_cpu else cpu_baseline(deck, mol, a.cpu_seconds)
+    line = {
+        "metric": "electron-moves/s, 128e DMC, mixed vs FP64 at equal E_L; p50/p99 job latency",
+        "value": R["moves_per_s"], "unit": "electron-moves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": R["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "mixed(bf16-split LCAO, f32 inverse, f64 log-det/weights/sums)",
+        "data": "synthetic (seeded molecule, random-init walkers)",
+        "config": {"workload": "ae32: 128-electron DMC, 32 Z=4 atoms, 416 spline basis functions, "
+                               "Slater-Jastrow, tau 0.01", "walkers": a.walkers, "electrons": ne,
+                   "basis": nb, "precision": head},
+        "roofline": roof, "cpu_baseline": cpu,
+        "fp64": {k: results["fp64"][k] for k in ("moves_per_s", "ms_per_step", "energy", "acceptance")},
+    }
+    if "mixed_bf16" in results:
+        line["mixed_bf16"] = {k: results["mixed_bf16"][k] for k in ("moves_per_s", "ms_per_step", "energy", "acceptance")}
+        line["mixed_over_fp64"] = results["mixed_bf16"]["moves_per_s"] / results["fp64"]["moves_per_s"]
+    line["kernels"] = R["kernels"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,113 @@
+/* qwalk_amd: MI355X walker engine for QWalk's VMC/DMC hot path — C ABI.
+ *
+ * Plain C types only; no exceptions cross this boundary.  Inputs are host-owned and copied;
+ * device state is owned by the engine.  One engine per host thread; each engine owns one
+ * HIP stream on the device it was created on.  Every call returns QW_OK (0) or a negative
+ * status; qw_last_error() then holds a message.
+ *
+ * Which reference interface each entry point replaces (paths relative to QWalk's tree):
+ *   qw_engine_create     Wavefunction_data / System / Pseudopotential construction from
+ *                        the TRIALFUNC / SYSTEM / PSEUDO input
+ *                        (src/wavefunction/Wavefunction_data.cpp:32-59,
+ *                         src/system/System.cpp:29-49,145-149)
+ *   qw_eval_fixed        Wavefunction::updateLap + getVal + getLap per electron, plus
+ *                        System::calcKinetic, calcLoc and Pseudopotential::calcNonlocWithTest
+ *                        at fixed configurations (src/wavefunction/Wavefunction.h:117-256,
+ *                        src/properties/Properties_gather.cpp:40-74,
+ *                        src/method/Test_method.cpp:128-176)
+ *   qw_method_read       METHOD { VMC | DMC } keyword reading with the reference defaults
+ *                        (src/method/Vmc_method.cpp:33-132, src/method/Dmc_method.cpp:33-160)
+ *   qw_walkers_set/get   Config_save_point restorePos / savePos for a walker population
+ *                        (src/method/Vmc_method.cpp:394-399,492)
+ *   qw_run               Vmc_method::runWithVariables / Dmc_method::runWithVariables
+ *                        (src/method/Vmc_method.cpp:351-558, src/method/Dmc_method.cpp:328-567)
+ *   qw_kernel_times      per-kernel device timings (no reference counterpart: the
+ *                        reference prints only a wall time per method,
+ *                        src/program_utils/ooqmc.cpp:158-168)
+ */
+#ifndef QWALK_AMD_H
+#define QWALK_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qw_engine qw_engine;
+typedef int qw_status;
+
+enum {
+  QW_OK = 0,
+  QW_ERR_ARG = -1,          /* bad argument (null pointer, size out of range) */
+  QW_ERR_INPUT = -2,        /* deck could not be read or is inconsistent */
+  QW_ERR_UNSUPPORTED = -3,  /* valid QWalk input outside this engine's scope */
+  QW_ERR_DEVICE = -4,       /* HIP error */
+  QW_ERR_NOMEM = -5         /* device allocation failed */
+};
+
+enum {
+  QW_PREC_FP64 = 0,        /* everything in FP64 */
+  QW_PREC_MIXED_FP32 = 1,  /* LCAO FP32, rank-1 updates FP32, FP64 LU refresh / log-det / sums */
+  QW_PREC_MIXED_BF16 = 2   /* as MIXED_FP32, LCAO on MFMA in BF16 with split-precision correction */
+};
+
+typedef struct qw_engine_opts {
+  int precision;            /* QW_PREC_* */
+  int device;               /* HIP device ordinal */
+  unsigned long long seed;  /* key of the counter-based random streams */
+  int refresh_interval;     /* mixed precision: FP64 LU recompute every this many moves (0 = 32) */
+} qw_engine_opts;
+
+typedef struct qw_method_desc {
+  int method;               /* 0 = VMC, 1 = DMC */
+  int nconfig;              /* walkers */
+  int nblock;
+  int nstep;
+  int ndecorr;              /* VMC: electron sweeps per step */
+  double timestep;
+  int feedback_interval;    /* DMC: steps between trial-energy updates and branching */
+  double feedback;
+  double max_weight;        /* DMC: MAX_POSS_WEIGHT */
+  double branch_start_cutoff;
+  int pp_stochastic;        /* 1 = the reference's default stochastic nonlocal evaluation */
+} qw_method_desc;
+
+/* Engine for the SYSTEM / PSEUDO / TRIALFUNC sections of a QWalk deck.  INCLUDE and ORBFILE
+ * paths are resolved relative to the deck's directory.  On failure *out is NULL and the
+ * message is available from qw_last_error(NULL). */
+qw_status qw_engine_create(const char* deck_path, const qw_engine_opts* opts, qw_engine** out);
+void qw_engine_destroy(qw_engine* e);
+const char* qw_last_error(const qw_engine* e);
+
+/* out6 = nup, ndn, natoms, nbasis, has_pseudo, max walkers resident so far */
+qw_status qw_engine_info(const qw_engine* e, int* out6);
+
+/* Fixed-configuration evaluation.
+ *   pos[K][Ne][3]; rot[9] quadrature rotation (rows = rotated x, y, z axes) or NULL;
+ *   pp_mode 0 = deterministic (static cutoff), 1 = stochastic with uniforms pp_rand[K][natoms][Ne]
+ * outputs: logpsi[K] (ln|Psi|), sign[K], grad[K][Ne][3] (grad Psi / Psi),
+ *          lap[K][Ne] (laplacian Psi / Psi), eparts[K][3] = kinetic, Coulomb, nonlocal. */
+qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const double* rot, int pp_mode,
+                        const double* pp_rand, double* logpsi, double* sign, double* grad,
+                        double* lap, double* eparts);
+
+/* Read the first METHOD section of a deck into *out (reference defaults applied). */
+qw_status qw_method_read(const char* deck_path, qw_method_desc* out);
+
+/* Walker population: pos[nw][Ne][3], weights[nw] (NULL = all 1). */
+qw_status qw_walkers_set(qw_engine* e, int nw, const double* pos, const double* weights);
+qw_status qw_walkers_get(qw_engine* e, int nw, double* pos, double* weights);
+
+/* Run VMC or DMC on the resident walkers.
+ *   eblock[nblock][5] = weighted block averages of E, kinetic, Coulomb, nonlocal, and the
+ *   block weight sum; stats[4] = acceptance, electron moves done, seconds, electron-moves/s. */
+qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* eblock, double* stats);
+
+/* Per-kernel device time of the last qw_run: names (comma separated) into buf, and
+ * avg_ms[i], count[i] for up to n kernels; returns the number of kernels in *nk. */
+qw_status qw_kernel_times(const qw_engine* e, char* buf, int buflen, double* avg_ms, int* count,
+                          int n, int* nk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,428 @@
+// Device-side data layout and helper functions shared by the walker-engine kernels.
+// One 64-lane wavefront works on one walker; lanes map to molecular orbitals in the LCAO,
+// to partner electrons / atoms in the Jastrow pair sums, and to spline shells in the basis.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "qw_setup.hpp"
+
+namespace qw {
+
+constexpr int kWave = 64;
+
+// Read-only system tables in device memory.
+struct DevSys {
+  int nup, ndn, ne, na, nb, nsh, ntab, nint;
+  int nee, nchunk, has_pp, nsp;  // nsp = max(nup, ndn): row length of the MO cache
+  const double* atom_pos;
+  const double* atom_Z;
+  const ShellDesc* shells;
+  const int* chunk_sh0;   // [nchunk+1] shell ranges whose functions fit one LDS tile
+  const double* spline;   // [ntab][nint][4]
+  const double* tab_inv;  // [ntab] 1/spacing (interval lookup)
+  const double* tab_sp;   // [ntab] spacing (height within the interval)
+  const double* C[2];     // [nb][nspin]
+  const float* Cf[2];     // FP32 copies
+  const JObj* ee;
+  const JObj* ei;
+  const int* atom_ei0;
+  const PPType* pp;
+  const int* atom_pp;
+  double vii;
+  int chunk_nf;           // max functions per chunk (LDS tile width)
+};
+
+// Per-walker state in device memory (walker-major).
+struct DevWalkers {
+  int nw;
+  double* pos;     // [nw][ne][3]
+  double* inv;     // [nw][2][nsp][nsp]  transposed inverse per spin (FP64 path)
+  float* invf;     // [nw][2][nsp][nsp]  FP32 path
+  double* mo;      // [nw][ne][5][nsp]   value, gradient, laplacian of the occupied MOs
+  float* mof;      // FP32 path
+  double* logdet;  // [nw][2]
+  double* sgn;     // [nw][2]
+  double* weight;  // [nw]
+  double* eold;    // [nw] local energy of the previous step
+  double* elast;   // [nw][4] last kinetic, Coulomb, nonlocal, total
+  int* acc;        // [nw] accepted moves since the last reset
+  double* bsum;    // [nw][6] block sums: w, wE, wKE, wV, wNL, steps
+};
+
+// ---------------------------------------------------------------- wave reductions
+__device__ inline double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline float wsumf(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// ---------------------------------------------------------------- Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC'11).
+__device__ inline void philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1,
+                              unsigned out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    unsigned h0 = __umulhi(0xD2511F53u, c0), l0 = 0xD2511F53u * c0;
+    unsigned h1 = __umulhi(0xCD9E8D57u, c2), l1 = 0xCD9E8D57u * c2;
+    unsigned n0 = h1 ^ c1 ^ k0, n2 = h0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = l1;
+    c2 = n2;
+    c3 = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+// Stream tags (counter word 3); the sweep number of a step sits in bits 8 and up.
+enum : unsigned { TAG_GAUSS1 = 0, TAG_ACC1 = 1, TAG_GAUSS2 = 2, TAG_ACC2 = 3, TAG_PP = 4, TAG_ROT = 5, TAG_BRANCH = 6 };
+__device__ inline double u53(unsigned a, unsigned b) {
+  return ((a >> 5) * 67108864.0 + (b >> 6)) * (1.0 / 9007199254740992.0);
+}
+__device__ inline double u32o(unsigned a) { return (a + 0.5) * (1.0 / 4294967296.0); }
+__device__ inline void gauss3(unsigned long long seed, unsigned w, unsigned step, unsigned idx, unsigned tag, double g[3]) {
+  unsigned o[4];
+  philox(w, step, idx, tag, (unsigned)seed, (unsigned)(seed >> 32), o);
+  const double twopi = 6.283185307179586476925;
+  double r0 = sqrt(-2.0 * log(u32o(o[0]))), t0 = twopi * u32o(o[1]);
+  double r1 = sqrt(-2.0 * log(u32o(o[2]))), t1 = twopi * u32o(o[3]);
+  double s0, c0, s1, c1;
+  sincos(t0, &s0, &c0);
+  sincos(t1, &s1, &c1);
+  g[0] = r0 * c0;
+  g[1] = r0 * s0;
+  g[2] = r1 * c1;
+}
+__device__ inline double uniform(unsigned long long seed, unsigned w, unsigned step, unsigned idx, unsigned tag) {
+  unsigned o[4];
+  philox(w, step, idx, tag, (unsigned)seed, (unsigned)(seed >> 32), o);
+  return u53(o[0], o[1]);
+}
+// Uniform random rotation from a unit quaternion (Shoemake, Graphics Gems III, 1992);
+// rows of rot are the rotated x, y, z axes.
+__device__ inline void random_rotation(unsigned long long seed, unsigned w, unsigned step, double rot[9]) {
+  unsigned o[4];
+  philox(w, step, 0, TAG_ROT, (unsigned)seed, (unsigned)(seed >> 32), o);
+  const double twopi = 6.283185307179586476925;
+  double u1 = u53(o[0], o[1]), u2 = u32o(o[2]), u3 = u32o(o[3]);
+  double a = sqrt(1 - u1), b = sqrt(u1);
+  double s2, c2, s3, c3;
+  sincos(twopi * u2, &s2, &c2);
+  sincos(twopi * u3, &s3, &c3);
+  double qw = a * s2, qx = a * c2, qy = b * s3, qz = b * c3;
+  rot[0] = 1 - 2 * (qy * qy + qz * qz);
+  rot[1] = 2 * (qx * qy - qz * qw);
+  rot[2] = 2 * (qx * qz + qy * qw);
+  rot[3] = 2 * (qx * qy + qz * qw);
+  rot[4] = 1 - 2 * (qx * qx + qz * qz);
+  rot[5] = 2 * (qy * qz - qx * qw);
+  rot[6] = 2 * (qx * qz - qy * qw);
+  rot[7] = 2 * (qy * qz + qx * qw);
+  rot[8] = 1 - 2 * (qx * qx + qy * qy);
+}
+
+// Cyrus drift limiter, a = 0.25 (src/method/Split_sample.cpp:58-78).
+__device__ inline void lim_drift(const double v[3], double tau, double out[3]) {
+  double d2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * 0.25;
+  double da = d2 > 1e-8 ? (sqrt(1. + 2 * tau * d2) - 1) / d2 : tau;
+  out[0] = v[0] * da;
+  out[1] = v[1] * da;
+  out[2] = v[2] * da;
+}
+// Exponent of the drift-diffusion Green's function from a to b (src/method/Split_sample.cpp:91-110).
+__device__ inline double trans(const double ra[3], const double da[3], const double rb[3], double tau) {
+  double dr[3];
+  lim_drift(da, tau, dr);
+  double p = 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double t = rb[d] - ra[d] - dr[d];
+    p -= t * t;
+  }
+  return p / (2.0 * tau);
+}
+
+// ---------------------------------------------------------------- basis
+// Values of one spline shell at distance vector (x,y,z), r: writes nf functions x 5
+// components to out[c*ld + i] (S, P, 5D, 6D, 10F; src/basis_function/Cubic_spline_calc.cpp:353-700).
+template <typename T>
+__device__ inline void shell_lap(int sym, double f, double fd, double f2, double x, double y, double z, T* out, int ld) {
+  auto put = [&](int i, double v0, double v1, double v2, double v3, double v4) {
+    out[i] = (T)v0;
+    out[ld + i] = (T)v1;
+    out[2 * ld + i] = (T)v2;
+    out[3 * ld + i] = (T)v3;
+    out[4 * ld + i] = (T)v4;
+  };
+  double v, h, v2, v3;
+  switch (sym) {
+    case SK_S:
+      put(0, f, fd * x, fd * y, fd * z, f2 + 2. * fd);
+      break;
+    case SK_P: {
+      double rx = fd * x, ry = fd * y, rz = fd * z;
+      put(0, f * x, rx * x + f, rx * y, rx * z, f2 * x + 4. * rx);
+      put(1, f * y, ry * x, ry * y + f, ry * z, f2 * y + 4. * ry);
+      put(2, f * z, rz * x, rz * y, rz * z + f, f2 * z + 4. * rz);
+      break;
+    }
+    case SK_5D:
+      v = 2. * z * z - x * x - y * y; h = v * fd;
+      put(0, v * f, h * x - 2 * f * x, h * y - 2 * f * y, h * z + 4 * f * z, v * f2 + 6 * h);
+      v = x * z; h = v * fd;
+      put(1, v * f, h * x + f * z, h * y, h * z + f * x, v * f2 + 6. * h);
+      v = y * z; h = v * fd;
+      put(2, v * f, h * x, h * y + f * z, h * z + f * y, v * f2 + 6. * h);
+      v = x * x - y * y; h = v * fd;
+      put(3, v * f, h * x + 2. * f * x, h * y - 2. * f * y, h * z, v * f2 + 6. * h);
+      v = x * y; h = v * fd;
+      put(4, v * f, h * x + f * y, h * y + f * x, h * z, v * f2 + 6. * h);
+      break;
+    case SK_6D:
+      v = x * x; h = v * fd;
+      put(0, v * f, h * x + f * 2. * x, h * y, h * z, v * f2 + 6. * h + 2. * f);
+      v = y * y; h = v * fd;
+      put(1, v * f, h * x, h * y + f * 2. * y, h * z, v * f2 + 6. * h + 2. * f);
+      v = z * z; h = v * fd;
+      put(2, v * f, h * x, h * y, h * z + f * 2. * z, v * f2 + 6. * h + 2. * f);
+      v = x * y; h = v * fd;
+      put(3, v * f, h * x + f * y, h * y + f * x, h * z, v * f2 + 6. * h);
+      v = x * z; h = v * fd;
+      put(4, v * f, h * x + f * z, h * y, h * z + f * x, v * f2 + 6. * h);
+      v = y * z; h = v * fd;
+      put(5, v * f, h * x, h * y + f * z, h * z + f * y, v * f2 + 6. * h);
+      break;
+    default:  // SK_10F
+      v2 = x * x; v3 = v2 * x; h = v3 * fd;
+      put(0, v3 * f, h * x + f * 3. * v2, h * y, h * z, v3 * f2 + 8. * h + 6. * f * x);
+      v2 = y * y; v3 = v2 * y; h = v3 * fd;
+      put(1, v3 * f, h * x, h * y + 3. * f * v2, h * z, v3 * f2 + 8. * h + 6. * f * y);
+      v2 = z * z; v3 = v2 * z; h = v3 * fd;
+      put(2, v3 * f, h * x, h * y, h * z + 3. * f * v2, v3 * f2 + 8. * h + 6. * f * z);
+      v2 = x * x; v3 = v2 * y; h = v3 * fd;
+      put(3, v3 * f, h * x + 2. * f * x * y, h * y + f * v2, h * z, v3 * f2 + 8. * h + 2. * f * y);
+      v2 = x * x; v3 = v2 * z; h = v3 * fd;
+      put(4, v3 * f, h * x + 2. * f * x * z, h * y, h * z + f * v2, v3 * f2 + 8. * h + 2. * f * z);
+      v2 = y * y; v3 = v2 * x; h = v3 * fd;
+      put(5, v3 * f, h * x + v2 * f, h * y + 2. * f * x * y, h * z, v3 * f2 + 8. * h + 2. * f * x);
+      v2 = y * y; v3 = v2 * z; h = v3 * fd;
+      put(6, v3 * f, h * x, h * y + 2. * f * y * z, h * z + f * v2, v3 * f2 + 8. * h + 2. * f * z);
+      v2 = z * z; v3 = x * v2; h = v3 * fd;
+      put(7, v3 * f, h * x + f * v2, h * y, h * z + 2. * f * x * z, v3 * f2 + 8. * h + 2. * f * x);
+      v2 = z * z; v3 = v2 * y; h = v3 * fd;
+      put(8, v3 * f, h * x, h * y + f * v2, h * z + 2. * f * y * z, v3 * f2 + 8. * h + 2. * f * y);
+      v3 = x * y * z; v2 = f * z; h = v3 * fd;
+      put(9, v3 * f, h * x + v2 * y, h * y + v2 * x, h * z + f * x * y, v3 * f2 + 8. * h);
+      break;
+  }
+}
+
+// Value-only variant for the pseudopotential quadrature (src/basis_function/Cubic_spline_calc.cpp:25-140).
+template <typename T>
+__device__ inline void shell_val(int sym, double f, double x, double y, double z, T* out) {
+  switch (sym) {
+    case SK_S: out[0] = (T)f; break;
+    case SK_P: out[0] = (T)(f * x); out[1] = (T)(f * y); out[2] = (T)(f * z); break;
+    case SK_5D:
+      out[0] = (T)((2. * z * z - x * x - y * y) * f);
+      out[1] = (T)((x * z) * f);
+      out[2] = (T)((y * z) * f);
+      out[3] = (T)((x * x - y * y) * f);
+      out[4] = (T)((x * y) * f);
+      break;
+    case SK_6D:
+      out[0] = (T)((x * x) * f); out[1] = (T)((y * y) * f); out[2] = (T)((z * z) * f);
+      out[3] = (T)((x * y) * f); out[4] = (T)((x * z) * f); out[5] = (T)((y * z) * f);
+      break;
+    default:
+      out[0] = (T)((x * x * x) * f); out[1] = (T)((y * y * y) * f); out[2] = (T)((z * z * z) * f);
+      out[3] = (T)((x * x * y) * f); out[4] = (T)((x * x * z) * f); out[5] = (T)((y * y * x) * f);
+      out[6] = (T)((y * y * z) * f); out[7] = (T)((z * z * x) * f); out[8] = (T)((z * z * y) * f);
+      out[9] = (T)((x * y * z) * f);
+      break;
+  }
+}
+
+// Basis values of the shells [sh0, sh1) at point p, written to chi[c*ld + (f - f0)]
+// (c = value, d/dx, d/dy, d/dz, laplacian), zero beyond each shell's cutoff.
+// Lanes take shells round-robin.  Table indices are clamped to the table.
+template <typename T>
+__device__ inline void basis_lap(const DevSys& S, const double p[3], int sh0, int sh1, int f0, T* chi, int ld, int lane) {
+  for (int sh = sh0 + lane; sh < sh1; sh += kWave) {
+    const ShellDesc d = S.shells[sh];
+    const double* ap = S.atom_pos + 3 * d.atom;
+    double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
+    double r2 = x * x + y * y + z * z, r = sqrt(r2);
+    T* o = chi + (d.f0 - f0);
+    if (r < d.rcut && r < d.objthr) {
+      int iv = (int)(r * S.tab_inv[d.table]);
+      iv = iv < 0 ? 0 : (iv >= S.nint ? S.nint - 1 : iv);
+      const double* c = S.spline + ((size_t)d.table * S.nint + iv) * 4;
+      double h = r - iv * S.tab_sp[d.table];
+      double f = c[0] + h * (c[1] + h * (c[2] + h * c[3]));
+      double fd = (c[1] + h * (2 * c[2] + h * (3 * c[3]))) / r;
+      double f2 = 2 * c[2] + 6 * h * c[3];
+      shell_lap<T>(d.sym, f, fd, f2, x, y, z, o, ld);
+    } else {
+      for (int i = 0; i < d.nf; i++)
+        for (int c5 = 0; c5 < 5; c5++) o[c5 * ld + i] = (T)0;
+    }
+  }
+}
+
+template <typename T>
+__device__ inline void basis_val(const DevSys& S, const double p[3], int sh0, int sh1, int f0, T* chi, int lane) {
+  for (int sh = sh0 + lane; sh < sh1; sh += kWave) {
+    const ShellDesc d = S.shells[sh];
+    const double* ap = S.atom_pos + 3 * d.atom;
+    double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
+    double r = sqrt(x * x + y * y + z * z);
+    T* o = chi + (d.f0 - f0);
+    if (r < d.rcut && r < d.objthr) {
+      int iv = (int)(r * S.tab_inv[d.table]);
+      iv = iv < 0 ? 0 : (iv >= S.nint ? S.nint - 1 : iv);
+      const double* c = S.spline + ((size_t)d.table * S.nint + iv) * 4;
+      double h = r - iv * S.tab_sp[d.table];
+      double f = c[0] + h * (c[1] + h * (c[2] + h * c[3]));
+      shell_val<T>(d.sym, f, x, y, z, o);
+    } else {
+      for (int i = 0; i < d.nf; i++) o[i] = (T)0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- Jastrow radial objects
+// CUTOFF_CUSP (src/basis_function/Cutoff_cusp.cpp:122-154) and POLYPADE
+// (src/basis_function/Poly_pade_function.cpp:183-237), contracted with coefficients cf.
+// Accumulates value u, gradient factor gfac (gradient = gfac * (x,y,z)) and laplacian.
+__device__ inline void jobj_lap(const JObj& o, const double* cf, double r, double& u, double& gfac, double& lap) {
+  if (r >= o.rcut) return;
+  if (o.kind == 0) {
+    double zz = r * o.rcutinv, zz2 = zz * zz, pp = zz - zz2 + zz * zz2 / 3;
+    double pade = 1. / (1 + o.gamma * pp), pade2 = pade * pade;
+    double ppd = 1. - 2. * zz + zz2, ppdd = -2. + 2. * zz;
+    double dadr = ppd * pade2 / r;
+    double dadr2 = ppdd * pade2 * o.rcutinv - 2. * o.gamma * ppd * ppd * pade2 * pade * o.rcutinv + 2. * dadr;
+    double c = cf[0];
+    u += c * (o.cusp * o.rcut * (pp * pade - o.pade0));
+    gfac += c * (o.cusp * dadr);
+    lap += c * (o.cusp * dadr2);
+    return;
+  }
+  double zz = r / o.rcut, zz2 = zz * zz, zpp = zz2 * (6. - 8. * zz + 3. * zz2), zpp1 = 1 - zpp;
+  double zppd = 12. * zz * (1. - 2. * zz + zz2), zppdd2 = 6. - 24. * zz + 18 * zz2;
+  for (int i = 0; i < o.nf; i++) {
+    double c = cf[i];
+    if (c == 0.0) continue;
+    double b = o.beta[i];
+    double zpade = 1. / (1 + b * zpp), zpade2 = zpade * zpade;
+    double crsd = -zppd * zpade2 * (b + 1) / (r * o.rcut);
+    double crsdd2 = -zpade2 * (zppdd2 - b * zppd * zppd * zpade) * (b + 1) / (o.rcut * o.rcut);
+    u += c * (zpp1 * zpade);
+    gfac += c * crsd;
+    lap += c * (2 * (crsdd2 + crsd));
+  }
+}
+__device__ inline void jobj_val(const JObj& o, const double* cf, double r, double& u) {
+  if (r >= o.rcut) return;
+  if (o.kind == 0) {
+    double zz = r * o.rcutinv, zz2 = zz * zz, pp = zz - zz2 + zz * zz2 / 3;
+    double pade = 1. / (1 + o.gamma * pp);
+    u += cf[0] * (o.cusp * o.rcut * (pp * pade - o.pade0));
+    return;
+  }
+  double zz = r / o.rcut, zz2 = zz * zz, zpp = zz2 * (6. - 8. * zz + 3. * zz2), zpp1 = 1 - zpp;
+  for (int i = 0; i < o.nf; i++)
+    if (cf[i] != 0.0) u += cf[i] * (zpp1 / (1 + o.beta[i] * zpp));
+}
+
+// Jastrow terms involving electron e placed at p (others at pos): value, gradient and,
+// when lap != nullptr, laplacian with respect to r_e.  Wave-cooperative; result in all lanes.
+__device__ inline double jastrow_e(const DevSys& S, int e, const double p[3], const double* pos, double g[3],
+                                   double* lap, int lane) {
+  double u = 0, gx = 0, gy = 0, gz = 0, l = 0;
+  for (int a = lane; a < S.na; a += kWave) {
+    const double* ap = S.atom_pos + 3 * a;
+    double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
+    double r = sqrt(x * x + y * y + z * z);
+    double gf = 0;
+    for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_lap(S.ei[o], S.ei[o].cl, r, u, gf, l);
+    gx += gf * x;
+    gy += gf * y;
+    gz += gf * z;
+  }
+  bool eup = e < S.nup;
+  for (int j = lane; j < S.ne; j += kWave) {
+    if (j == e) continue;
+    double x = p[0] - pos[3 * j], y = p[1] - pos[3 * j + 1], z = p[2] - pos[3 * j + 2];
+    double r = sqrt(x * x + y * y + z * z);
+    bool unlike = (j < S.nup) != eup;
+    double gf = 0;
+    for (int o = 0; o < S.nee; o++) jobj_lap(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, u, gf, l);
+    gx += gf * x;
+    gy += gf * y;
+    gz += gf * z;
+  }
+  u = wsum(u);
+  g[0] = wsum(gx);
+  g[1] = wsum(gy);
+  g[2] = wsum(gz);
+  if (lap) *lap = wsum(l);
+  return u;
+}
+
+// Value-only Jastrow terms of electron e at p (pseudopotential quadrature).
+__device__ inline double jastrow_e_val(const DevSys& S, int e, const double p[3], const double* pos, int lane) {
+  double u = 0;
+  for (int a = lane; a < S.na; a += kWave) {
+    const double* ap = S.atom_pos + 3 * a;
+    double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
+    double r = sqrt(x * x + y * y + z * z);
+    for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_val(S.ei[o], S.ei[o].cl, r, u);
+  }
+  bool eup = e < S.nup;
+  for (int j = lane; j < S.ne; j += kWave) {
+    if (j == e) continue;
+    double x = p[0] - pos[3 * j], y = p[1] - pos[3 * j + 1], z = p[2] - pos[3 * j + 2];
+    double r = sqrt(x * x + y * y + z * z);
+    bool unlike = (j < S.nup) != eup;
+    for (int o = 0; o < S.nee; o++) jobj_val(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, u);
+  }
+  return wsum(u);
+}
+
+// Semilocal pseudopotential channel values v_l(r) (src/basis_function/Rgaussian_function.cpp:180-202,
+// src/system/Pseudopotential.cpp:305-318).
+__device__ inline void pp_radial(const PPType& P, double Z, double r, double* vl) {
+  for (int l = 0; l < P.L; l++) {
+    double v = 0;
+    for (int j = 0; j < P.nterm[l]; j++) {
+      double ex = P.a[l][j] * (r * r);
+      ex = ex < 60.0 ? ex : 60.0;
+      v += P.c[l][j] * pow(r, P.n[l][j]) * exp(-ex);
+    }
+    vl[l] = v;
+  }
+  if (P.addzeff) vl[P.L - 1] += Z / r;
+}
+
+__device__ inline double legendre(double x, int n) {
+  switch (n) {
+    case 0: return 1;
+    case 1: return x;
+    case 2: return .5 * (3 * x * x - 1);
+    case 3: return .5 * (5 * x * x * x - 3 * x);
+    case 4: return 0.125 * (35 * x * x * x * x - 30 * x * x + 3);
+    default: return 0.125 * (63 * x * x * x * x * x - 70 * x * x * x + 15 * x);
+  }
+}
+
+}  // namespace qw

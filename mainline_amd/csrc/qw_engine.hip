@@ -1,0 +1,668 @@
+// Walker engine: device memory, kernel launches, VMC/DMC drivers and the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/qwalk_amd.h"
+#include "qw_kernels.hpp"
+#include "qw_mixed.hpp"
+#include "qw_setup.hpp"
+
+using namespace qw;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevErr : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) throw DevErr(std::string(#x) + ": " + hipGetErrorString(e_));            \
+  } while (0)
+
+constexpr int kChunkNF = 640;  // basis functions per LDS tile
+
+int pick_ns(int n) {
+  int ns = 4;
+  while (ns < n) ns <<= 1;
+  return ns;
+}
+
+}  // namespace
+
+struct qw_engine {
+  std::string err;
+  int device = 0, precision = 0, refresh = 32;
+  unsigned long long seed = 1;
+  SysTables T;
+  DevSys S{};
+  DevWalkers W{};
+  int cap = 0, nw = 0, ns = 4;
+  hipStream_t stream = nullptr;
+  unsigned step = 0;
+  std::vector<void*> sys_allocs, walker_allocs;
+  DmcCtl* dctl = nullptr;
+  unsigned long long* keys = nullptr;
+  int *vals = nullptr, *psrc = nullptr, *pdst = nullptr;
+  double* dred = nullptr;
+  int sort_cap = 0;
+  bool mixed_ready = false;
+  bool dmc_ready = false;  // DMC control state valid for the resident walkers
+  DmcCtl ctl_host{};
+  MixedTables MT{};
+  // per-kernel timing with events on the engine stream
+  bool timing = true;
+  struct Ev {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Ev> evs;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> knames;
+  std::vector<double> ktime;
+  std::vector<int> kcount;
+
+  ~qw_engine() {
+    hipSetDevice(device);
+    if (stream) hipStreamSynchronize(stream);
+    for (void* p : sys_allocs) hipFree(p);
+    for (void* p : walker_allocs) hipFree(p);
+    for (auto& e : evs) pool.push_back(e.a), pool.push_back(e.b);
+    for (auto ev : pool) hipEventDestroy(ev);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  T* upload(const std::vector<T>& v) {
+    void* p = nullptr;
+    size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
+    HIPCHK(hipMalloc(&p, n));
+    sys_allocs.push_back(p);
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return (T*)p;
+  }
+  template <typename T>
+  T* walloc(size_t n) {
+    void* p = nullptr;
+    HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    walker_allocs.push_back(p);
+    HIPCHK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return (T*)p;
+  }
+
+  int kind_of(const char* name) {
+    for (size_t i = 0; i < knames.size(); i++)
+      if (knames[i] == name) return (int)i;
+    knames.push_back(name);
+    ktime.push_back(0);
+    kcount.push_back(0);
+    return (int)knames.size() - 1;
+  }
+  hipEvent_t ev() {
+    if (pool.empty()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      return e;
+    }
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  // Launch wrapper: records start/stop events around the launch on the engine stream.
+  template <typename F>
+  void launch(const char* name, F f) {
+    if (!timing) {
+      f();
+      HIPCHK(hipGetLastError());
+      return;
+    }
+    Ev e{kind_of(name), ev(), ev()};
+    HIPCHK(hipEventRecord(e.a, stream));
+    f();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e.b, stream));
+    evs.push_back(e);
+    if (evs.size() > 4096) drain_events();
+  }
+  void drain_events() {
+    HIPCHK(hipStreamSynchronize(stream));
+    for (auto& e : evs) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+      ktime[e.kind] += ms;
+      kcount[e.kind] += 1;
+      pool.push_back(e.a);
+      pool.push_back(e.b);
+    }
+    evs.clear();
+  }
+  void reset_times() {
+    drain_events();
+    std::fill(ktime.begin(), ktime.end(), 0.0);
+    std::fill(kcount.begin(), kcount.end(), 0);
+  }
+
+  void build_device_system() {
+    S.nup = T.nup;
+    S.ndn = T.ndn;
+    S.ne = T.ne();
+    S.na = T.na;
+    S.nb = T.nb;
+    S.nsh = (int)T.shells.size();
+    S.ntab = T.ntab;
+    S.nint = T.nint;
+    S.nee = (int)T.ee.size();
+    S.has_pp = T.pptypes.empty() ? 0 : 1;
+    S.nsp = std::max(T.nup, T.ndn);
+    S.vii = T.vii;
+    // shell chunks whose functions fit one LDS tile
+    std::vector<int> ch{0};
+    int acc = 0, maxnf = 0;
+    for (size_t i = 0; i < T.shells.size(); i++) {
+      if (acc + T.shells[i].nf > kChunkNF) {
+        ch.push_back((int)i);
+        maxnf = std::max(maxnf, acc);
+        acc = 0;
+      }
+      acc += T.shells[i].nf;
+    }
+    maxnf = std::max(maxnf, acc);
+    ch.push_back((int)T.shells.size());
+    S.nchunk = (int)ch.size() - 1;
+    S.chunk_nf = maxnf;
+    S.atom_pos = upload(T.atom_pos);
+    S.atom_Z = upload(T.atom_Z);
+    S.shells = upload(T.shells);
+    S.chunk_sh0 = upload(ch);
+    S.spline = upload(T.spline);
+    S.tab_inv = upload(T.tab_invspacing);
+    S.tab_sp = upload(T.tab_spacing);
+    for (int s = 0; s < 2; s++) {
+      S.C[s] = upload(T.C[s]);
+      std::vector<float> cf(T.C[s].begin(), T.C[s].end());
+      S.Cf[s] = upload(cf);
+    }
+    S.ee = upload(T.ee);
+    S.ei = upload(T.ei);
+    S.atom_ei0 = upload(T.atom_ei0);
+    S.pp = upload(T.pptypes);
+    S.atom_pp = upload(T.atom_pp);
+    dctl = upload(std::vector<DmcCtl>(1));
+    dred = upload(std::vector<double>(16));
+  }
+
+  void ensure_walkers(int n) {
+    if (n <= cap) {
+      W.nw = n;
+      nw = n;
+      return;
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    for (void* p : walker_allocs) hipFree(p);
+    walker_allocs.clear();
+    const size_t ne = S.ne, nsp = S.nsp;
+    W.pos = walloc<double>((size_t)n * ne * 3);
+    if (precision == QW_PREC_FP64) {
+      W.inv = walloc<double>((size_t)n * 2 * nsp * nsp);
+      W.mo = walloc<double>((size_t)n * ne * 5 * nsp);
+      W.invf = nullptr;
+      W.mof = nullptr;
+    } else {
+      // the FP64 MO value matrix is kept for the periodic FP64 LU recompute
+      W.mo = walloc<double>((size_t)n * ne * 5 * nsp);
+      W.inv = walloc<double>((size_t)n * 2 * nsp * nsp);
+      W.invf = walloc<float>((size_t)n * 2 * nsp * nsp);
+      W.mof = walloc<float>((size_t)n * ne * 5 * nsp);
+    }
+    W.logdet = walloc<double>((size_t)n * 2);
+    W.sgn = walloc<double>((size_t)n * 2);
+    W.weight = walloc<double>(n);
+    W.eold = walloc<double>(n);
+    W.elast = walloc<double>((size_t)n * 4);
+    W.acc = walloc<int>(n);
+    W.bsum = walloc<double>((size_t)n * 6);
+    int P = 1;
+    while (P < n) P <<= 1;
+    keys = walloc<unsigned long long>(P);
+    vals = walloc<int>(P);
+    psrc = walloc<int>(n / 2 + 1);
+    pdst = walloc<int>(n / 2 + 1);
+    cap = n;
+    W.nw = n;
+    nw = n;
+  }
+
+  // ---- kernels
+  size_t lds_rows() const { return 5 * (size_t)S.chunk_nf * sizeof(double); }
+  size_t lds_sweep() const { return (3 * (size_t)S.ne + 5 * (size_t)S.chunk_nf + 2 * (size_t)ns) * sizeof(double); }
+  size_t lds_local() const { return (7 * (size_t)S.ne + 5 * (size_t)S.chunk_nf) * sizeof(double); }
+
+  void run_mo_rows() {
+    launch("k_mo_rows", [&] {
+      hipLaunchKernelGGL(k_mo_rows, dim3((unsigned)((size_t)nw * S.ne)), dim3(64), lds_rows(), stream, S, W, nw);
+    });
+  }
+  template <typename TI>
+  void run_lu_t(TI* out) {
+    auto go = [&](auto kern) {
+      launch("k_lu", [&] {
+        hipLaunchKernelGGL(kern, dim3(2 * nw), dim3(64), 0, stream, S, (const double*)W.mo, out, W.logdet, W.sgn, nw);
+      });
+    };
+    switch (ns) {
+      case 4: go(k_lu<4, TI>); break;
+      case 8: go(k_lu<8, TI>); break;
+      case 16: go(k_lu<16, TI>); break;
+      case 32: go(k_lu<32, TI>); break;
+      default: go(k_lu<64, TI>); break;
+    }
+  }
+  void refresh_all() {
+    if (precision == QW_PREC_FP64) {
+      run_mo_rows();
+      run_lu_t<double>(W.inv);
+    } else {
+      mixed_refresh_all(*this);
+    }
+  }
+  void run_sweep(int s, int k0, int k1, double tau, unsigned sweep, int depth, int restrict_nodes) {
+    SweepArgs A{s, k0, k1, tau, seed, step, sweep, restrict_nodes};
+    if (precision != QW_PREC_FP64) {
+      mixed_sweep(*this, A, depth);
+      return;
+    }
+    auto go = [&](auto kern) {
+      launch("k_sweep", [&] { hipLaunchKernelGGL(kern, dim3(nw), dim3(64), lds_sweep(), stream, S, W, A); });
+    };
+    if (depth == 1) {
+      switch (ns) {
+        case 4: go(k_sweep<4, 1>); break;
+        case 8: go(k_sweep<8, 1>); break;
+        case 16: go(k_sweep<16, 1>); break;
+        case 32: go(k_sweep<32, 1>); break;
+        default: go(k_sweep<64, 1>); break;
+      }
+    } else {
+      switch (ns) {
+        case 4: go(k_sweep<4, 2>); break;
+        case 8: go(k_sweep<8, 2>); break;
+        case 16: go(k_sweep<16, 2>); break;
+        case 32: go(k_sweep<32, 2>); break;
+        default: go(k_sweep<64, 2>); break;
+      }
+    }
+  }
+  void run_local(const LocalArgs& A) {
+    if (precision != QW_PREC_FP64) {
+      mixed_local(*this, A);
+      return;
+    }
+    launch("k_local", [&] { hipLaunchKernelGGL(k_local, dim3(nw), dim3(64), lds_local(), stream, S, W, A); });
+  }
+  void reduce(double* host7, int reset) {
+    launch("k_reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, stream, W, dred, reset); });
+    HIPCHK(hipMemcpyAsync(host7, dred, 7 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+};
+
+#include "qw_mixed_host.hpp"
+
+// ====================================================================== C ABI
+namespace {
+template <typename F>
+qw_status guarded(qw_engine* e, F f) {
+  try {
+    if (e) HIPCHK(hipSetDevice(e->device));
+    return f();
+  } catch (DevErr& x) {
+    if (e) e->err = x.what();
+    return QW_ERR_DEVICE;
+  } catch (std::bad_alloc&) {
+    if (e) e->err = "host allocation failed";
+    return QW_ERR_NOMEM;
+  } catch (std::exception& x) {
+    if (e) e->err = x.what();
+    return QW_ERR_INPUT;
+  }
+}
+}  // namespace
+
+extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opts* opts, qw_engine** out) {
+  if (!out || !deck_path) {
+    g_create_error = "null argument";
+    return QW_ERR_ARG;
+  }
+  *out = nullptr;
+  std::unique_ptr<qw_engine> e(new qw_engine);
+  if (opts) {
+    e->precision = opts->precision;
+    e->device = opts->device;
+    e->seed = opts->seed;
+    if (opts->refresh_interval > 0) e->refresh = opts->refresh_interval;
+  }
+  if (e->precision < 0 || e->precision > 2) {
+    g_create_error = "unknown precision";
+    return QW_ERR_ARG;
+  }
+  bool unsup = false;
+  try {
+    e->T = read_deck(deck_path, &unsup);
+  } catch (std::exception& x) {
+    g_create_error = x.what();
+    return unsup ? QW_ERR_UNSUPPORTED : QW_ERR_INPUT;
+  }
+  if (std::max(e->T.nup, e->T.ndn) > 64) {
+    g_create_error = "more than 64 electrons of one spin are not supported by this engine build";
+    return QW_ERR_UNSUPPORTED;
+  }
+  try {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (e->device < 0 || e->device >= ndev) throw DevErr("no HIP device " + std::to_string(e->device));
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamCreate(&e->stream));
+    e->ns = pick_ns(std::max(e->T.nup, e->T.ndn));
+    e->build_device_system();
+    if (e->precision != QW_PREC_FP64) mixed_setup(*e);
+  } catch (std::exception& x) {
+    g_create_error = x.what();
+    return QW_ERR_DEVICE;
+  }
+  *out = e.release();
+  return QW_OK;
+}
+
+extern "C" void qw_engine_destroy(qw_engine* e) { delete e; }
+
+extern "C" const char* qw_last_error(const qw_engine* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+extern "C" qw_status qw_engine_info(const qw_engine* e, int* out6) {
+  if (!e || !out6) return QW_ERR_ARG;
+  out6[0] = e->T.nup;
+  out6[1] = e->T.ndn;
+  out6[2] = e->T.na;
+  out6[3] = e->T.nb;
+  out6[4] = e->T.pptypes.empty() ? 0 : 1;
+  out6[5] = e->cap;
+  return QW_OK;
+}
+
+extern "C" qw_status qw_walkers_set(qw_engine* e, int n, const double* pos, const double* weights) {
+  if (!e || !pos || n <= 0) return QW_ERR_ARG;
+  return guarded(e, [&] {
+    e->ensure_walkers(n);
+    const size_t ne = e->S.ne;
+    HIPCHK(hipMemcpyAsync(e->W.pos, pos, (size_t)n * ne * 3 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    std::vector<double> w(n, 1.0);
+    if (weights) std::copy(weights, weights + n, w.begin());
+    HIPCHK(hipMemcpyAsync(e->W.weight, w.data(), n * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)n * 6 * sizeof(double), e->stream));
+    HIPCHK(hipMemsetAsync(e->W.acc, 0, (size_t)n * sizeof(int), e->stream));
+    e->refresh_all();
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->dmc_ready = false;
+    return QW_OK;
+  });
+}
+
+extern "C" qw_status qw_walkers_get(qw_engine* e, int n, double* pos, double* weights) {
+  if (!e || n <= 0 || n > e->nw) return QW_ERR_ARG;
+  return guarded(e, [&] {
+    const size_t ne = e->S.ne;
+    if (pos) HIPCHK(hipMemcpyAsync(pos, e->W.pos, (size_t)n * ne * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    if (weights) HIPCHK(hipMemcpyAsync(weights, e->W.weight, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return QW_OK;
+  });
+}
+
+extern "C" qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const double* rot, int pp_mode,
+                                   const double* pp_rand, double* logpsi, double* sign, double* grad, double* lap,
+                                   double* eparts) {
+  if (!e || K <= 0 || !pos || !logpsi || !sign || !grad || !lap || !eparts) return QW_ERR_ARG;
+  if (pp_mode == 1 && !pp_rand && e->S.has_pp) return QW_ERR_ARG;
+  return guarded(e, [&] {
+    e->ensure_walkers(K);
+    e->dmc_ready = false;
+    const size_t ne = e->S.ne, na = e->S.na;
+    HIPCHK(hipMemcpyAsync(e->W.pos, pos, (size_t)K * ne * 3 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    e->refresh_all();
+    std::vector<void*> tmp;
+    auto dalloc = [&](size_t bytes) {
+      void* p = nullptr;
+      HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+      tmp.push_back(p);
+      return p;
+    };
+    struct Free {
+      std::vector<void*>& v;
+      ~Free() {
+        for (void* p : v) hipFree(p);
+      }
+    } fr{tmp};
+    double* d_rot = nullptr;
+    double* d_rand = nullptr;
+    if (rot) {
+      d_rot = (double*)dalloc(9 * sizeof(double));
+      HIPCHK(hipMemcpyAsync(d_rot, rot, 9 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    }
+    if (pp_mode == 1 && pp_rand) {
+      d_rand = (double*)dalloc((size_t)K * na * ne * sizeof(double));
+      HIPCHK(hipMemcpyAsync(d_rand, pp_rand, (size_t)K * na * ne * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    }
+    double* d_lp = (double*)dalloc(K * sizeof(double));
+    double* d_sg = (double*)dalloc(K * sizeof(double));
+    double* d_gr = (double*)dalloc((size_t)K * ne * 3 * sizeof(double));
+    double* d_la = (double*)dalloc((size_t)K * ne * sizeof(double));
+    double* d_ep = (double*)dalloc((size_t)K * 3 * sizeof(double));
+    LocalArgs A{};
+    A.mode = 0;
+    A.pp_mode = pp_mode;
+    A.rot = d_rot;
+    A.pp_rand = d_rand;
+    A.seed = e->seed;
+    A.step = e->step;
+    A.out_logpsi = d_lp;
+    A.out_sign = d_sg;
+    A.out_grad = d_gr;
+    A.out_lap = d_la;
+    A.out_eparts = d_ep;
+    A.ctl = e->dctl;
+    e->run_local(A);
+    HIPCHK(hipMemcpyAsync(logpsi, d_lp, K * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(sign, d_sg, K * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(grad, d_gr, (size_t)K * ne * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(lap, d_la, (size_t)K * ne * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(eparts, d_ep, (size_t)K * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return QW_OK;
+  });
+}
+
+extern "C" qw_status qw_method_read(const char* deck_path, qw_method_desc* out) {
+  if (!deck_path || !out) return QW_ERR_ARG;
+  try {
+    MethodDesc d = read_method(deck_path);
+    out->method = d.method;
+    out->nconfig = d.nconfig;
+    out->nblock = d.nblock;
+    out->nstep = d.nstep;
+    out->ndecorr = d.ndecorr;
+    out->timestep = d.timestep;
+    out->feedback_interval = d.feedback_interval;
+    out->feedback = d.feedback;
+    out->max_weight = d.max_weight;
+    out->branch_start_cutoff = d.branch_start_cutoff;
+    out->pp_stochastic = d.pp_stochastic;
+    return QW_OK;
+  } catch (std::exception& x) {
+    g_create_error = x.what();
+    return QW_ERR_INPUT;
+  }
+}
+
+// VMC: src/method/Vmc_method.cpp:351-558.  DMC: src/method/Dmc_method.cpp:328-600,776-1156.
+extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* eblock, double* stats) {
+  if (!e || !m || e->nw <= 0) return QW_ERR_ARG;
+  if (m->timestep <= 0 || m->nblock < 0 || m->nstep < 1) {
+    e->err = "qw_run needs timestep > 0, nblock >= 0, nstep >= 1";
+    return QW_ERR_ARG;
+  }
+  return guarded(e, [&] {
+    e->reset_times();
+    const bool dmc = m->method == 1;
+    const int nw = e->nw, ne = e->S.ne;
+    const double tau = m->timestep;
+    auto t0 = std::chrono::steady_clock::now();
+    long long moves = 0;
+    double accepted_total = 0;
+    LocalArgs A{};
+    A.pp_mode = m->pp_stochastic ? 1 : 0;
+    A.seed = e->seed;
+    A.ctl = e->dctl;
+    HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)nw * 6 * sizeof(double), e->stream));
+    HIPCHK(hipMemsetAsync(e->W.acc, 0, (size_t)nw * sizeof(int), e->stream));
+    DmcCtl ctl{};
+    double red[7];
+    if (dmc && e->dmc_ready) {
+      // continuing a DMC run on the same population: keep E_ref, E_T, cutoffs, tau_eff
+      ctl = e->ctl_host;
+      ctl.tau = tau;
+      ctl.feedback = m->feedback;
+      ctl.max_weight = m->max_weight;
+      HIPCHK(hipMemcpyAsync(e->dctl, &ctl, sizeof(DmcCtl), hipMemcpyHostToDevice, e->stream));
+    } else if (dmc) {
+      // initial local energies, reference energy and branching cutoffs
+      A.mode = 3;
+      A.step = e->step;
+      e->run_local(A);
+      std::vector<double> el((size_t)nw * 4), w(nw);
+      HIPCHK(hipMemcpyAsync(el.data(), e->W.elast, el.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipMemcpyAsync(w.data(), e->W.weight, nw * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      double ea = 0, tw = 0;
+      for (int i = 0; i < nw; i++) ea += el[4 * i + 3] * w[i], tw += w[i];
+      ea /= tw;
+      double v = 0;
+      for (int i = 0; i < nw; i++) v += (el[4 * i + 3] - ea) * (el[4 * i + 3] - ea);
+      double sig = std::sqrt(v / nw);
+      ctl.eref = ea;
+      ctl.cut_start = m->branch_start_cutoff * sig;
+      ctl.cut_stop = ctl.cut_start * 1.5;
+      ctl.feedback = m->feedback;
+      ctl.max_weight = m->max_weight;
+      ctl.tau = tau;
+      ctl.teff = tau;
+      ctl.etrial = ea - m->feedback * std::log(tw / nw);
+      HIPCHK(hipMemcpyAsync(e->dctl, &ctl, sizeof(DmcCtl), hipMemcpyHostToDevice, e->stream));
+    }
+    e->step++;
+    const int fbi = std::max(1, m->feedback_interval);
+    std::vector<double> bavg;
+    for (int b = 0; b < m->nblock; b++) {
+      if (!dmc) e->refresh_all();
+      for (int st = 0; st < m->nstep;) {
+        const int np = dmc ? std::min(fbi, m->nstep - st) : 1;
+        for (int p = 0; p < np; p++) {
+          const int nsw = dmc ? 1 : std::max(1, m->ndecorr);
+          for (int sw = 0; sw < nsw; sw++) {
+            for (int s = 0; s < 2; s++) {
+              int n = s ? e->T.ndn : e->T.nup;
+              if (n == 0) continue;
+              e->run_sweep(s, 0, n, tau, (unsigned)sw, dmc ? 1 : 2, dmc ? 1 : 0);
+            }
+          }
+          moves += (long long)nw * ne * nsw;
+          A.mode = dmc ? 2 : 1;
+          A.step = e->step;
+          e->run_local(A);
+          e->step++;
+        }
+        st += np;
+        if (dmc) {
+          e->launch("k_branch", [&] {
+            hipLaunchKernelGGL(k_branch, dim3(1), dim3(1024), 0, e->stream, e->W, e->dctl, e->keys, e->vals, e->psrc,
+                               e->pdst, nw * ne * np, 1, e->seed, e->step);
+          });
+          e->launch("k_copy", [&] {
+            hipLaunchKernelGGL(k_copy, dim3(nw / 2 + 1), dim3(256), 0, e->stream, e->S, e->W, (const DmcCtl*)e->dctl,
+                               (const int*)e->psrc, (const int*)e->pdst);
+          });
+        } else {
+          // acceptance bookkeeping for VMC
+        }
+      }
+      e->reduce(red, 1);
+      double sw = red[0];
+      if (eblock) {
+        eblock[5 * b + 0] = red[1] / sw;
+        eblock[5 * b + 1] = red[2] / sw;
+        eblock[5 * b + 2] = red[3] / sw;
+        eblock[5 * b + 3] = red[4] / sw;
+        eblock[5 * b + 4] = sw;
+      }
+      bavg.push_back(red[1] / sw);
+      if (dmc) {
+        // E_ref <- average over the blocks so far, then the trial energy from the population
+        double s = 0;
+        for (double x : bavg) s += x;
+        HIPCHK(hipMemcpyAsync(&ctl, e->dctl, sizeof(DmcCtl), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        ctl.eref = s / bavg.size();
+        ctl.etrial = ctl.eref - m->feedback * std::log(red[6] / nw);
+        HIPCHK(hipMemcpyAsync(e->dctl, &ctl, sizeof(DmcCtl), hipMemcpyHostToDevice, e->stream));
+      }
+    }
+    if (dmc) {
+      HIPCHK(hipMemcpyAsync(&ctl, e->dctl, sizeof(DmcCtl), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+      e->ctl_host = ctl;
+      e->dmc_ready = true;
+    }
+    // acceptance: sum of per-walker accepted counters (DMC resets them at each branch)
+    std::vector<int> acc(nw);
+    HIPCHK(hipMemcpyAsync(acc.data(), e->W.acc, nw * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int a : acc) accepted_total += a;
+    e->drain_events();
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (stats) {
+      stats[0] = dmc ? ctl.teff / tau : (moves ? accepted_total / (double)moves : 0.0);
+      stats[1] = (double)moves;
+      stats[2] = secs;
+      stats[3] = secs > 0 ? moves / secs : 0.0;
+    }
+    return QW_OK;
+  });
+}
+
+extern "C" qw_status qw_kernel_times(const qw_engine* e, char* buf, int buflen, double* avg_ms, int* count, int n,
+                                     int* nk) {
+  if (!e) return QW_ERR_ARG;
+  std::string names;
+  int k = (int)e->knames.size();
+  for (int i = 0; i < k; i++) {
+    if (i) names += ",";
+    names += e->knames[i];
+    if (i < n) {
+      if (avg_ms) avg_ms[i] = e->kcount[i] ? e->ktime[i] / e->kcount[i] : 0.0;
+      if (count) count[i] = e->kcount[i];
+    }
+  }
+  if (buf && buflen > 0) snprintf(buf, buflen, "%s", names.c_str());
+  if (nk) *nk = k;
+  return QW_OK;
+}

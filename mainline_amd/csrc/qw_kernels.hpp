@@ -1,0 +1,725 @@
+// Walker-engine kernels (FP64 path).  One 64-lane wavefront = one walker per block.
+//
+//   k_mo_rows    basis + LCAO for every electron of every walker (walker refresh, fixed-config
+//                evaluation); src/orbitals/MO_matrix_cutoff.h:480-560
+//   k_lu<NS>     pivoted LU, log|det|, sign and transposed inverse per walker and spin;
+//                src/program_utils/MatrixAlgebrac.cpp:108-224,316-421
+//   k_sweep<NS,D> single-electron moves of one spin channel: drift, proposal, ratio, Jastrow,
+//                acceptance, Sherman-Morrison update with the inverse held in registers;
+//                src/method/Split_sample.cpp:302-540, src/program_utils/MatrixAlgebrac.cpp:766-838
+//   k_local      local energy (kinetic, Coulomb, nonlocal pseudopotential), DMC weight, block
+//                sums; src/system/System.cpp:53-140, Molecular_system.cpp:209-284,
+//                Pseudopotential.cpp:596-815, src/method/Dmc_method.cpp:801-824
+//   k_branch     trial energy, effective timestep, fixed-population pair branching;
+//                src/method/Dmc_method.cpp:776-797,961-995
+//   k_copy       walker state copies decided by k_branch
+#pragma once
+#include "qw_device.hpp"
+
+namespace qw {
+
+// Scalars of the DMC control loop kept on the device (read by k_local, written by k_branch).
+struct DmcCtl {
+  double etrial, teff, cut_start, cut_stop, max_weight, eref, feedback, tau;
+  double sumw;     // last weight sum
+  int npairs;
+  int pad;
+};
+
+// ---------------------------------------------------------------- LCAO of one point
+// Lane j < n accumulates MO j: phi[c] = sum_b C[b][j] chi[c][b] over all basis chunks.
+// chi is an LDS tile of 5 x S.chunk_nf doubles.  Block-wide barriers separate the tile
+// producer (basis_lap) from the consumer.
+__device__ inline void lcao_lap(const DevSys& S, int s, const double p[3], double* chi, double phi[5], int lane) {
+  const int n = s ? S.ndn : S.nup, ld = S.chunk_nf;
+  for (int c = 0; c < 5; c++) phi[c] = 0;
+  for (int ch = 0; ch < S.nchunk; ch++) {
+    int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
+    int f0 = S.shells[sh0].f0, f1 = sh1 < S.nsh ? S.shells[sh1].f0 : S.nb;
+    __syncthreads();
+    basis_lap<double>(S, p, sh0, sh1, f0, chi, ld, lane);
+    __syncthreads();
+    if (lane < n) {
+      const double* Cp = S.C[s] + (size_t)f0 * n + lane;
+      double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+      int nf = f1 - f0;
+#pragma unroll 4
+      for (int b = 0; b < nf; b++) {
+        double c = Cp[(size_t)b * n];
+        a0 += c * chi[b];
+        a1 += c * chi[ld + b];
+        a2 += c * chi[2 * ld + b];
+        a3 += c * chi[3 * ld + b];
+        a4 += c * chi[4 * ld + b];
+      }
+      phi[0] += a0;
+      phi[1] += a1;
+      phi[2] += a2;
+      phi[3] += a3;
+      phi[4] += a4;
+    }
+  }
+}
+
+__device__ inline double lcao_val(const DevSys& S, int s, const double p[3], double* chi, int lane) {
+  const int n = s ? S.ndn : S.nup;
+  double phi = 0;
+  for (int ch = 0; ch < S.nchunk; ch++) {
+    int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
+    int f0 = S.shells[sh0].f0, f1 = sh1 < S.nsh ? S.shells[sh1].f0 : S.nb;
+    __syncthreads();
+    basis_val<double>(S, p, sh0, sh1, f0, chi, lane);
+    __syncthreads();
+    if (lane < n) {
+      const double* Cp = S.C[s] + (size_t)f0 * n + lane;
+      double a = 0;
+      int nf = f1 - f0;
+#pragma unroll 4
+      for (int b = 0; b < nf; b++) a += Cp[(size_t)b * n] * chi[b];
+      phi += a;
+    }
+  }
+  return phi;
+}
+
+// ---------------------------------------------------------------- k_mo_rows
+__global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x;
+  const long blk = blockIdx.x;
+  const int w = (int)(blk / S.ne), e = (int)(blk % S.ne);
+  if (w >= nw) return;
+  const int s = e < S.nup ? 0 : 1, n = s ? S.ndn : S.nup;
+  double p[3];
+  for (int d = 0; d < 3; d++) p[d] = W.pos[((size_t)w * S.ne + e) * 3 + d];
+  double phi[5];
+  lcao_lap(S, s, p, lds, phi, lane);
+  if (lane < n) {
+    double* row = W.mo + ((size_t)w * S.ne + e) * 5 * S.nsp;
+    for (int c = 0; c < 5; c++) row[c * S.nsp + lane] = phi[c];
+  }
+}
+
+// ---------------------------------------------------------------- k_lu
+// Right-looking LU with partial pivoting and implicit row scaling (the same pivot choice as
+// the reference's Crout ludcmp: largest vv[i]*|a[i][j]|, last index on ties), log|det| and
+// sign, then the transposed inverse: lane j solves A x = e_j and stores x as row j.
+template <int NS, typename TI>
+__global__ __launch_bounds__(64) void k_lu(DevSys S, const double* mo, TI* inv_out, double* logdet_out,
+                                           double* sgn_out, int nw) {
+  __shared__ double a[NS * NS];
+  __shared__ double vv[NS];
+  __shared__ int indx[NS], perm[NS];
+  __shared__ int singular;
+  const int lane = threadIdx.x;
+  const int w = blockIdx.x >> 1, s = blockIdx.x & 1;
+  if (w >= nw) return;
+  const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
+  TI* out = inv_out + ((size_t)w * 2 + s) * nsp * nsp;
+  if (n == 0) {
+    if (lane == 0) {
+      logdet_out[2 * w + s] = 0;
+      sgn_out[2 * w + s] = 1;
+    }
+    return;
+  }
+  for (int k = 0; k < n; k++)
+    if (lane < n) a[k * NS + lane] = mo[((size_t)w * S.ne + off + k) * 5 * nsp + lane];
+  if (lane == 0) singular = 0;
+  __syncthreads();
+  if (lane < n) {
+    double big = 0;
+    for (int j = 0; j < n; j++) big = fmax(big, fabs(a[lane * NS + j]));
+    if (big == 0.0) singular = 1;
+    vv[lane] = big == 0.0 ? 0.0 : 1.0 / big;
+  }
+  __syncthreads();
+  double sign = 1;
+  for (int j = 0; j < n && !singular; j++) {
+    double dum = (lane >= j && lane < n) ? vv[lane] * fabs(a[lane * NS + j]) : -1.0;
+    int idx = lane;
+    for (int o = 32; o > 0; o >>= 1) {
+      double od = __shfl_xor(dum, o);
+      int oi = __shfl_xor(idx, o);
+      if (od > dum || (od == dum && oi > idx)) {
+        dum = od;
+        idx = oi;
+      }
+    }
+    const int imax = idx;
+    if (imax != j) {
+      for (int k = lane; k < n; k += kWave) {
+        double t = a[imax * NS + k];
+        a[imax * NS + k] = a[j * NS + k];
+        a[j * NS + k] = t;
+      }
+      sign = -sign;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      if (imax != j) vv[imax] = vv[j];
+      indx[j] = imax;
+      if (a[j * NS + j] == 0.0) singular = 1;
+    }
+    __syncthreads();
+    if (singular) break;
+    double inv_p = 1.0 / a[j * NS + j];
+    if (lane > j && lane < n) a[lane * NS + j] *= inv_p;
+    __syncthreads();
+    if (lane > j && lane < n) {
+      double ujk = a[j * NS + lane];
+      for (int i = j + 1; i < n; i++) a[i * NS + lane] -= a[i * NS + j] * ujk;
+    }
+    __syncthreads();
+  }
+  if (singular) {
+    if (lane == 0) {
+      logdet_out[2 * w + s] = -1e201;
+      sgn_out[2 * w + s] = 1;
+    }
+    for (int k = lane; k < n * nsp; k += kWave) out[k] = (TI)0;
+    return;
+  }
+  if (lane == 0) {
+    double ld = 0;
+    for (int i = 0; i < n; i++) {
+      ld += log(fabs(a[i * NS + i]));
+      if (a[i * NS + i] < 0) sign = -sign;
+    }
+    logdet_out[2 * w + s] = ld;
+    sgn_out[2 * w + s] = sign;
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int i = 0; i < n; i++) {
+      int t = perm[i];
+      perm[i] = perm[indx[i]];
+      perm[indx[i]] = t;
+    }
+  }
+  __syncthreads();
+  double x[NS];
+#pragma unroll
+  for (int i = 0; i < NS; i++) x[i] = (i < n && perm[i] == lane) ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 1; i < NS; i++) {
+    if (i < n) {
+      double acc = x[i];
+#pragma unroll
+      for (int k = 0; k < i; k++) acc -= a[i * NS + k] * x[k];
+      x[i] = acc;
+    }
+  }
+#pragma unroll
+  for (int i = NS - 1; i >= 0; i--) {
+    if (i < n) {
+      double acc = x[i];
+#pragma unroll
+      for (int k = i + 1; k < NS; k++)
+        if (k < n) acc -= a[i * NS + k] * x[k];
+      x[i] = acc / a[i * NS + i];
+    }
+  }
+  if (lane < n) {
+#pragma unroll
+    for (int i = 0; i < NS; i++)
+      if (i < n) out[(size_t)lane * nsp + i] = (TI)x[i];
+  }
+}
+
+// ---------------------------------------------------------------- k_sweep
+struct SweepArgs {
+  int s, k0, k1;
+  double tau;
+  unsigned long long seed;
+  unsigned step, sweep;
+  int restrict_nodes;
+};
+
+template <int NS, int DEPTH>
+__global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs A) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x, w = blockIdx.x;
+  if (w >= W.nw) return;
+  const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
+  double* pos = lds;
+  double* chi = pos + 3 * ne;
+  double* rowb = chi + 5 * S.chunk_nf;
+  double* phib = rowb + NS;
+  for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  double inv[NS];
+  const double* ig = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
+#pragma unroll
+  for (int j = 0; j < NS; j++) inv[j] = (lane < n && j < n) ? ig[(size_t)lane * nsp + j] : 0.0;
+  double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
+  int nacc = 0;
+  double* mog = W.mo + (size_t)w * ne * 5 * nsp;
+  const double tau = A.tau, stau = sqrt(tau);
+  __syncthreads();
+  for (int k = A.k0; k < A.k1; k++) {
+    const int e = off + k;
+    double r0[3] = {pos[3 * e], pos[3 * e + 1], pos[3 * e + 2]};
+    if (lane == k) {
+#pragma unroll
+      for (int j = 0; j < NS; j++) rowb[j] = inv[j];
+    }
+    __syncthreads();
+    const double rk = lane < n ? rowb[lane] : 0.0;
+    double m1 = 0, m2 = 0, m3 = 0;
+    if (lane < n) {
+      m1 = mog[((size_t)e * 5 + 1) * nsp + lane];
+      m2 = mog[((size_t)e * 5 + 2) * nsp + lane];
+      m3 = mog[((size_t)e * 5 + 3) * nsp + lane];
+    }
+    // drift at the old position: determinant part from the cached MO row, Jastrow part fresh
+    double d0[3], gj0[3];
+    const double gdx = wsum(rk * m1), gdy = wsum(rk * m2), gdz = wsum(rk * m3);
+    const double u0 = jastrow_e(S, e, r0, pos, gj0, nullptr, lane);
+    d0[0] = gdx + gj0[0];
+    d0[1] = gdy + gj0[1];
+    d0[2] = gdz + gj0[2];
+    double phi[5];
+    double r1[3], d1[3], ln1 = 0, R = 0;
+    bool accepted = false;
+    double acc_first = 0;
+    for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
+      const double t = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
+      double g[3], dr[3], rn[3], dn[3];
+      gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      lim_drift(d0, t, dr);
+      for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
+      lcao_lap(S, s, rn, chi, phi, lane);
+      double t0 = wsum(rk * phi[0]);
+      double t1 = wsum(rk * phi[1]), t2 = wsum(rk * phi[2]), t3 = wsum(rk * phi[3]);
+      double gj[3];
+      double un = jastrow_e(S, e, rn, pos, gj, nullptr, lane);
+      dn[0] = t1 / t0 + gj[0];
+      dn[1] = t2 / t0 + gj[1];
+      dn[2] = t3 / t0 + gj[2];
+      double lnn = log(fabs(t0)) + (un - u0);
+      bool neg = t0 < 0;
+      double acc;
+      if (dep == 1) {
+        double rr = exp(lnn);
+        acc = (A.restrict_nodes && neg) ? 0.0
+                                        : fmin(1.0, rr * rr * exp(trans(rn, dn, r0, t) - trans(r0, d0, rn, t)));
+        acc_first = acc;
+        for (int d = 0; d < 3; d++) r1[d] = rn[d], d1[d] = dn[d];
+        ln1 = lnn;
+      } else {
+        // delayed rejection from the origin to the second point, through the rejected first
+        // point (src/method/Split_sample.cpp:302-373)
+        double rr = exp(lnn);
+        double pt = (trans(rn, dn, r1, tau) - trans(r0, d0, r1, tau)) + (trans(rn, dn, r0, t) - trans(r0, d0, rn, t));
+        double r21 = exp(ln1 - lnn);
+        double a21 = fmin(1.0, r21 * r21 * exp(trans(r1, d1, rn, tau) - trans(rn, dn, r1, tau)));
+        double num = 1 - a21, den = 1 - acc_first;
+        if (A.restrict_nodes && neg) acc = 0;
+        else if (fabs(den) < 1e-10) acc = fabs(num) < 1e-10 ? 0.0 : 1.0;
+        else acc = fmin(1.0, rr * rr * exp(pt) * num / den);
+      }
+      double u = uniform(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
+      if (acc + u > 1.0) {
+        accepted = true;
+        R = t0;
+        for (int d = 0; d < 3; d++) r1[d] = rn[d];
+      }
+    }
+    if (accepted) {
+      // Sherman-Morrison on the transposed inverse for the replaced row k
+      if (lane < n) phib[lane] = phi[0];
+      __syncthreads();
+      if (lane < n) {
+        double tk = 0;
+#pragma unroll
+        for (int j = 0; j < NS; j++)
+          if (j < n) tk += inv[j] * phib[j];
+        if (lane == k) {
+          const double f = 1.0 / R;
+#pragma unroll
+          for (int j = 0; j < NS; j++) inv[j] = rowb[j] * f;
+        } else {
+          const double pk = tk * (-1.0 / R);
+#pragma unroll
+          for (int j = 0; j < NS; j++)
+            if (j < n) inv[j] += rowb[j] * pk;
+        }
+        for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + lane] = phi[c];
+      }
+      logdet += log(fabs(R));
+      if (R < 0) sgn = -sgn;
+      nacc++;
+      if (lane < 3) pos[3 * e + lane] = r1[lane];
+    }
+    __syncthreads();
+  }
+  double* igw = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
+  if (lane < n) {
+#pragma unroll
+    for (int j = 0; j < NS; j++)
+      if (j < n) igw[(size_t)lane * nsp + j] = inv[j];
+  }
+  for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
+  if (lane == 0) {
+    W.logdet[2 * w + s] = logdet;
+    W.sgn[2 * w + s] = sgn;
+    W.acc[w] += nacc;
+  }
+}
+
+// ---------------------------------------------------------------- k_local
+struct LocalArgs {
+  int mode;              // 0 = evaluate (write outputs), 1 = VMC sums, 2 = DMC weight + sums, 3 = init (elast only)
+  int pp_mode;           // 0 deterministic, 1 stochastic
+  const double* rot;     // [9] fixed rotation (mode 0) or null
+  const double* pp_rand; // [nw][na][ne] uniforms (mode 0, stochastic) or null
+  unsigned long long seed;
+  unsigned step;
+  double* out_logpsi;    // mode 0 outputs
+  double* out_sign;
+  double* out_grad;
+  double* out_lap;
+  double* out_eparts;
+  DmcCtl* ctl;
+};
+
+__global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs A) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x, w = blockIdx.x;
+  if (w >= W.nw) return;
+  const int ne = S.ne, nsp = S.nsp;
+  double* pos = lds;
+  double* gD = pos + 3 * ne;
+  double* lD = gD + 3 * ne;
+  double* chi = lD + ne;
+  for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  const double* mog = W.mo + (size_t)w * ne * 5 * nsp;
+  // determinant gradient / laplacian per electron: lane k owns row k of its spin's inverse
+  for (int s = 0; s < 2; s++) {
+    const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0;
+    const double* ig = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
+    for (int k = lane; k < n; k += kWave) {
+      const int e = off + k;
+      double t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+      for (int j = 0; j < n; j++) {
+        double iv = ig[(size_t)k * nsp + j];
+        t1 += mog[((size_t)e * 5 + 1) * nsp + j] * iv;
+        t2 += mog[((size_t)e * 5 + 2) * nsp + j] * iv;
+        t3 += mog[((size_t)e * 5 + 3) * nsp + j] * iv;
+        t4 += mog[((size_t)e * 5 + 4) * nsp + j] * iv;
+      }
+      gD[3 * e] = t1;
+      gD[3 * e + 1] = t2;
+      gD[3 * e + 2] = t3;
+      lD[e] = t4;
+    }
+  }
+  __syncthreads();
+  // Jastrow per electron (lanes over electrons), kinetic energy, Coulomb
+  double ke = 0, utot = 0, vei = 0, vee = 0;
+  for (int i = lane; i < ne; i += kWave) {
+    const double* pi = pos + 3 * i;
+    double gx = 0, gy = 0, gz = 0, l = 0, u = 0;
+    for (int a = 0; a < S.na; a++) {
+      const double* ap = S.atom_pos + 3 * a;
+      double x = pi[0] - ap[0], y = pi[1] - ap[1], z = pi[2] - ap[2];
+      double r = sqrt(x * x + y * y + z * z);
+      vei += S.atom_Z[a] / r;
+      double gf = 0;
+      for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_lap(S.ei[o], S.ei[o].cl, r, u, gf, l);
+      gx += gf * x;
+      gy += gf * y;
+      gz += gf * z;
+    }
+    const bool iup = i < S.nup;
+    for (int j = 0; j < ne; j++) {
+      if (j == i) continue;
+      double x = pi[0] - pos[3 * j], y = pi[1] - pos[3 * j + 1], z = pi[2] - pos[3 * j + 2];
+      double r = sqrt(x * x + y * y + z * z);
+      if (j < i) vee += 1.0 / r;
+      bool unlike = (j < S.nup) != iup;
+      double gf = 0, upair = 0;
+      for (int o = 0; o < S.nee; o++) jobj_lap(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, upair, gf, l);
+      if (j > i) u += upair;
+      gx += gf * x;
+      gy += gf * y;
+      gz += gf * z;
+    }
+    utot += u;
+    double dot = gD[3 * i] * gx + gD[3 * i + 1] * gy + gD[3 * i + 2] * gz;
+    double lap = lD[i] + (l + (gx * gx + gy * gy + gz * gz)) + 2 * dot;
+    ke += -0.5 * lap;
+    if (A.mode == 0) {
+      double* og = A.out_grad + ((size_t)w * ne + i) * 3;
+      og[0] = gD[3 * i] + gx;
+      og[1] = gD[3 * i + 1] + gy;
+      og[2] = gD[3 * i + 2] + gz;
+      A.out_lap[(size_t)w * ne + i] = lap;
+    }
+  }
+  ke = wsum(ke);
+  utot = wsum(utot);
+  double vloc = -wsum(vei) + S.vii + wsum(vee);
+
+  // nonlocal pseudopotential (src/system/Pseudopotential.cpp:596-815)
+  double vnl = 0;
+  if (S.has_pp) {
+    double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (A.mode == 0) {
+      if (A.rot)
+        for (int i = 0; i < 9; i++) rot[i] = A.rot[i];
+    } else {
+      random_rotation(A.seed, (unsigned)w, A.step, rot);
+    }
+    for (int a = 0; a < S.na; a++) {
+      const int pk = S.atom_pp[a];
+      if (pk < 0) continue;
+      const PPType& P = S.pp[pk];
+      const double* ap = S.atom_pos + 3 * a;
+      for (int e0 = 0; e0 < ne; e0 += kWave) {
+        const int e = e0 + lane;
+        bool want = false;
+        double vl[kMaxPPL];
+        if (e < ne) {
+          double x = pos[3 * e] - ap[0], y = pos[3 * e + 1] - ap[1], z = pos[3 * e + 2] - ap[2];
+          double r = sqrt(x * x + y * y + z * z);
+          pp_radial(P, S.atom_Z[a], r, vl);
+          if (A.pp_mode == 0) {
+            want = r < P.cutoff;
+          } else {
+            double strength = 0;
+            for (int l = 0; l < P.L - 1; l++) strength += 10.0 * (2 * l + 1) * fabs(vl[l]);
+            strength = fmin(1.0, strength);
+            double rr = A.mode == 0 ? A.pp_rand[((size_t)w * S.na + a) * ne + e]
+                                    : uniform(A.seed, (unsigned)w, A.step, (unsigned)(a * ne + e), TAG_PP);
+            want = strength > rr;
+          }
+          vnl += vl[P.L - 1];  // local channel for every electron
+        }
+        unsigned long long mask = __ballot(want);
+        while (mask) {
+          const int b = __ffsll((long long)mask) - 1;
+          mask &= mask - 1;
+          const int ee = e0 + b;
+          const int s = ee < S.nup ? 0 : 1, k = ee - (s ? S.nup : 0), n = s ? S.ndn : S.nup;
+          const double* re = pos + 3 * ee;
+          double od[3] = {re[0] - ap[0], re[1] - ap[1], re[2] - ap[2]};
+          double orr = sqrt(od[0] * od[0] + od[1] * od[1] + od[2] * od[2]);
+          double v[kMaxPPL];
+          pp_radial(P, S.atom_Z[a], orr, v);
+          if (A.pp_mode == 1) {
+            double strength = 0;
+            for (int l = 0; l < P.L - 1; l++) strength += 10.0 * (2 * l + 1) * fabs(v[l]);
+            strength = fmin(1.0, strength);
+            if (fabs(strength) > 0)
+              for (int l = 0; l < P.L - 1; l++) v[l] /= strength;
+          }
+          double gtmp[3];
+          const double u_old = jastrow_e_val(S, ee, re, pos, lane);
+          (void)gtmp;
+          const double ivk = lane < n ? W.inv[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
+          double nl = 0;
+          for (int q = 0; q < P.aip; q++) {
+            double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];
+            double qy = P.q[q][0] * rot[1] + P.q[q][1] * rot[4] + P.q[q][2] * rot[7];
+            double qz = P.q[q][0] * rot[2] + P.q[q][1] * rot[5] + P.q[q][2] * rot[8];
+            double np_[3] = {re[0] + (qx * orr - od[0]), re[1] + (qy * orr - od[1]), re[2] + (qz * orr - od[2])};
+            double nd[3] = {np_[0] - ap[0], np_[1] - ap[1], np_[2] - ap[2]};
+            double nr = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+            double cs = (nd[0] * od[0] + nd[1] * od[1] + nd[2] * od[2]) / (nr * orr);
+            double phi = lcao_val(S, s, np_, chi, lane);
+            double Rq = wsum(ivk * phi);
+            double uq = jastrow_e_val(S, ee, np_, pos, lane);
+            double ratio = Rq * exp(uq - u_old) * P.w[q];
+            double ts = 0;
+            for (int l = 0; l < P.L - 1; l++) ts += (2 * l + 1) * v[l] * legendre(cs, l);
+            nl += ts * ratio;
+          }
+          if (lane == 0) vnl += nl;
+        }
+      }
+    }
+    vnl = wsum(vnl);
+  }
+  const double etot = ke + vloc + vnl;
+  if (lane == 0) {
+    double* el = W.elast + 4 * (size_t)w;
+    el[0] = ke;
+    el[1] = vloc;
+    el[2] = vnl;
+    el[3] = etot;
+    if (A.mode == 0) {
+      A.out_logpsi[w] = W.logdet[2 * w] + W.logdet[2 * w + 1] + utot;
+      A.out_sign[w] = W.sgn[2 * w] * W.sgn[2 * w + 1];
+      A.out_eparts[3 * w] = ke;
+      A.out_eparts[3 * w + 1] = vloc;
+      A.out_eparts[3 * w + 2] = vnl;
+    } else if (A.mode == 3) {
+      W.eold[w] = etot;
+    } else {
+      double wt = W.weight[w];
+      if (A.mode == 2) {
+        const DmcCtl& C = *A.ctl;
+        double teffac = C.teff / 2.0;
+        double eo = W.eold[w];
+        double fbet = fmax(C.etrial - etot, C.etrial - eo);
+        if (fbet > C.cut_stop) teffac = 0;
+        else if (fbet > C.cut_start) teffac = teffac * (1. - (fbet - C.cut_start) / (C.cut_stop - C.cut_start));
+        wt *= exp(teffac * (C.etrial * 2 - etot - eo));
+        if (wt > C.max_weight) wt = C.max_weight;
+        W.weight[w] = wt;
+      }
+      W.eold[w] = etot;
+      double* bs = W.bsum + 6 * (size_t)w;
+      bs[0] += wt;
+      bs[1] += wt * etot;
+      bs[2] += wt * ke;
+      bs[3] += wt * vloc;
+      bs[4] += wt * vnl;
+      bs[5] += 1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- k_branch
+// Single block.  Trial energy and effective timestep from the population, then the
+// fixed-population pair comb: sort by weight (ties by walker index), pair the p-th largest
+// weight >= 1.8 with the p-th smallest, keep one of the two with probability
+// w_big / (w_big + w_small), both slots get the mean weight.
+__global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsigned long long* keys, int* vals,
+                                                 int* pair_src, int* pair_dst, int nmoves, int do_branch,
+                                                 unsigned long long seed, unsigned event) {
+  __shared__ double red[1024];
+  __shared__ long long redi[1024];
+  const int t = threadIdx.x, nw = W.nw;
+  double sw = 0;
+  long long sa = 0;
+  for (int i = t; i < nw; i += blockDim.x) {
+    sw += W.weight[i];
+    sa += W.acc[i];
+  }
+  red[t] = sw;
+  redi[t] = sa;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (t < o) {
+      red[t] += red[t + o];
+      redi[t] += redi[t + o];
+    }
+    __syncthreads();
+  }
+  DmcCtl& C = *ctl;
+  if (t == 0) {
+    if (nmoves > 0) C.teff = C.tau * (double)redi[0] / (double)nmoves;
+    C.sumw = red[0];
+    C.etrial = C.eref - C.feedback * log(red[0] / nw);
+    C.npairs = 0;
+  }
+  __syncthreads();
+  for (int i = t; i < nw; i += blockDim.x) W.acc[i] = 0;
+  if (!do_branch) return;
+  int P = 1;
+  while (P < nw) P <<= 1;
+  for (int i = t; i < P; i += blockDim.x) {
+    if (i < nw) {
+      keys[i] = (unsigned long long)__double_as_longlong(W.weight[i]);
+      vals[i] = i;
+    } else {
+      keys[i] = ~0ull;
+      vals[i] = i;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P; i += blockDim.x) {
+        int l = i ^ j;
+        if (l > i) {
+          bool up = (i & k) == 0;
+          unsigned long long ki = keys[i], kl = keys[l];
+          int vi = vals[i], vl = vals[l];
+          bool gt = ki > kl || (ki == kl && vi > vl);
+          if (gt == up) {
+            keys[i] = kl;
+            keys[l] = ki;
+            vals[i] = vl;
+            vals[l] = vi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // pairs: p-th from the top with p-th from the bottom while the top weight >= 1.8
+  __shared__ int npair;
+  if (t == 0) {
+    int np = 0;
+    for (int i = nw - 1; i > 1 && 2 * np + 1 < nw; i--, np++)
+      if (W.weight[vals[i]] < 1.8) break;
+    npair = np;
+    C.npairs = np;
+  }
+  __syncthreads();
+  for (int p = t; p < npair; p += blockDim.x) {
+    int big = vals[nw - 1 - p], sm = vals[p];
+    double wb = W.weight[big], ws = W.weight[sm];
+    double w1 = wb / (wb + ws);
+    double u = uniform(seed, (unsigned)p, event, 0, TAG_BRANCH);
+    double nwt = (wb + ws) / 2.0;
+    if (w1 + u >= 1.0) {
+      pair_src[p] = big;
+      pair_dst[p] = sm;
+    } else {
+      pair_src[p] = sm;
+      pair_dst[p] = big;
+    }
+    W.weight[big] = nwt;
+    W.weight[sm] = nwt;
+  }
+}
+
+// Copies the state of walker pair_src[p] into walker pair_dst[p] for p < ctl->npairs.
+__global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcCtl* ctl, const int* pair_src,
+                                              const int* pair_dst) {
+  const int p = blockIdx.x;
+  if (p >= ctl->npairs) return;
+  const int src = pair_src[p], dst = pair_dst[p], t = threadIdx.x;
+  const int ne = S.ne, nsp = S.nsp;
+  const size_t npos = (size_t)ne * 3, ninv = (size_t)2 * nsp * nsp, nmo = (size_t)ne * 5 * nsp;
+  for (size_t i = t; i < npos; i += blockDim.x) W.pos[dst * npos + i] = W.pos[src * npos + i];
+  if (W.inv)
+    for (size_t i = t; i < ninv; i += blockDim.x) W.inv[dst * ninv + i] = W.inv[src * ninv + i];
+  if (W.invf)
+    for (size_t i = t; i < ninv; i += blockDim.x) W.invf[dst * ninv + i] = W.invf[src * ninv + i];
+  if (W.mo)
+    for (size_t i = t; i < nmo; i += blockDim.x) W.mo[dst * nmo + i] = W.mo[src * nmo + i];
+  if (W.mof)
+    for (size_t i = t; i < nmo; i += blockDim.x) W.mof[dst * nmo + i] = W.mof[src * nmo + i];
+  if (t < 2) {
+    W.logdet[2 * dst + t] = W.logdet[2 * src + t];
+    W.sgn[2 * dst + t] = W.sgn[2 * src + t];
+  }
+  if (t < 4) W.elast[4 * (size_t)dst + t] = W.elast[4 * (size_t)src + t];
+  if (t == 0) W.eold[dst] = W.eold[src];
+}
+
+// Block sums over walkers: out[0..5] = sums of bsum columns, out[6] = current weight sum.
+__global__ __launch_bounds__(256) void k_reduce(DevWalkers W, double* out, int reset) {
+  __shared__ double red[7][256];
+  const int t = threadIdx.x;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = t; i < W.nw; i += blockDim.x) {
+    for (int c = 0; c < 6; c++) acc[c] += W.bsum[6 * (size_t)i + c];
+    acc[6] += W.weight[i];
+    if (reset)
+      for (int c = 0; c < 6; c++) W.bsum[6 * (size_t)i + c] = 0;
+  }
+  for (int c = 0; c < 7; c++) red[c][t] = acc[c];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int c = 0; c < 7; c++) red[c][t] += red[c][t + o];
+    __syncthreads();
+  }
+  if (t < 7) out[t] = red[t][0];
+}
+
+}  // namespace qw

@@ -1,0 +1,91 @@
+// Host-side setup of the walker engine: reads the QWalk deck (SYSTEM, PSEUDO, TRIALFUNC,
+// METHOD) and flattens it into the tables the device kernels use.
+#pragma once
+#include <string>
+#include <vector>
+
+namespace qw {
+
+constexpr int kMaxJF = 8;     // functions per Jastrow radial object
+constexpr int kMaxPPL = 6;    // pseudopotential channels
+constexpr int kMaxPPT = 6;    // Gaussian terms per channel
+constexpr int kMaxAIP = 12;   // quadrature points
+
+enum SymKind { SK_S = 0, SK_P = 1, SK_5D = 2, SK_6D = 3, SK_10F = 4 };
+
+// One radial spline shell placed on an atom.
+struct ShellDesc {
+  int table;     // spline table index
+  int sym;       // SymKind
+  int f0;        // first basis function (global index)
+  int nf;        // number of angular functions
+  int atom;
+  int pad;
+  double rcut;   // cutoff of this shell's functions
+  double objthr; // cutoff of the basis object the shell belongs to
+};
+
+// A Jastrow radial object (CUTOFF_CUSP: kind 0, nf 1; POLYPADE: kind 1) with its
+// coefficients folded in: cl = like-spin (or one-body) coefficient, cu = unlike-spin.
+struct JObj {
+  int kind;
+  int nf;
+  double rcut, rcutinv, cusp, gamma, pade0;
+  double beta[kMaxJF];
+  double cl[kMaxJF];
+  double cu[kMaxJF];
+};
+
+// Semilocal pseudopotential of one atom type (RGAUSSIAN OLDQMC).
+struct PPType {
+  int L;                     // channels; the last is the local one
+  int nterm[kMaxPPL];
+  double n[kMaxPPL][kMaxPPT];  // power of r (already decremented by 2)
+  double a[kMaxPPL][kMaxPPT];
+  double c[kMaxPPL][kMaxPPT];
+  int aip;
+  int addzeff;
+  double q[kMaxAIP][3];
+  double w[kMaxAIP];
+  double cutoff;
+};
+
+struct SysTables {
+  int nup = 0, ndn = 0, na = 0, nb = 0;
+  std::vector<double> atom_pos;   // [na][3]
+  std::vector<double> atom_Z;     // [na]
+  std::vector<ShellDesc> shells;  // grouped by atom
+  std::vector<int> atom_shell0;   // [na+1]
+  int ntab = 0, nint = 0;         // spline tables, intervals per table (padded to the max)
+  std::vector<double> spline;     // [ntab][nint][4]
+  std::vector<double> tab_invspacing, tab_spacing;
+  std::vector<double> C[2];       // per spin [nb][nspin]
+  std::vector<JObj> ee;           // electron-electron objects (merged over groups)
+  std::vector<JObj> ei;           // electron-ion objects, atom ranges in atom_ei0
+  std::vector<int> atom_ei0;      // [na+1]
+  std::vector<PPType> pptypes;
+  std::vector<int> atom_pp;       // [na], -1 = no pseudopotential
+  double vii = 0;                 // ion-ion Coulomb energy
+  int ne() const { return nup + ndn; }
+};
+
+struct MethodDesc {
+  int method = 0;
+  int nconfig = 1;
+  int nblock = 100;
+  int nstep = 100;
+  int ndecorr = 2;
+  double timestep = 0.5;
+  int feedback_interval = 5;
+  double feedback = 1.0;
+  double max_weight = 7.0;
+  double branch_start_cutoff = 10.0;
+  int pp_stochastic = 1;
+};
+
+// Throws std::runtime_error with a message on bad or unsupported input.
+// `unsupported` is set when the input is valid QWalk but outside this engine's scope.
+SysTables read_deck(const std::string& path, bool* unsupported);
+MethodDesc read_method(const std::string& path);
+
+}  // namespace qw

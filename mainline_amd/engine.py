@@ -1,0 +1,156 @@
+"""ctypes binding of libqwalk_amd.so (the C ABI in include/qwalk_amd.h).
+
+The product path: every call goes to the HIP engine.  If the shared library is missing or
+fails to load, the import of this module's Engine raises; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqwalk_amd.so")
+
+PREC = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16": 2}
+STATUS = {0: "ok", -1: "bad argument", -2: "bad input", -3: "unsupported", -4: "device error", -5: "out of memory"}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class EngineOpts(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("device", ctypes.c_int), ("seed", ctypes.c_ulonglong),
+                ("refresh_interval", ctypes.c_int)]
+
+
+class MethodDesc(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("nconfig", ctypes.c_int), ("nblock", ctypes.c_int),
+                ("nstep", ctypes.c_int), ("ndecorr", ctypes.c_int), ("timestep", ctypes.c_double),
+                ("feedback_interval", ctypes.c_int), ("feedback", ctypes.c_double),
+                ("max_weight", ctypes.c_double), ("branch_start_cutoff", ctypes.c_double),
+                ("pp_stochastic", ctypes.c_int)]
+
+
+class QwError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s (%s)" % (msg, STATUS.get(status, status)))
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libqwalk_amd.so is not built; run __graft_entry__.build() or "
+                              "make -f mainline_amd/Makefile")
+        L = ctypes.CDLL(LIB_PATH)
+        L.qw_engine_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(EngineOpts), ctypes.POINTER(ctypes.c_void_p)]
+        L.qw_engine_destroy.argtypes = [ctypes.c_void_p]
+        L.qw_last_error.restype = ctypes.c_char_p
+        L.qw_last_error.argtypes = [ctypes.c_void_p]
+        L.qw_engine_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.qw_eval_fixed.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
+                                    _dp, _dp, _dp, _dp, _dp]
+        L.qw_method_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(MethodDesc)]
+        L.qw_walkers_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.qw_walkers_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.qw_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(MethodDesc), _dp, _dp]
+        L.qw_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names of the qw_* functions the loaded library exports (no device calls)."""
+    L = lib()
+    names = []
+    for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
+              "qw_method_read", "qw_walkers_set", "qw_walkers_get", "qw_run", "qw_kernel_times"]:
+        if hasattr(L, n):
+            names.append(n)
+    return names
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def read_method(deck):
+    m = MethodDesc()
+    st = lib().qw_method_read(deck.encode(), ctypes.byref(m))
+    if st:
+        raise QwError(st, lib().qw_last_error(None).decode())
+    return m
+
+
+def method(kind, nconfig, nblock, nstep, timestep, ndecorr=2, feedback_interval=5, feedback=1.0,
+           max_weight=7.0, branch_start_cutoff=10.0, pp_stochastic=1):
+    return MethodDesc(1 if kind.upper() == "DMC" else 0, nconfig, nblock, nstep, ndecorr, timestep,
+                      feedback_interval, feedback, max_weight, branch_start_cutoff, pp_stochastic)
+
+
+class Engine:
+    """One walker engine on one device for the SYSTEM / PSEUDO / TRIALFUNC of a deck."""
+
+    def __init__(self, deck, precision="fp64", device=0, seed=20261202, refresh_interval=0):
+        h = ctypes.c_void_p()
+        opts = EngineOpts(PREC[precision], device, seed, refresh_interval)
+        st = lib().qw_engine_create(deck.encode(), ctypes.byref(opts), ctypes.byref(h))
+        if st:
+            raise QwError(st, lib().qw_last_error(None).decode())
+        self.h = h
+        info = (ctypes.c_int * 6)()
+        lib().qw_engine_info(self.h, info)
+        self.nup, self.ndn, self.natoms, self.nbasis, self.has_pp = list(info)[:5]
+        self.ne = self.nup + self.ndn
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qw_engine_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, st):
+        if st:
+            raise QwError(st, lib().qw_last_error(self.h).decode())
+
+    def eval(self, pos, rot=None, pp_mode=0, pp_rand=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        K = pos.shape[0]
+        out = dict(logpsi=np.zeros(K), sign=np.zeros(K), grad=np.zeros((K, self.ne, 3)),
+                   lap=np.zeros((K, self.ne)), eparts=np.zeros((K, 3)))
+        r = None if rot is None else np.ascontiguousarray(rot, dtype=np.float64)
+        pr = None if pp_rand is None else np.ascontiguousarray(pp_rand, dtype=np.float64)
+        self._check(lib().qw_eval_fixed(self.h, K, _p(pos), _p(r), pp_mode, _p(pr), _p(out["logpsi"]),
+                                        _p(out["sign"]), _p(out["grad"]), _p(out["lap"]), _p(out["eparts"])))
+        return out
+
+    def set_walkers(self, pos, weights=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        self._check(lib().qw_walkers_set(self.h, pos.shape[0], _p(pos), _p(w)))
+
+    def get_walkers(self, nw):
+        pos = np.zeros((nw, self.ne, 3))
+        w = np.zeros(nw)
+        self._check(lib().qw_walkers_get(self.h, nw, _p(pos), _p(w)))
+        return pos, w
+
+    def run(self, m):
+        eb = np.zeros((max(m.nblock, 1), 5))
+        stats = np.zeros(4)
+        self._check(lib().qw_run(self.h, ctypes.byref(m), _p(eb), _p(stats)))
+        return dict(eblock=eb[:m.nblock], acceptance=stats[0], moves=stats[1], seconds=stats[2],
+                    moves_per_s=stats[3])
+
+    def kernel_times(self):
+        buf = ctypes.create_string_buffer(4096)
+        avg = np.zeros(64)
+        cnt = (ctypes.c_int * 64)()
+        nk = ctypes.c_int()
+        self._check(lib().qw_kernel_times(self.h, buf, 4096, _p(avg), cnt, 64, ctypes.byref(nk)))
+        names = buf.value.decode().split(",") if nk.value else []
+        return {n: (avg[i], cnt[i]) for i, n in enumerate(names)}

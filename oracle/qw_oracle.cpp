@@ -1891,7 +1891,7 @@ extern "C" double orc_run(orc_system* o, const orc_run_opts* opt, double* pos, d
                          [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first < b.first; });
         int small = 0;
         unsigned bev = step;
-        for (int i = nw - 1, pair = 0; i > 1; i--, pair++) {
+        for (int i = nw - 1, pair = 0; i > 1 && 2 * pair + 1 < nw; i--, pair++) {
           if (srt[i].first < 1.8) break;
           int big = srt[i].second, sm = srt[small].second;
           double w1 = w[big] / (w[big] + w[sm]);

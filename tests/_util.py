@@ -1,0 +1,51 @@
+"""Shared helpers for the tests: fixture loading and synthetic deck materialisation."""
+import hashlib
+import os
+import tempfile
+
+import numpy as np
+
+from mainline_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+_DECK_DIR = os.path.join(tempfile.gettempdir(), "qw_test_decks_%d" % os.getuid())
+
+SYNTH = {"h4": synth.h4, "pp16": synth.pp16, "ae32": synth.ae32}
+
+
+def synth_deck(name):
+    """Write the synthetic molecule `name` once per session; returns (deck path, molecule)."""
+    mol = SYNTH[name]()
+    d = os.path.join(_DECK_DIR, name)
+    deck = os.path.join(d, name + ".wfeval")
+    if not os.path.exists(deck):
+        mol.write(d)
+    return deck, mol
+
+
+def golden(name):
+    """Fixture dict and the deck path it was made from."""
+    f = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    g = {k: f[k] for k in f.files}
+    deck = str(g["deck"])
+    if deck.startswith("synth:"):
+        path, mol = synth_deck(deck[6:])
+        orb = open(os.path.join(os.path.dirname(path), mol.name + ".orb"), "rb").read()
+        assert hashlib.sha256(orb).hexdigest() == str(g["orb_sha256"]), "synthetic generator drifted"
+    else:
+        path = os.path.join(GOLDEN, deck)
+    return g, path
+
+
+def relerr(a, b, floor=1e-300):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if a.size else 0.0
+
+
+def scaled_err(a, b):
+    """max |a-b| / max |b| (for vectors whose entries cross zero)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))

@@ -1,0 +1,152 @@
+"""Writes the golden vectors under tests/golden/ from the compiled reference.
+
+Runs in the build container only: it needs oracle/_ref/qw_golden, which oracle/Makefile.ref
+builds from the reference sources.  For each case it evaluates fixed electron
+configurations through the reference's own Wavefunction / System / Pseudopotential objects
+(deterministic pseudopotential, identity quadrature rotation) and stores inputs and outputs
+as an .npz fixture:
+
+  pos[K][Ne][3], logpsi[K], sign[K], grad[K][Ne][3], lap[K][Ne], kinetic[K], vloc[K],
+  vnonloc[K]  (+ orb_sha256 for generated decks, to detect generator drift)
+
+Cases:
+  h_atom    the reference's tests/h deck (H atom, AOSPLINE S shells, CUTOFF_MO)
+  h2_jast   the reference's tests/h2 deck with its two-group Jastrow (cusp + POLYPADE)
+  n2_pp     the reference's tests/n2 basis, orbitals and BFD pseudopotential (AIP 6), one
+            determinant, with a POLYPADE Jastrow of nonzero coefficients
+  h4, pp16, ae32   the seeded synthetic molecules of mainline_amd/synth.py (4, 64, 128 e)
+
+Usage: python tests/golden/make_golden.py   (from the repository root)
+"""
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from mainline_amd import synth  # noqa: E402
+from oracle.oracle import REF_GOLDEN, parse_golden  # noqa: E402
+
+REF_TESTS = "/root/reference/tests"
+OUT = os.path.dirname(os.path.abspath(__file__))
+DECKS = os.path.join(OUT, "decks")
+
+N2_SLATER = """SLATER
+ORBITALS {
+  CUTOFF_MO
+  MAGNIFY 1
+  NMO 8
+  ORBFILE qw.orb
+  INCLUDE qw.basis
+  CENTERS { USEGLOBAL }
+}
+DETWT { 1.0 }
+STATES {
+  1 2 3 4 5
+  1 2 3 4 5
+}
+"""
+N2_JAST = """JASTROW2
+GROUP {
+  EEBASIS { EE CUTOFF_CUSP GAMMA 24.0 CUSP 1.0 CUTOFF 7.5 }
+  EEBASIS { EE CUTOFF_CUSP GAMMA 24.0 CUSP 1.0 CUTOFF 7.5 }
+  TWOBODY_SPIN { FREEZE LIKE_COEFFICIENTS { 0.25 0.0 } UNLIKE_COEFFICIENTS { 0.0 0.5 } }
+}
+GROUP {
+  EEBASIS { EE POLYPADE BETA0 0.5 NFUNC 3 RCUT 7.5 }
+  EIBASIS { N POLYPADE BETA0 0.2 NFUNC 4 RCUT 7.5 }
+  ONEBODY { COEFFICIENTS { N -0.3 0.1 -0.05 0.02 } }
+  TWOBODY { COEFFICIENTS { 0.12 -0.04 0.01 } }
+}
+"""
+
+
+def run_ref(deck_dir, deck, pos):
+    cfg = os.path.join(deck_dir, "golden_cfg.txt")
+    synth.write_configs(cfg, pos)
+    out = subprocess.run([REF_GOLDEN, deck, "golden_cfg.txt"], cwd=deck_dir, capture_output=True,
+                         text=True, check=True).stdout
+    os.remove(cfg)
+    g = parse_golden(out)
+    K = len(g)
+    return dict(
+        pos=np.asarray(pos),
+        logpsi=np.array([c["logpsi"] for c in g]),
+        sign=np.array([c["sign"] for c in g], dtype=np.float64),
+        grad=np.array([c["elec"][:, :3] for c in g]),
+        lap=np.array([c["elec"][:, 3] for c in g]),
+        kinetic=np.array([c["kinetic"] for c in g]),
+        vloc=np.array([c["vloc"] for c in g]),
+        vnonloc=np.array([c["vnonloc"] for c in g]),
+    ) if K else None
+
+
+def clouds(centers, counts, k, seed, spread):
+    rng = np.random.default_rng(seed)
+    idx = np.concatenate([np.arange(n) % len(centers) for n in counts])
+    centers = np.asarray(centers, dtype=np.float64)
+    return centers[idx][None] + spread * rng.standard_normal((k, len(idx), 3))
+
+
+def ref_deck_cases(tmp):
+    cases = {}
+    # H atom (tests/h): qw.sys + qw.slater (+ qw.basis, qw.orb)
+    d = os.path.join(DECKS, "h_atom")
+    os.makedirs(d, exist_ok=True)
+    for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb"]:
+        shutil.copy(os.path.join(REF_TESTS, "h", f), d)
+    with open(os.path.join(d, "h.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC { INCLUDE qw.slater }\n")
+    cases["h_atom"] = (d, "h.wfeval", clouds([[0, 0, 0]], [1, 0], 16, 11, 1.3))
+    # H2 with Jastrow (tests/h2)
+    d = os.path.join(DECKS, "h2_jast")
+    os.makedirs(d, exist_ok=True)
+    for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb", "jast2.wfin"]:
+        shutil.copy(os.path.join(REF_TESTS, "h2", f), d)
+    with open(os.path.join(d, "h2.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW WF1 { INCLUDE qw.slater } "
+                "WF2 { INCLUDE jast2.wfin } }\n")
+    cases["h2_jast"] = (d, "h2.wfeval", clouds([[0, 0, -1.3228], [0, 0, 1.3228]], [1, 1], 16, 12, 1.0))
+    # N2 with the BFD pseudopotential (tests/n2), one determinant
+    d = os.path.join(DECKS, "n2_pp")
+    os.makedirs(d, exist_ok=True)
+    for f in ["qw.sys", "qw.basis", "qw.orb"]:
+        shutil.copy(os.path.join(REF_TESTS, "n2", f), d)
+    with open(os.path.join(d, "n2.slater"), "w") as f:
+        f.write(N2_SLATER)
+    with open(os.path.join(d, "n2.jast2"), "w") as f:
+        f.write(N2_JAST)
+    with open(os.path.join(d, "n2.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW WF1 { INCLUDE n2.slater } "
+                "WF2 { INCLUDE n2.jast2 } }\n")
+    cases["n2_pp"] = (d, "n2.wfeval", clouds([[0, 0, -1.60626709], [0, 0, 1.60626709]], [5, 5], 12, 13, 0.9))
+    return cases
+
+
+def main():
+    if not os.path.exists(REF_GOLDEN):
+        sys.exit("build the reference first: make -f oracle/Makefile.ref -j8")
+    with tempfile.TemporaryDirectory() as tmp:
+        cases = ref_deck_cases(tmp)
+        for name, (d, deck, pos) in cases.items():
+            res = run_ref(d, deck, pos)
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), deck=os.path.relpath(os.path.join(d, deck), OUT), **res)
+            print(name, res["pos"].shape)
+        for mol, k in [(synth.h4(), 16), (synth.pp16(), 8), (synth.ae32(), 4)]:
+            d = os.path.join(tmp, mol.name)
+            deck = os.path.basename(mol.write(d))
+            orb = open(os.path.join(d, mol.name + ".orb"), "rb").read()
+            pos = synth.random_configs(mol, k, seed=101)
+            res = run_ref(d, deck, pos)
+            np.savez_compressed(os.path.join(OUT, mol.name + ".npz"), deck="synth:" + mol.name,
+                                orb_sha256=hashlib.sha256(orb).hexdigest(), **res)
+            print(mol.name, res["pos"].shape)
+
+
+if __name__ == "__main__":
+    main()

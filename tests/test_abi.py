@@ -1,0 +1,54 @@
+"""The C-ABI library: loads, exports every declared entry point, and rejects bad input
+before it touches a device (no GPU needed)."""
+import os
+import re
+
+import pytest
+
+from mainline_amd import engine as E
+from tests._util import ROOT, golden
+
+HEADER = os.path.join(ROOT, "include", "qwalk_amd.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(qw_\w+)\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared()
+    assert len(names) >= 10
+    L = E.lib()
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_missing_deck_is_an_input_error():
+    with pytest.raises(E.QwError) as ex:
+        E.Engine("/nonexistent/deck")
+    assert ex.value.status == -2
+
+
+def test_multideterminant_is_reported_unsupported(tmp_path):
+    g, deck = golden("h_atom")
+    d = os.path.dirname(deck)
+    txt = open(os.path.join(d, "qw.slater")).read().replace("DETWT { 1.0 }", "DETWT { 1.0 0.5 }")
+    (tmp_path / "qw.slater").write_text(txt)
+    for f in ["qw.sys", "qw.basis", "qw.orb"]:
+        (tmp_path / f).write_text(open(os.path.join(d, f)).read())
+    (tmp_path / "x.wfeval").write_text("INCLUDE qw.sys\nTRIALFUNC { INCLUDE qw.slater }\n")
+    with pytest.raises(E.QwError) as ex:
+        E.Engine(str(tmp_path / "x.wfeval"))
+    assert ex.value.status == -3
+
+
+def test_method_read_applies_reference_defaults(tmp_path):
+    p = tmp_path / "m.dmc"
+    p.write_text("METHOD { DMC TIMESTEP 0.01 }\n")
+    m = E.read_method(str(p))
+    assert (m.method, m.nblock, m.nconfig, m.nstep, m.feedback_interval) == (1, 100, 2048, 100, 5)
+    assert (m.max_weight, m.branch_start_cutoff, m.feedback) == (7.0, 10.0, 1.0)
+    p.write_text("METHOD { VMC NCONFIG 256 TIMESTEP 0.5 NBLOCK 50 NSTEP 20 }\n")
+    m = E.read_method(str(p))
+    assert (m.method, m.nconfig, m.nblock, m.nstep, m.ndecorr, m.timestep) == (0, 256, 50, 20, 2, 0.5)

@@ -1,0 +1,113 @@
+"""Parity of the HIP engine with the oracle and the reference's golden vectors (MI355X)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from mainline_amd import engine as E
+from mainline_amd import synth
+from oracle import oracle as O
+from tests._util import golden, relerr, scaled_err, synth_deck
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["h_atom", "h2_jast", "n2_pp", "h4", "pp16", "ae32"]
+TOL = 1e-10  # FP64 path: relative tolerance against the reference's FP64 evaluation
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_eval_matches_reference_golden(name):
+    g, deck = golden(name)
+    _say("eval golden", name, g["pos"].shape)
+    eng = E.Engine(deck, "fp64")
+    r = eng.eval(g["pos"])
+    assert np.array_equal(r["sign"], g["sign"])
+    assert relerr(r["logpsi"], g["logpsi"]) < TOL
+    for k in range(len(g["logpsi"])):
+        assert scaled_err(r["grad"][k], g["grad"][k]) < TOL
+        assert scaled_err(r["lap"][k], g["lap"][k]) < TOL
+    assert relerr(r["eparts"][:, 0], g["kinetic"]) < TOL
+    assert relerr(r["eparts"][:, 1], g["vloc"]) < TOL
+    assert np.max(np.abs(r["eparts"][:, 2] - g["vnonloc"])) <= TOL * max(1.0, np.max(np.abs(g["vnonloc"])))
+
+
+@pytest.mark.parametrize("name", ["h4", "pp16", "ae32"])
+def test_fp64_eval_matches_oracle_random_rotation_and_stochastic_pp(name):
+    deck, mol = synth_deck(name)
+    _say("eval oracle", name)
+    eng = E.Engine(deck, "fp64")
+    orc = O.Oracle(deck)
+    pos = synth.random_configs(mol, 6, seed=77)
+    rng = np.random.default_rng(9)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    rot = q.T.copy()
+    rand = rng.uniform(size=(6, mol.natoms, mol.nelec))
+    for mode in (0, 1):
+        a = eng.eval(pos, rot=rot, pp_mode=mode, pp_rand=rand)
+        b = orc.eval(pos, rot=rot, pp_mode=mode, pp_rand=rand)
+        assert np.array_equal(a["sign"], b["sign"])
+        assert relerr(a["logpsi"], b["logpsi"]) < TOL
+        assert scaled_err(a["grad"], b["grad"]) < TOL
+        assert scaled_err(a["lap"], b["lap"]) < TOL
+        assert relerr(a["eparts"][:, :2], b["eparts"][:, :2]) < TOL
+        assert np.max(np.abs(a["eparts"][:, 2] - b["eparts"][:, 2])) < TOL * max(1, np.max(np.abs(b["eparts"][:, 2])))
+
+
+def test_singular_determinant_is_flagged_like_the_reference():
+    """Two same-spin electrons on one point: det = 0, ln|det| = -1e201 (log_value(0))."""
+    deck, mol = synth_deck("h4")
+    eng = E.Engine(deck, "fp64")
+    pos = synth.random_configs(mol, 2, seed=5)
+    pos[1, 1] = pos[1, 0]
+    r = eng.eval(pos)
+    assert r["logpsi"][1] < -1e200
+    assert np.isfinite(r["logpsi"][0])
+
+
+@pytest.mark.parametrize("name,method", [("h4", "VMC"), ("h4", "DMC"), ("pp16", "DMC"), ("pp16", "VMC")])
+def test_trajectory_matches_oracle(name, method):
+    """Same seed, same counter-based random streams: the device run and the oracle's CPU run
+    take the same moves, so positions, weights and block energies agree to round-off."""
+    deck, mol = synth_deck(name)
+    _say("trajectory", name, method)
+    nw = 16
+    pos = synth.random_configs(mol, nw, seed=21)
+    tau = 0.3 if name == "h4" else 0.02
+    eng = E.Engine(deck, "fp64", seed=4242)
+    eng.set_walkers(pos)
+    m = E.method(method, nw, nblock=2, nstep=5, timestep=tau)
+    res = eng.run(m)
+    gp, gw = eng.get_walkers(nw)
+    orc = O.Oracle(deck)
+    o = orc.run(pos, method, nblock=2, nstep=5, timestep=tau, seed=4242, step0=0)
+    assert np.max(np.abs(gp - o["pos"])) < 1e-8
+    assert np.max(np.abs(gw - o["weights"])) < 1e-8
+    assert np.max(np.abs(res["eblock"][:, :4] - o["eblock"][:, :4])) < 1e-7 * max(1, np.max(np.abs(o["eblock"][:, 0])))
+
+
+def test_h_atom_vmc_and_dmc_known_answers():
+    """tests/h: VMC reproduces the GAMESS Hartree-Fock energy -0.4998098113 and DMC the
+    exact -0.5 (tests/h/run_test.py:20-23,57), each within 3 sigma."""
+    g, deck = golden("h_atom")
+    eng = E.Engine(deck, "fp64", seed=99)
+    rng = np.random.default_rng(2)
+    nw = 2048
+    eng.set_walkers(rng.normal(size=(nw, 1, 3)))
+    eng.run(E.method("VMC", nw, nblock=2, nstep=20, timestep=0.5))
+    res = eng.run(E.method("VMC", nw, nblock=20, nstep=20, timestep=0.5))
+    e = res["eblock"][:, 0]
+    err = e.std(ddof=1) / math.sqrt(len(e))
+    _say("H VMC", e.mean(), err)
+    assert abs(e.mean() - (-0.4998098113)) < 3 * err + 1e-5
+    res = eng.run(E.method("DMC", nw, nblock=2, nstep=200, timestep=0.005))
+    res = eng.run(E.method("DMC", nw, nblock=20, nstep=100, timestep=0.005))
+    e = res["eblock"][:, 0]
+    err = e.std(ddof=1) / math.sqrt(len(e))
+    _say("H DMC", e.mean(), err)
+    assert abs(e.mean() - (-0.5)) < 3 * err + 2e-4
