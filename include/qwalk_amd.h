@@ -47,7 +47,8 @@ enum {
 enum {
   QW_PREC_FP64 = 0,        /* everything in FP64 */
   QW_PREC_MIXED_FP32 = 1,  /* LCAO FP32, rank-1 updates FP32, FP64 LU refresh / log-det / sums */
-  QW_PREC_MIXED_BF16 = 2   /* as MIXED_FP32, LCAO on MFMA in BF16 with split-precision correction */
+  QW_PREC_MIXED_BF16 = 2,  /* as MIXED_FP32, LCAO on MFMA: BF16 split terms, 3 products */
+  QW_PREC_MIXED_BF16X6 = 3 /* as MIXED_BF16 with three-term splits and 6 products */
 };
 
 typedef struct qw_engine_opts {

@@ -258,7 +258,7 @@ struct qw_engine {
   void run_lu_t(TI* out) {
     auto go = [&](auto kern) {
       launch("k_lu", [&] {
-        hipLaunchKernelGGL(kern, dim3(2 * nw), dim3(64), 0, stream, S, (const double*)W.mo, out, W.logdet, W.sgn, nw);
+        hipLaunchKernelGGL(kern, dim3(2 * nw), dim3(64), 0, stream, S, (const double*)W.mo, out, W.logdet, W.sgn, nw, -1);
       });
     };
     switch (ns) {
@@ -309,7 +309,7 @@ struct qw_engine {
       mixed_local(*this, A);
       return;
     }
-    launch("k_local", [&] { hipLaunchKernelGGL(k_local, dim3(nw), dim3(64), lds_local(), stream, S, W, A); });
+    launch("k_local", [&] { hipLaunchKernelGGL(k_local<double>, dim3(nw), dim3(64), lds_local(), stream, S, W, A); });
   }
   void reduce(double* host7, int reset) {
     launch("k_reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, stream, W, dred, reset); });
@@ -353,7 +353,7 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
     e->seed = opts->seed;
     if (opts->refresh_interval > 0) e->refresh = opts->refresh_interval;
   }
-  if (e->precision < 0 || e->precision > 2) {
+  if (e->precision < 0 || e->precision > 3) {
     g_create_error = "unknown precision";
     return QW_ERR_ARG;
   }

@@ -106,13 +106,14 @@ __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) 
 // sign, then the transposed inverse: lane j solves A x = e_j and stores x as row j.
 template <int NS, typename TI>
 __global__ __launch_bounds__(64) void k_lu(DevSys S, const double* mo, TI* inv_out, double* logdet_out,
-                                           double* sgn_out, int nw) {
+                                           double* sgn_out, int nw, int only_spin) {
   __shared__ double a[NS * NS];
   __shared__ double vv[NS];
   __shared__ int indx[NS], perm[NS];
   __shared__ int singular;
   const int lane = threadIdx.x;
-  const int w = blockIdx.x >> 1, s = blockIdx.x & 1;
+  const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+  const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
   if (w >= nw) return;
   const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
   TI* out = inv_out + ((size_t)w * 2 + s) * nsp * nsp;
@@ -381,6 +382,22 @@ struct LocalArgs {
   DmcCtl* ctl;
 };
 
+// TM: element type of the MO cache and inverse (double on the FP64 path, float on the mixed
+// path); the local-energy arithmetic and sums are FP64 on both.
+template <typename TM>
+__device__ inline const TM* mo_cache(const DevWalkers& W);
+template <>
+__device__ inline const double* mo_cache<double>(const DevWalkers& W) { return W.mo; }
+template <>
+__device__ inline const float* mo_cache<float>(const DevWalkers& W) { return W.mof; }
+template <typename TM>
+__device__ inline const TM* inv_store(const DevWalkers& W);
+template <>
+__device__ inline const double* inv_store<double>(const DevWalkers& W) { return W.inv; }
+template <>
+__device__ inline const float* inv_store<float>(const DevWalkers& W) { return W.invf; }
+
+template <typename TM>
 __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs A) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, w = blockIdx.x;
@@ -391,11 +408,11 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
   double* lD = gD + 3 * ne;
   double* chi = lD + ne;
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
-  const double* mog = W.mo + (size_t)w * ne * 5 * nsp;
+  const TM* mog = mo_cache<TM>(W) + (size_t)w * ne * 5 * nsp;
   // determinant gradient / laplacian per electron: lane k owns row k of its spin's inverse
   for (int s = 0; s < 2; s++) {
     const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0;
-    const double* ig = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
+    const TM* ig = inv_store<TM>(W) + ((size_t)w * 2 + s) * nsp * nsp;
     for (int k = lane; k < n; k += kWave) {
       const int e = off + k;
       double t1 = 0, t2 = 0, t3 = 0, t4 = 0;
@@ -515,7 +532,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
           double gtmp[3];
           const double u_old = jastrow_e_val(S, ee, re, pos, lane);
           (void)gtmp;
-          const double ivk = lane < n ? W.inv[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
+          const double ivk = lane < n ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
           double nl = 0;
           for (int q = 0; q < P.aip; q++) {
             double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];

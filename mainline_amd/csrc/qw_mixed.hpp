@@ -1,14 +1,389 @@
-// Mixed-precision path: FP32 inverse and MO cache, BF16 split-precision or FP32 LCAO,
-// FP64 LU refresh, log-determinant and sums.
+// Mixed-precision path.
+//
+//   inverse of each spin channel and the MO cache in FP32; the inverse is held in VGPRs for a
+//   sweep and updated by FP32 Sherman-Morrison;
+//   LCAO either FP32 on the VALU (lane = MO) or on MFMA (v_mfma_f32_16x16x32_bf16) with the
+//   operands split into BF16 terms and FP32 accumulation: x = x0 + x1 (+ x2), products kept
+//   down to the chosen order (3 products: "bf16x3", 6 products: "bf16x6"; Henry, Tang and
+//   Heinecke, "Leveraging the bfloat16 artificial intelligence datatype for higher-precision
+//   computations", ARITH 2019; error-compensated splitting as in Ootomo and Yokota, 2022);
+//   basis splines and Jastrow pair terms in FP32 (distances from FP64 positions);
+//   FP64: positions, the periodic LU recompute of each inverse (every `refresh` moves),
+//   log|det| accumulation, local-energy sums, DMC weights and branching.
 #pragma once
 #include "qw_kernels.hpp"
 
 struct qw_engine;
 
 namespace qw {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+enum LcaoMode { LCAO_FP32 = 0, LCAO_BF16X3 = 1, LCAO_BF16X6 = 2 };
+
+// Device tables of the mixed path.
 struct MixedTables {
   int ready;
+  int ldchi;                  // basis functions per LDS row (nb rounded up to 32)
+  int nkstep;                 // ceil(nb / 32)
+  int ntile[2];               // ceil(nspin / 16)
+  const float* spline;        // [ntab][nint][4] FP32
+  const uint4* Cfrag[2][3];   // per spin, split term: [kstep][tile][64 lanes] x 8 bf16 (16 bytes)
+  const float* Cf[2];         // dense FP32 [nb][nspin]
 };
+
+__device__ inline __bf16 to_bf16(float x) { return (__bf16)x; }
+
+// Split x into up to three BF16 terms with x ~= t0 + t1 + t2.
+__device__ inline void split3(float x, __bf16& t0, __bf16& t1, __bf16& t2) {
+  t0 = to_bf16(x);
+  float r = x - (float)t0;
+  t1 = to_bf16(r);
+  r = r - (float)t1;
+  t2 = to_bf16(r);
+}
+
+// FP32 basis values of shells [sh0, sh1) at point p (FP64), written as FP32 to chi[c*ld + f].
+__device__ inline void basis_lap_f32(const DevSys& S, const MixedTables& M, const double p[3], int sh0, int sh1,
+                                     int f0, float* chi, int ld, int lane) {
+  for (int sh = sh0 + lane; sh < sh1; sh += kWave) {
+    const ShellDesc d = S.shells[sh];
+    const double* ap = S.atom_pos + 3 * d.atom;
+    float x = (float)(p[0] - ap[0]), y = (float)(p[1] - ap[1]), z = (float)(p[2] - ap[2]);
+    float r = sqrtf(x * x + y * y + z * z);
+    float* o = chi + (d.f0 - f0);
+    if (r < (float)d.rcut && r < (float)d.objthr) {
+      float inv = (float)S.tab_inv[d.table];
+      int iv = (int)(r * inv);
+      iv = iv < 0 ? 0 : (iv >= S.nint ? S.nint - 1 : iv);
+      const float4 c = *(const float4*)(M.spline + ((size_t)d.table * S.nint + iv) * 4);
+      float h = r - iv * (float)S.tab_sp[d.table];
+      float f = c.x + h * (c.y + h * (c.z + h * c.w));
+      float fd = (c.y + h * (2.f * c.z + h * (3.f * c.w))) / r;
+      float f2 = 2.f * c.z + 6.f * h * c.w;
+      shell_lap<float>(d.sym, f, fd, f2, x, y, z, o, ld);
+    } else {
+      for (int i = 0; i < d.nf; i++)
+        for (int c5 = 0; c5 < 5; c5++) o[c5 * ld + i] = 0.f;
+    }
+  }
+}
+
+// FP32 Jastrow terms of electron e at p: value, gradient (and laplacian when lap != nullptr).
+__device__ inline float jobj_lap_f(const JObj& o, const double* cf, float r, float& u, float& gf, float& lap) {
+  if (r >= (float)o.rcut) return 0.f;
+  if (o.kind == 0) {
+    float rci = (float)o.rcutinv, g = (float)o.gamma;
+    float zz = r * rci, zz2 = zz * zz, pp = zz - zz2 + zz * zz2 * (1.f / 3.f);
+    float pade = 1.f / (1.f + g * pp), pade2 = pade * pade;
+    float ppd = 1.f - 2.f * zz + zz2, ppdd = -2.f + 2.f * zz;
+    float dadr = ppd * pade2 / r;
+    float dadr2 = ppdd * pade2 * rci - 2.f * g * ppd * ppd * pade2 * pade * rci + 2.f * dadr;
+    float c = (float)cf[0], cu = (float)o.cusp;
+    u += c * (cu * (float)o.rcut * (pp * pade - (float)o.pade0));
+    gf += c * (cu * dadr);
+    lap += c * (cu * dadr2);
+    return 0.f;
+  }
+  float rc = (float)o.rcut;
+  float zz = r / rc, zz2 = zz * zz, zpp = zz2 * (6.f - 8.f * zz + 3.f * zz2), zpp1 = 1.f - zpp;
+  float zppd = 12.f * zz * (1.f - 2.f * zz + zz2), zppdd2 = 6.f - 24.f * zz + 18.f * zz2;
+  float irr = 1.f / (r * rc), irc2 = 1.f / (rc * rc);
+  for (int i = 0; i < o.nf; i++) {
+    float c = (float)cf[i];
+    if (c == 0.f) continue;
+    float b = (float)o.beta[i];
+    float zpade = 1.f / (1.f + b * zpp), zpade2 = zpade * zpade;
+    float crsd = -zppd * zpade2 * (b + 1.f) * irr;
+    float crsdd2 = -zpade2 * (zppdd2 - b * zppd * zppd * zpade) * (b + 1.f) * irc2;
+    u += c * (zpp1 * zpade);
+    gf += c * crsd;
+    lap += c * (2.f * (crsdd2 + crsd));
+  }
+  return 0.f;
+}
+
+__device__ inline float jastrow_e_f(const DevSys& S, int e, const double p[3], const double* pos, float g[3], int lane) {
+  float u = 0, gx = 0, gy = 0, gz = 0, l = 0;
+  for (int a = lane; a < S.na; a += kWave) {
+    const double* ap = S.atom_pos + 3 * a;
+    float x = (float)(p[0] - ap[0]), y = (float)(p[1] - ap[1]), z = (float)(p[2] - ap[2]);
+    float r = sqrtf(x * x + y * y + z * z);
+    float gf = 0;
+    for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_lap_f(S.ei[o], S.ei[o].cl, r, u, gf, l);
+    gx += gf * x;
+    gy += gf * y;
+    gz += gf * z;
+  }
+  const bool eup = e < S.nup;
+  for (int j = lane; j < S.ne; j += kWave) {
+    if (j == e) continue;
+    float x = (float)(p[0] - pos[3 * j]), y = (float)(p[1] - pos[3 * j + 1]), z = (float)(p[2] - pos[3 * j + 2]);
+    float r = sqrtf(x * x + y * y + z * z);
+    bool unlike = (j < S.nup) != eup;
+    float gf = 0;
+    for (int o = 0; o < S.nee; o++) jobj_lap_f(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, u, gf, l);
+    gx += gf * x;
+    gy += gf * y;
+    gz += gf * z;
+  }
+  g[0] = wsumf(gx);
+  g[1] = wsumf(gy);
+  g[2] = wsumf(gz);
+  return wsumf(u);
+}
+
+// LCAO of one point in the mixed path.  On return lane j < n holds phi[c] (c = value,
+// gradient, laplacian) of MO j.  chi: LDS tile 5 x ld floats (FP32 basis values);
+// phis: LDS 5 x 64 floats used to move MFMA accumulators to lane = MO.
+template <int MODE>
+__device__ inline void lcao_lap_mx(const DevSys& S, const MixedTables& M, int s, const double p[3], float* chi,
+                                   float* phis, float phi[5], int lane) {
+  const int n = s ? S.ndn : S.nup, ld = M.ldchi;
+  for (int c = 0; c < 5; c++) phi[c] = 0.f;
+  if (MODE == LCAO_FP32) {
+    {
+      __syncthreads();
+      basis_lap_f32(S, M, p, 0, S.nsh, 0, chi, ld, lane);
+      __syncthreads();
+      if (lane < n) {
+        const float* Cp = M.Cf[s] + lane;
+        float a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+        const int nf = S.nb;
+#pragma unroll 4
+        for (int b = 0; b < nf; b++) {
+          float c = Cp[(size_t)b * n];
+          a0 += c * chi[b];
+          a1 += c * chi[ld + b];
+          a2 += c * chi[2 * ld + b];
+          a3 += c * chi[3 * ld + b];
+          a4 += c * chi[4 * ld + b];
+        }
+        phi[0] += a0;
+        phi[1] += a1;
+        phi[2] += a2;
+        phi[3] += a3;
+        phi[4] += a4;
+      }
+    }
+    return;
+  }
+  // MFMA: D[c][j] += sum_k A[c][k] B[k][j], A = basis values (rows = 5 components, padded to
+  // 16), B = coefficient fragments pre-packed per (kstep, tile, lane).  The LDS tile holds
+  // all basis functions (ldchi >= nb).
+  __syncthreads();
+  basis_lap_f32(S, M, p, 0, S.nsh, 0, chi, ld, lane);
+  __syncthreads();
+  const int row = lane & 15, kq = (lane >> 4) * 8;
+  const int ntile = M.ntile[s];
+  f32x4 acc[4];
+  for (int t = 0; t < 4; t++) acc[t] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int ks = 0; ks < M.nkstep; ks++) {
+    bf16x8 a0, a1, a2;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      int b = ks * 32 + kq + i;
+      float x = (row < 5 && b < S.nb) ? chi[row * ld + b] : 0.f;
+      __bf16 t0, t1, t2;
+      split3(x, t0, t1, t2);
+      a0[i] = t0;
+      a1[i] = t1;
+      a2[i] = t2;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (t < ntile) {
+        const size_t fi = ((size_t)ks * ntile + t) * 64 + lane;
+        uint4 q0 = M.Cfrag[s][0][fi], q1 = M.Cfrag[s][1][fi];
+        bf16x8 b0 = *reinterpret_cast<bf16x8*>(&q0), b1 = *reinterpret_cast<bf16x8*>(&q1);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[t], 0, 0, 0);
+        if (MODE == LCAO_BF16X6) {
+          uint4 q2 = M.Cfrag[s][2][fi];
+          bf16x8 b2 = *reinterpret_cast<bf16x8*>(&q2);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b2, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b0, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // accumulator layout: lane holds rows 4*(lane/16)+i (i<4), column lane%16 of each tile
+  __syncthreads();
+  const int rbase = (lane >> 4) * 4, col = lane & 15;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    if (t < ntile) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int r = rbase + i;
+        if (r < 5) phis[r * 64 + t * 16 + col] = acc[t][i];
+      }
+    }
+  }
+  __syncthreads();
+  if (lane < n)
+    for (int c = 0; c < 5; c++) phi[c] = phis[c * 64 + lane];
+}
+
+// ---------------------------------------------------------------- k_mo_rows_mx
+template <int MODE>
+__global__ __launch_bounds__(64) void k_mo_rows_mx(DevSys S, MixedTables M, DevWalkers W, int nw) {
+  extern __shared__ float ldsf[];
+  const int lane = threadIdx.x;
+  const long blk = blockIdx.x;
+  const int w = (int)(blk / S.ne), e = (int)(blk % S.ne);
+  if (w >= nw) return;
+  const int s = e < S.nup ? 0 : 1, n = s ? S.ndn : S.nup;
+  double p[3];
+  for (int d = 0; d < 3; d++) p[d] = W.pos[((size_t)w * S.ne + e) * 3 + d];
+  float phi[5];
+  lcao_lap_mx<MODE>(S, M, s, p, ldsf, ldsf + 5 * M.ldchi, phi, lane);
+  if (lane < n) {
+    float* row = W.mof + ((size_t)w * S.ne + e) * 5 * S.nsp;
+    for (int c = 0; c < 5; c++) row[c * S.nsp + lane] = phi[c];
+  }
+}
+
+// Copies the FP32 MO value rows of spin s into the FP64 staging rows read by the LU kernel.
+__global__ __launch_bounds__(256) void k_widen_values(DevSys S, DevWalkers W, int nw) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t per = (size_t)S.ne * S.nsp;
+  if (i >= (size_t)nw * per) return;
+  const size_t w = i / per, r = i % per, e = r / S.nsp, j = r % S.nsp;
+  const size_t idx = ((w * S.ne + e) * 5) * S.nsp + j;
+  W.mo[idx] = (double)W.mof[idx];
+}
+
+// ---------------------------------------------------------------- k_sweep_mx
+template <int NS, int DEPTH, int MODE>
+__global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W, SweepArgs A) {
+  extern __shared__ double ldsd[];
+  const int lane = threadIdx.x, w = blockIdx.x;
+  if (w >= W.nw) return;
+  const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
+  double* pos = ldsd;
+  float* chi = (float*)(pos + 3 * ne);
+  float* phis = chi + 5 * M.ldchi;
+  float* rowb = phis + 5 * 64;
+  float* phib = rowb + NS;
+  for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  float inv[NS];
+  const float* ig = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
+#pragma unroll
+  for (int j = 0; j < NS; j++) inv[j] = (lane < n && j < n) ? ig[(size_t)lane * nsp + j] : 0.f;
+  double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
+  int nacc = 0;
+  float* mog = W.mof + (size_t)w * ne * 5 * nsp;
+  const double tau = A.tau, stau = sqrt(tau);
+  __syncthreads();
+  for (int k = A.k0; k < A.k1; k++) {
+    const int e = off + k;
+    double r0[3] = {pos[3 * e], pos[3 * e + 1], pos[3 * e + 2]};
+    if (lane == k) {
+#pragma unroll
+      for (int j = 0; j < NS; j++) rowb[j] = inv[j];
+    }
+    __syncthreads();
+    const float rk = lane < n ? rowb[lane] : 0.f;
+    float m1 = 0, m2 = 0, m3 = 0;
+    if (lane < n) {
+      m1 = mog[((size_t)e * 5 + 1) * nsp + lane];
+      m2 = mog[((size_t)e * 5 + 2) * nsp + lane];
+      m3 = mog[((size_t)e * 5 + 3) * nsp + lane];
+    }
+    float gj0[3];
+    const float u0 = jastrow_e_f(S, e, r0, pos, gj0, lane);
+    double d0[3] = {(double)(wsumf(rk * m1) + gj0[0]), (double)(wsumf(rk * m2) + gj0[1]), (double)(wsumf(rk * m3) + gj0[2])};
+    float phi[5];
+    double r1[3], d1[3], ln1 = 0;
+    float R = 0;
+    bool accepted = false;
+    double acc_first = 0;
+    for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
+      const double t = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
+      double g[3], dr[3], rn[3], dn[3];
+      gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      lim_drift(d0, t, dr);
+      for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
+      lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, phi, lane);
+      float t0 = wsumf(rk * phi[0]);
+      float t1 = wsumf(rk * phi[1]), t2 = wsumf(rk * phi[2]), t3 = wsumf(rk * phi[3]);
+      float gj[3];
+      float un = jastrow_e_f(S, e, rn, pos, gj, lane);
+      float it0 = 1.f / t0;
+      dn[0] = (double)(t1 * it0 + gj[0]);
+      dn[1] = (double)(t2 * it0 + gj[1]);
+      dn[2] = (double)(t3 * it0 + gj[2]);
+      double lnn = log(fabs((double)t0)) + (double)(un - u0);
+      bool neg = t0 < 0;
+      double acc;
+      if (dep == 1) {
+        double rr = exp(lnn);
+        acc = (A.restrict_nodes && neg) ? 0.0 : fmin(1.0, rr * rr * exp(trans(rn, dn, r0, t) - trans(r0, d0, rn, t)));
+        acc_first = acc;
+        for (int d = 0; d < 3; d++) r1[d] = rn[d], d1[d] = dn[d];
+        ln1 = lnn;
+      } else {
+        double rr = exp(lnn);
+        double pt = (trans(rn, dn, r1, tau) - trans(r0, d0, r1, tau)) + (trans(rn, dn, r0, t) - trans(r0, d0, rn, t));
+        double r21 = exp(ln1 - lnn);
+        double a21 = fmin(1.0, r21 * r21 * exp(trans(r1, d1, rn, tau) - trans(rn, dn, r1, tau)));
+        double num = 1 - a21, den = 1 - acc_first;
+        if (A.restrict_nodes && neg) acc = 0;
+        else if (fabs(den) < 1e-10) acc = fabs(num) < 1e-10 ? 0.0 : 1.0;
+        else acc = fmin(1.0, rr * rr * exp(pt) * num / den);
+      }
+      double u = uniform(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
+      if (acc + u > 1.0) {
+        accepted = true;
+        R = t0;
+        for (int d = 0; d < 3; d++) r1[d] = rn[d];
+      }
+    }
+    if (accepted) {
+      if (lane < n) phib[lane] = phi[0];
+      __syncthreads();
+      if (lane < n) {
+        float tk = 0;
+#pragma unroll
+        for (int j = 0; j < NS; j++)
+          if (j < n) tk += inv[j] * phib[j];
+        if (lane == k) {
+          const float f = 1.f / R;
+#pragma unroll
+          for (int j = 0; j < NS; j++) inv[j] = rowb[j] * f;
+        } else {
+          const float pk = tk * (-1.f / R);
+#pragma unroll
+          for (int j = 0; j < NS; j++)
+            if (j < n) inv[j] += rowb[j] * pk;
+        }
+        for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + lane] = phi[c];
+      }
+      logdet += log(fabs((double)R));
+      if (R < 0) sgn = -sgn;
+      nacc++;
+      if (lane < 3) pos[3 * e + lane] = r1[lane];
+    }
+    __syncthreads();
+  }
+  float* igw = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
+  if (lane < n) {
+#pragma unroll
+    for (int j = 0; j < NS; j++)
+      if (j < n) igw[(size_t)lane * nsp + j] = inv[j];
+  }
+  for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
+  if (lane == 0) {
+    W.logdet[2 * w + s] = logdet;
+    W.sgn[2 * w + s] = sgn;
+    W.acc[w] += nacc;
+  }
+}
+
 }  // namespace qw
 
 void mixed_setup(qw_engine& e);

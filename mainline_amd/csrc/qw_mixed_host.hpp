@@ -1,11 +1,153 @@
 // Host side of the mixed-precision path (included by qw_engine.hip after qw_engine).
 #pragma once
+#include <cstdint>
 
-void mixed_setup(qw_engine& e) { throw std::runtime_error("mixed precision is not available in this build"); }
-void mixed_refresh_all(qw_engine& e) { throw std::runtime_error("mixed precision is not available in this build"); }
-void mixed_sweep(qw_engine& e, const qw::SweepArgs& A, int depth) {
-  throw std::runtime_error("mixed precision is not available in this build");
+namespace {
+
+// Round-to-nearest-even BF16 of a float (returned as the raw 16-bit pattern).
+uint16_t bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(r >> 16);
 }
-void mixed_local(qw_engine& e, const qw::LocalArgs& A) {
-  throw std::runtime_error("mixed precision is not available in this build");
+float bf16_val(uint16_t b) {
+  uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+int lcao_mode(int precision) {
+  return precision == QW_PREC_MIXED_FP32 ? LCAO_FP32 : (precision == QW_PREC_MIXED_BF16X6 ? LCAO_BF16X6 : LCAO_BF16X3);
+}
+
+}  // namespace
+
+void mixed_setup(qw_engine& e) {
+  const SysTables& T = e.T;
+  MixedTables& M = e.MT;
+  M.ldchi = (T.nb + 31) / 32 * 32;
+  M.nkstep = M.ldchi / 32;
+  std::vector<float> spf(T.spline.begin(), T.spline.end());
+  M.spline = e.upload(spf);
+  for (int s = 0; s < 2; s++) {
+    const int n = s ? T.ndn : T.nup;
+    M.ntile[s] = (n + 15) / 16;
+    std::vector<float> cf(T.C[s].begin(), T.C[s].end());
+    M.Cf[s] = e.upload(cf);
+    // B fragments of v_mfma_f32_16x16x32_bf16: lane l holds rows k = 8*(l/16)+i, column l%16
+    const size_t nfrag = (size_t)M.nkstep * std::max(M.ntile[s], 1) * 64;
+    std::vector<uint16_t> part[3];
+    for (auto& v : part) v.assign(nfrag * 8, 0);
+    for (int ks = 0; ks < M.nkstep; ks++)
+      for (int t = 0; t < M.ntile[s]; t++)
+        for (int l = 0; l < 64; l++) {
+          const int j = t * 16 + (l & 15);
+          for (int i = 0; i < 8; i++) {
+            const int b = ks * 32 + (l >> 4) * 8 + i;
+            double c = (b < T.nb && j < n) ? T.C[s][(size_t)b * n + j] : 0.0;
+            const size_t at = (((size_t)ks * M.ntile[s] + t) * 64 + l) * 8 + i;
+            for (int q = 0; q < 3; q++) {
+              uint16_t bb = bf16_bits((float)c);
+              part[q][at] = bb;
+              c -= (double)bf16_val(bb);
+            }
+          }
+        }
+    for (int q = 0; q < 3; q++) {
+      std::vector<uint4> packed(nfrag);
+      std::memcpy(packed.data(), part[q].data(), nfrag * 16);
+      M.Cfrag[s][q] = e.upload(packed);
+    }
+  }
+  M.ready = 1;
+}
+
+namespace {
+size_t lds_mx_rows(const qw_engine& e) { return (5 * (size_t)e.MT.ldchi + 5 * 64) * sizeof(float); }
+size_t lds_mx_sweep(const qw_engine& e) {
+  return 3 * (size_t)e.S.ne * sizeof(double) + (5 * (size_t)e.MT.ldchi + 5 * 64 + 2 * (size_t)e.ns) * sizeof(float);
+}
+
+// FP64 LU recompute of the FP32 inverses of spin `only_spin` (or both when -1) from the
+// current FP32 MO value rows.
+void mixed_lu(qw_engine& e, int only_spin) {
+  const size_t tot = (size_t)e.nw * e.S.ne * e.S.nsp;
+  e.launch("k_widen", [&] {
+    hipLaunchKernelGGL(k_widen_values, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e.stream, e.S, e.W, e.nw);
+  });
+  auto go = [&](auto kern) {
+    e.launch("k_lu", [&] {
+      hipLaunchKernelGGL(kern, dim3(only_spin < 0 ? 2 * e.nw : e.nw), dim3(64), 0, e.stream, e.S, (const double*)e.W.mo,
+                         e.W.invf, e.W.logdet, e.W.sgn, e.nw, only_spin);
+    });
+  };
+  switch (e.ns) {
+    case 4: go(k_lu<4, float>); break;
+    case 8: go(k_lu<8, float>); break;
+    case 16: go(k_lu<16, float>); break;
+    case 32: go(k_lu<32, float>); break;
+    default: go(k_lu<64, float>); break;
+  }
+}
+
+template <int MODE>
+void mixed_rows_t(qw_engine& e) {
+  e.launch("k_mo_rows", [&] {
+    hipLaunchKernelGGL(k_mo_rows_mx<MODE>, dim3((unsigned)((size_t)e.nw * e.S.ne)), dim3(64), lds_mx_rows(e), e.stream,
+                       e.S, e.MT, e.W, e.nw);
+  });
+}
+
+template <int DEPTH, int MODE>
+void mixed_sweep_t(qw_engine& e, const SweepArgs& A) {
+  auto go = [&](auto kern) {
+    e.launch("k_sweep", [&] { hipLaunchKernelGGL(kern, dim3(e.nw), dim3(64), lds_mx_sweep(e), e.stream, e.S, e.MT, e.W, A); });
+  };
+  switch (e.ns) {
+    case 4: go(k_sweep_mx<4, DEPTH, MODE>); break;
+    case 8: go(k_sweep_mx<8, DEPTH, MODE>); break;
+    case 16: go(k_sweep_mx<16, DEPTH, MODE>); break;
+    case 32: go(k_sweep_mx<32, DEPTH, MODE>); break;
+    default: go(k_sweep_mx<64, DEPTH, MODE>); break;
+  }
+}
+}  // namespace
+
+void mixed_refresh_all(qw_engine& e) {
+  switch (lcao_mode(e.precision)) {
+    case LCAO_FP32: mixed_rows_t<LCAO_FP32>(e); break;
+    case LCAO_BF16X6: mixed_rows_t<LCAO_BF16X6>(e); break;
+    default: mixed_rows_t<LCAO_BF16X3>(e); break;
+  }
+  mixed_lu(e, -1);
+}
+
+// Moves of electrons [k0, k1) of spin s, with an FP64 LU recompute of that spin's inverse
+// after every `refresh` moves.
+void mixed_sweep(qw_engine& e, const SweepArgs& A0, int depth) {
+  const int mode = lcao_mode(e.precision);
+  for (int k0 = A0.k0; k0 < A0.k1; k0 += e.refresh) {
+    SweepArgs A = A0;
+    A.k0 = k0;
+    A.k1 = std::min(A0.k1, k0 + e.refresh);
+    if (depth == 1) {
+      if (mode == LCAO_FP32) mixed_sweep_t<1, LCAO_FP32>(e, A);
+      else if (mode == LCAO_BF16X6) mixed_sweep_t<1, LCAO_BF16X6>(e, A);
+      else mixed_sweep_t<1, LCAO_BF16X3>(e, A);
+    } else {
+      if (mode == LCAO_FP32) mixed_sweep_t<2, LCAO_FP32>(e, A);
+      else if (mode == LCAO_BF16X6) mixed_sweep_t<2, LCAO_BF16X6>(e, A);
+      else mixed_sweep_t<2, LCAO_BF16X3>(e, A);
+    }
+    mixed_lu(e, A.s);
+  }
+}
+
+void mixed_local(qw_engine& e, const LocalArgs& A) {
+  e.launch("k_local", [&] {
+    hipLaunchKernelGGL(k_local<float>, dim3(e.nw), dim3(64), e.lds_local(), e.stream, e.S, e.W, A);
+  });
 }

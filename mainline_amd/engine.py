@@ -11,7 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libqwalk_amd.so")
 
-PREC = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16": 2}
+PREC = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16": 2, "mixed_bf16x6": 3}
 STATUS = {0: "ok", -1: "bad argument", -2: "bad input", -3: "unsupported", -4: "device error", -5: "out of memory"}
 
 _dp = ctypes.POINTER(ctypes.c_double)
