@@ -62,6 +62,24 @@ def run_gpu(deck, mol, precision, nw, steps, warmup, tau):
                 acceptance=float(res["acceptance"]), kernels={k: [v[0], int(v[1])] for k, v in kt.items()})
 
 
+def accuracy_leg(deck, mol, precs, nw, nblock, nstep, tau):
+    """DMC of every precision from the same walkers with the same random streams:
+    block-averaged energies, their mean and standard error per precision."""
+    from mainline_amd import engine as E
+    from mainline_amd import synth
+    pos = synth.random_configs(mol, nw, seed=9)
+    out = {}
+    for p in precs:
+        eng = E.Engine(deck, p, seed=99)
+        eng.set_walkers(pos)
+        eng.run(E.method("DMC", nw, nblock=1, nstep=nstep, timestep=tau))  # equilibration block
+        eb = eng.run(E.method("DMC", nw, nblock=nblock, nstep=nstep, timestep=tau))["eblock"][:, 0]
+        out[p] = dict(mean=float(eb.mean()), err=float(eb.std(ddof=1) / np.sqrt(len(eb))), blocks=eb.tolist())
+        log("[accuracy %s] E = %.5f +- %.5f" % (p, out[p]["mean"], out[p]["err"]))
+        eng.close()
+    return out
+
+
 def cpu_baseline(deck, mol, seconds_target):
     """The oracle's FP64 CPU engine (same algorithm, threads over walkers) on a bounded sample."""
     from oracle import oracle as O
@@ -91,6 +109,7 @@ def main():
     ap.add_argument("--precision", default="auto", help="fp64, mixed_fp32, mixed_bf16 or auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-blocks", type=int, default=6, help="0 disables the energy-agreement leg")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the walker engine runs on a single MI355X in this build; "
@@ -102,7 +121,7 @@ def main():
     deck = mol.write(tmp)
     tau = 0.01
     results = {}
-    precs = ["fp64", "mixed_bf16"] if a.precision == "auto" else [a.precision]
+    precs = ["fp64", "mixed_fp32", "mixed_bf16", "mixed_bf16x6"] if a.precision == "auto" else [a.precision]
     for p in precs:
         try:
             results[p] = run_gpu(deck, mol, p, a.walkers, a.steps, a.warmup, tau)
@@ -137,9 +156,17 @@ def main():
         "roofline": roof, "cpu_baseline": cpu,
         "fp64": {k: results["fp64"][k] for k in ("moves_per_s", "ms_per_step", "energy", "acceptance")},
     }
-    if "mixed_bf16" in results:
-        line["mixed_bf16"] = {k: results["mixed_bf16"][k] for k in ("moves_per_s", "ms_per_step", "energy", "acceptance")}
-        line["mixed_over_fp64"] = results["mixed_bf16"]["moves_per_s"] / results["fp64"]["moves_per_s"]
+    for p in precs[1:]:
+        if p in results:
+            line[p] = {k: results[p][k] for k in ("moves_per_s", "ms_per_step", "energy", "acceptance")}
+            line[p]["over_fp64"] = results[p]["moves_per_s"] / results["fp64"]["moves_per_s"]
+    if a.accuracy_blocks > 0:
+        acc = accuracy_leg(deck, mol, list(results), a.walkers, a.accuracy_blocks, 10, tau)
+        ref = acc["fp64"]
+        for p, v in acc.items():
+            v["delta_vs_fp64"] = v["mean"] - ref["mean"]
+            v["combined_err"] = float(np.hypot(v["err"], ref["err"]))
+        line["energy_agreement"] = acc
     line["kernels"] = R["kernels"]
     print(json.dumps(line))
 

@@ -98,9 +98,22 @@ qw_status qw_method_read(const char* deck_path, qw_method_desc* out);
 qw_status qw_walkers_set(qw_engine* e, int nw, const double* pos, const double* weights);
 qw_status qw_walkers_get(qw_engine* e, int nw, double* pos, double* weights);
 
+/* Fields of one block record written by qw_run (the quantities of a QWalk block log,
+ * src/properties/Properties.cpp:578-641, src/properties/Properties_block.cpp:73-138):
+ *   0 E, 1 kinetic, 2 Coulomb, 3 nonlocal      weighted block averages
+ *   4 weight sum, 5 points (walker-steps)
+ *   6..9 per-point variances of E, kinetic, Coulomb, nonlocal
+ *   10 mean weight, 11 weight variance */
+#define QW_BLOCK_FIELDS 12
+
+/* Random initial walkers as the reference's Molecular_sample::randomGuess
+ * (src/system/Molecular_sample.cpp:194-260): electrons of each spin in cubes of half-width
+ * inirange around the atoms by charge, the rest around the centroid.  pos[nw][Ne][3]. */
+qw_status qw_walkers_random(const qw_engine* e, int nw, unsigned long long seed, double inirange, double* pos);
+
 /* Run VMC or DMC on the resident walkers.
- *   eblock[nblock][5] = weighted block averages of E, kinetic, Coulomb, nonlocal, and the
- *   block weight sum; stats[4] = acceptance, electron moves done, seconds, electron-moves/s. */
+ *   eblock[nblock][QW_BLOCK_FIELDS] block records (see above);
+ *   stats[4] = acceptance, electron moves done, seconds, electron-moves/s. */
 qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* eblock, double* stats);
 
 /* Per-kernel device time of the last qw_run: names (comma separated) into buf, and

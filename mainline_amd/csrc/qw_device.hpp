@@ -46,7 +46,7 @@ struct DevWalkers {
   double* eold;    // [nw] local energy of the previous step
   double* elast;   // [nw][4] last kinetic, Coulomb, nonlocal, total
   int* acc;        // [nw] accepted moves since the last reset
-  double* bsum;    // [nw][6] block sums: w, wE, wKE, wV, wNL, steps
+  double* bsum;    // [nw][16] block sums (see kBsum)
 };
 
 // ---------------------------------------------------------------- wave reductions

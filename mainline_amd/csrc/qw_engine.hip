@@ -75,13 +75,13 @@ struct qw_engine {
   std::vector<int> kcount;
 
   ~qw_engine() {
-    hipSetDevice(device);
-    if (stream) hipStreamSynchronize(stream);
-    for (void* p : sys_allocs) hipFree(p);
-    for (void* p : walker_allocs) hipFree(p);
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : sys_allocs) (void)hipFree(p);
+    for (void* p : walker_allocs) (void)hipFree(p);
     for (auto& e : evs) pool.push_back(e.a), pool.push_back(e.b);
-    for (auto ev : pool) hipEventDestroy(ev);
-    if (stream) hipStreamDestroy(stream);
+    for (auto ev : pool) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 
   template <typename T>
@@ -200,7 +200,7 @@ struct qw_engine {
     S.pp = upload(T.pptypes);
     S.atom_pp = upload(T.atom_pp);
     dctl = upload(std::vector<DmcCtl>(1));
-    dred = upload(std::vector<double>(16));
+    dred = upload(std::vector<double>(kBsum + 1));
   }
 
   void ensure_walkers(int n) {
@@ -210,7 +210,7 @@ struct qw_engine {
       return;
     }
     HIPCHK(hipStreamSynchronize(stream));
-    for (void* p : walker_allocs) hipFree(p);
+    for (void* p : walker_allocs) (void)hipFree(p);
     walker_allocs.clear();
     const size_t ne = S.ne, nsp = S.nsp;
     W.pos = walloc<double>((size_t)n * ne * 3);
@@ -232,7 +232,7 @@ struct qw_engine {
     W.eold = walloc<double>(n);
     W.elast = walloc<double>((size_t)n * 4);
     W.acc = walloc<int>(n);
-    W.bsum = walloc<double>((size_t)n * 6);
+    W.bsum = walloc<double>((size_t)n * kBsum);
     int P = 1;
     while (P < n) P <<= 1;
     keys = walloc<unsigned long long>(P);
@@ -311,9 +311,9 @@ struct qw_engine {
     }
     launch("k_local", [&] { hipLaunchKernelGGL(k_local<double>, dim3(nw), dim3(64), lds_local(), stream, S, W, A); });
   }
-  void reduce(double* host7, int reset) {
+  void reduce(double* host, int reset) {
     launch("k_reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, stream, W, dred, reset); });
-    HIPCHK(hipMemcpyAsync(host7, dred, 7 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(host, dred, (kBsum + 1) * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
 };
@@ -409,13 +409,69 @@ extern "C" qw_status qw_walkers_set(qw_engine* e, int n, const double* pos, cons
     std::vector<double> w(n, 1.0);
     if (weights) std::copy(weights, weights + n, w.begin());
     HIPCHK(hipMemcpyAsync(e->W.weight, w.data(), n * sizeof(double), hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)n * 6 * sizeof(double), e->stream));
+    HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)n * kBsum * sizeof(double), e->stream));
     HIPCHK(hipMemsetAsync(e->W.acc, 0, (size_t)n * sizeof(int), e->stream));
     e->refresh_all();
     HIPCHK(hipStreamSynchronize(e->stream));
     e->dmc_ready = false;
     return QW_OK;
   });
+}
+
+namespace {
+// Host Philox4x32-10 (same streams as the device helper) for walker initialisation.
+void philox_host(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0, unsigned k1, unsigned out[4]) {
+  for (int r = 0; r < 10; r++) {
+    unsigned long long p0 = 0xD2511F53ull * c0, p1 = 0xCD9E8D57ull * c2;
+    unsigned n0 = (unsigned)(p1 >> 32) ^ c1 ^ k0, n2 = (unsigned)(p0 >> 32) ^ c3 ^ k1;
+    c0 = n0;
+    c1 = (unsigned)p1;
+    c2 = n2;
+    c3 = (unsigned)p0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+}  // namespace
+
+extern "C" qw_status qw_walkers_random(const qw_engine* e, int nw, unsigned long long seed, double inirange,
+                                       double* pos) {
+  if (!e || nw <= 0 || !pos || inirange <= 0) return QW_ERR_ARG;
+  const SysTables& T = e->T;
+  const int ne = T.ne(), na = T.na;
+  double cen[3] = {0, 0, 0};
+  for (int a = 0; a < na; a++)
+    for (int d = 0; d < 3; d++) cen[d] += T.atom_pos[3 * a + d] / na;
+  for (int w = 0; w < nw; w++) {
+    double* p = pos + (size_t)w * ne * 3;
+    auto u3 = [&](int el, double* out) {
+      unsigned o[4];
+      philox_host((unsigned)w, (unsigned)el, 0u, 7u, (unsigned)seed, (unsigned)(seed >> 32), o);
+      for (int d = 0; d < 3; d++) out[d] = (o[d] + 0.5) * (1.0 / 4294967296.0);
+    };
+    int el = 0;
+    for (int spin = 0; spin < 2; spin++)
+      for (int a = 0; a < na; a++) {
+        const int zi = (int)T.atom_Z[a];
+        const int nplace = (int)(T.atom_Z[a] / 2) + ((1 - spin) * zi) % 2;
+        for (int k = 0; k < nplace; k++, el++) {
+          if (el >= ne) continue;
+          double u[3];
+          u3(el, u);
+          for (int d = 0; d < 3; d++) p[3 * el + d] = T.atom_pos[3 * a + d] + 2 * inirange * (u[d] - .5);
+        }
+      }
+    for (; el < ne; el++) {
+      double u[3];
+      u3(el, u);
+      for (int d = 0; d < 3; d++) p[3 * el + d] = 4 * inirange * (u[d] - .5) + cen[d];
+    }
+  }
+  return QW_OK;
 }
 
 extern "C" qw_status qw_walkers_get(qw_engine* e, int n, double* pos, double* weights) {
@@ -450,7 +506,7 @@ extern "C" qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const
     struct Free {
       std::vector<void*>& v;
       ~Free() {
-        for (void* p : v) hipFree(p);
+        for (void* p : v) (void)hipFree(p);
       }
     } fr{tmp};
     double* d_rot = nullptr;
@@ -533,10 +589,10 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
     A.pp_mode = m->pp_stochastic ? 1 : 0;
     A.seed = e->seed;
     A.ctl = e->dctl;
-    HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)nw * 6 * sizeof(double), e->stream));
+    HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)nw * kBsum * sizeof(double), e->stream));
     HIPCHK(hipMemsetAsync(e->W.acc, 0, (size_t)nw * sizeof(int), e->stream));
     DmcCtl ctl{};
-    double red[7];
+    double red[kBsum + 1];
     if (dmc && e->dmc_ready) {
       // continuing a DMC run on the same population: keep E_ref, E_T, cutoffs, tau_eff
       ctl = e->ctl_host;
@@ -606,13 +662,22 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
         }
       }
       e->reduce(red, 1);
-      double sw = red[0];
+      double sw = red[0], np_ = red[5];
       if (eblock) {
-        eblock[5 * b + 0] = red[1] / sw;
-        eblock[5 * b + 1] = red[2] / sw;
-        eblock[5 * b + 2] = red[3] / sw;
-        eblock[5 * b + 3] = red[4] / sw;
-        eblock[5 * b + 4] = sw;
+        double* eb = eblock + (size_t)QW_BLOCK_FIELDS * b;
+        auto var = [&](int c) { return red[c + 1] / np_ - (red[c] / np_) * (red[c] / np_); };
+        eb[0] = red[1] / sw;
+        eb[1] = red[2] / sw;
+        eb[2] = red[3] / sw;
+        eb[3] = red[4] / sw;
+        eb[4] = sw;
+        eb[5] = np_;
+        eb[6] = var(6);
+        eb[7] = var(8);
+        eb[8] = var(10);
+        eb[9] = var(12);
+        eb[10] = sw / np_;
+        eb[11] = red[14] / np_ - (sw / np_) * (sw / np_);
       }
       bavg.push_back(red[1] / sw);
       if (dmc) {
@@ -622,7 +687,7 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
         HIPCHK(hipMemcpyAsync(&ctl, e->dctl, sizeof(DmcCtl), hipMemcpyDeviceToHost, e->stream));
         HIPCHK(hipStreamSynchronize(e->stream));
         ctl.eref = s / bavg.size();
-        ctl.etrial = ctl.eref - m->feedback * std::log(red[6] / nw);
+        ctl.etrial = ctl.eref - m->feedback * std::log(red[kBsum] / nw);
         HIPCHK(hipMemcpyAsync(e->dctl, &ctl, sizeof(DmcCtl), hipMemcpyHostToDevice, e->stream));
       }
     }

@@ -18,6 +18,10 @@
 
 namespace qw {
 
+// Per-walker block accumulators: w, wE, wKE, wV, wNL, steps, E, E^2, KE, KE^2, V, V^2, NL,
+// NL^2, w^2, (spare).
+constexpr int kBsum = 16;
+
 // Scalars of the DMC control loop kept on the device (read by k_local, written by k_branch).
 struct DmcCtl {
   double etrial, teff, cut_start, cut_stop, max_weight, eref, feedback, tau;
@@ -585,13 +589,23 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
         W.weight[w] = wt;
       }
       W.eold[w] = etot;
-      double* bs = W.bsum + 6 * (size_t)w;
+      double* bs = W.bsum + kBsum * (size_t)w;
       bs[0] += wt;
       bs[1] += wt * etot;
       bs[2] += wt * ke;
       bs[3] += wt * vloc;
       bs[4] += wt * vnl;
       bs[5] += 1;
+      // unweighted first and second moments (per-point variances of the block log)
+      bs[6] += etot;
+      bs[7] += etot * etot;
+      bs[8] += ke;
+      bs[9] += ke * ke;
+      bs[10] += vloc;
+      bs[11] += vloc * vloc;
+      bs[12] += vnl;
+      bs[13] += vnl * vnl;
+      bs[14] += wt * wt;
     }
   }
 }
@@ -718,25 +732,28 @@ __global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcC
   if (t == 0) W.eold[dst] = W.eold[src];
 }
 
-// Block sums over walkers: out[0..5] = sums of bsum columns, out[6] = current weight sum.
+// Block sums over walkers: out[0..kBsum-1] = sums of the bsum columns, out[kBsum] = current
+// weight sum.
 __global__ __launch_bounds__(256) void k_reduce(DevWalkers W, double* out, int reset) {
-  __shared__ double red[7][256];
+  constexpr int NC = kBsum + 1;
+  __shared__ double red[NC][256];
   const int t = threadIdx.x;
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  double acc[NC];
+  for (int c = 0; c < NC; c++) acc[c] = 0;
   for (int i = t; i < W.nw; i += blockDim.x) {
-    for (int c = 0; c < 6; c++) acc[c] += W.bsum[6 * (size_t)i + c];
-    acc[6] += W.weight[i];
+    for (int c = 0; c < kBsum; c++) acc[c] += W.bsum[kBsum * (size_t)i + c];
+    acc[kBsum] += W.weight[i];
     if (reset)
-      for (int c = 0; c < 6; c++) W.bsum[6 * (size_t)i + c] = 0;
+      for (int c = 0; c < kBsum; c++) W.bsum[kBsum * (size_t)i + c] = 0;
   }
-  for (int c = 0; c < 7; c++) red[c][t] = acc[c];
+  for (int c = 0; c < NC; c++) red[c][t] = acc[c];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (t < o)
-      for (int c = 0; c < 7; c++) red[c][t] += red[c][t + o];
+      for (int c = 0; c < NC; c++) red[c][t] += red[c][t + o];
     __syncthreads();
   }
-  if (t < 7) out[t] = red[t][0];
+  if (t < NC) out[t] = red[t][0];
 }
 
 }  // namespace qw

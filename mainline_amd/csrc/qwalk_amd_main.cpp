@@ -1,0 +1,188 @@
+// qwalk_amd: runs the METHOD { VMC | DMC } sections of a QWalk deck on the MI355X walker engine,
+// through the C ABI only, and writes QWalk's own output files so the reference's tooling
+// (gosling, tests/qwtest.py) reads them unchanged:
+//   <deck>.log     block records (src/properties/Properties.cpp:578-641,
+//                  src/properties/Properties_block.cpp:73-138)
+//   <deck>.config  walker positions (+ DMC weight) (src/program_utils/qmc_io.h:371-423,
+//                  src/system/Sample_point.cpp:120-148, src/method/Dmc_method.cpp:1228-1265)
+// Walkers come from <deck>.config when it exists (READCONFIG), otherwise from the reference's
+// random guess followed by VMC equilibration (src/method/Generate_sample.cpp:24-71).
+//
+// usage: qwalk_amd <deck> [--precision fp64|mixed_fp32|mixed_bf16|mixed_bf16x6] [--device N]
+//                         [--seed S] [--walkers N]
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/qwalk_amd.h"
+
+namespace {
+
+int fail(const char* what, const qw_engine* e) {
+  fprintf(stderr, "qwalk_amd: %s: %s\n", what, qw_last_error(e));
+  return 1;
+}
+
+bool read_configs(const std::string& path, int ne, std::vector<double>& pos, std::vector<double>& w) {
+  std::ifstream in(path);
+  if (!in) return false;
+  std::string tok;
+  while (in >> tok) {
+    if (tok != "nElec") continue;
+    int n = 0, nd = 0;
+    std::string d;
+    in >> n >> d >> nd;
+    if (n != ne || nd != 3) return false;
+    for (int i = 0; i < 3 * n; i++) {
+      double x;
+      in >> x;
+      pos.push_back(x);
+    }
+    double wt = 1.0;
+    std::streampos at = in.tellg();
+    std::string next;
+    if (in >> next && next == "DMC") {
+      std::string br, key;
+      in >> br >> key >> wt;
+    } else {
+      in.clear();
+      in.seekg(at);
+    }
+    w.push_back(wt);
+  }
+  return !w.empty();
+}
+
+void write_configs(const std::string& path, int nw, int ne, const std::vector<double>& pos,
+                   const std::vector<double>& w, bool dmc) {
+  std::rename(path.c_str(), (path + ".backup").c_str());
+  std::ofstream os(path);
+  os.precision(15);
+  for (int i = 0; i < nw; i++) {
+    os << " walker { \n";
+    os << "nElec " << ne << " ndim 3\n";
+    for (int e = 0; e < ne; e++) {
+      const double* p = &pos[((size_t)i * ne + e) * 3];
+      os << p[0] << " " << p[1] << " " << p[2] << " \n";
+    }
+    if (dmc) os << "DMC { \nweight " << w[i] << "\nsign 1\n}\n";
+    os << " } \n";
+  }
+}
+
+void write_log(const std::string& path, const char* label, double tau, bool dmc, const std::vector<double>& eb,
+               int nblock) {
+  std::ofstream lo(path, std::ios::app);
+  lo.precision(16);
+  lo << "#-------------------------------------------------\n";
+  lo << "#" << (dmc ? "DMC" : "VMC") << " run: timestep " << tau << "\n";
+  lo << "#-------------------------------------------------\n\n\n";
+  for (int b = 0; b < nblock; b++) {
+    const double* r = &eb[(size_t)b * QW_BLOCK_FIELDS];
+    auto sec = [&](const char* name, double v) { lo << "   " << name << " { " << v << "   } \n"; };
+    lo << "block { \n";
+    lo << "   label " << label << "\n";
+    lo << "   totweight " << r[5] << "\n";
+    lo << "   nautocorr 0\n";
+    sec("total_energy", r[0]);
+    sec("total_energyvar", r[6]);
+    sec("weight", r[10]);
+    sec("weightvar", r[11]);
+    sec("kinetic_energy", r[1]);
+    sec("kinetic_energyvar", r[7]);
+    sec("potential_energy", r[2]);
+    sec("potential_energyvar", r[8]);
+    sec("nonlocal_energy", r[3]);
+    sec("nonlocal_energyvar", r[9]);
+    lo << "   autocorr_energy { } \n";
+    lo << "} \n\n";
+  }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: qwalk_amd <deck> [--precision P] [--device N] [--seed S] [--walkers N]\n");
+    return 2;
+  }
+  std::string deck = argv[1];
+  qw_engine_opts opts{QW_PREC_FP64, 0, 20261202ull, 0};
+  int walkers_override = 0;
+  for (int i = 2; i + 1 < argc; i += 2) {
+    std::string k = argv[i], v = argv[i + 1];
+    if (k == "--precision") {
+      opts.precision = v == "fp64" ? QW_PREC_FP64 : v == "mixed_fp32" ? QW_PREC_MIXED_FP32
+                       : v == "mixed_bf16x6" ? QW_PREC_MIXED_BF16X6 : QW_PREC_MIXED_BF16;
+    } else if (k == "--device") {
+      opts.device = atoi(v.c_str());
+    } else if (k == "--seed") {
+      opts.seed = strtoull(v.c_str(), nullptr, 10);
+    } else if (k == "--walkers") {
+      walkers_override = atoi(v.c_str());
+    }
+  }
+  qw_method_desc m;
+  if (qw_method_read(deck.c_str(), &m) != QW_OK) return fail("METHOD", nullptr);
+  qw_engine* e = nullptr;
+  if (qw_engine_create(deck.c_str(), &opts, &e) != QW_OK) return fail("engine", nullptr);
+  int info[6];
+  qw_engine_info(e, info);
+  const int ne = info[0] + info[1];
+  const bool dmc = m.method == 1;
+  if (walkers_override > 0) m.nconfig = walkers_override;
+  if (m.timestep <= 0) m.timestep = 0.5;  // the reference tunes it when TIMESTEP is absent
+  const std::string cfg = deck + ".config";
+  std::vector<double> pos, w;
+  bool have = read_configs(cfg, ne, pos, w);
+  int nw = m.nconfig;
+  if (have) {
+    // round-robin reuse of the stored walkers, as read_configurations distributes them
+    std::vector<double> p2((size_t)nw * ne * 3), w2(nw);
+    int nr = (int)w.size();
+    for (int i = 0; i < nw; i++) {
+      for (int k = 0; k < ne * 3; k++) p2[(size_t)i * ne * 3 + k] = pos[(size_t)(i % nr) * ne * 3 + k];
+      w2[i] = dmc ? w[i % nr] : 1.0;
+    }
+    pos.swap(p2);
+    w.swap(w2);
+  } else {
+    pos.assign((size_t)nw * ne * 3, 0.0);
+    w.assign(nw, 1.0);
+    if (qw_walkers_random(e, nw, opts.seed, 3.0, pos.data()) != QW_OK) return fail("walkers", e);
+  }
+  if (qw_walkers_set(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
+  if (!have) {
+    // equilibration from the random guess (VMC, depth-2 split moves)
+    qw_method_desc eq = m;
+    eq.method = 0;
+    eq.nblock = 1;
+    eq.nstep = 50;
+    eq.ndecorr = 2;
+    eq.timestep = dmc ? std::max(m.timestep, 0.05) : m.timestep;
+    std::vector<double> tmp(QW_BLOCK_FIELDS);
+    if (qw_run(e, &eq, tmp.data(), nullptr) != QW_OK) return fail("equilibration", e);
+  }
+  std::vector<double> eb((size_t)std::max(m.nblock, 1) * QW_BLOCK_FIELDS);
+  double stats[4];
+  if (qw_run(e, &m, eb.data(), stats) != QW_OK) return fail("run", e);
+  write_log(deck + ".log", dmc ? "dmc" : "vmc", m.timestep, dmc, eb, m.nblock);
+  if (qw_walkers_get(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
+  write_configs(cfg, nw, ne, pos, w, dmc);
+  double s = 0, s2 = 0;
+  for (int b = 0; b < m.nblock; b++) s += eb[(size_t)b * QW_BLOCK_FIELDS];
+  s /= std::max(m.nblock, 1);
+  for (int b = 0; b < m.nblock; b++) s2 += std::pow(eb[(size_t)b * QW_BLOCK_FIELDS] - s, 2);
+  double err = m.nblock > 1 ? std::sqrt(s2 / (m.nblock - 1) / m.nblock) : 0.0;
+  printf("%s: %d walkers, %d blocks x %d steps, timestep %g\n", dmc ? "DMC" : "VMC", nw, m.nblock, m.nstep,
+         m.timestep);
+  printf("total_energy %.10f +/- %.10f\n", s, err);
+  printf("acceptance %.4f  electron-moves/s %.4e  wall %.2f s\n", stats[0], stats[3], stats[2]);
+  qw_engine_destroy(e);
+  return 0;
+}

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libqwalk_amd.so")
 PREC = {"fp64": 0, "mixed_fp32": 1, "mixed_bf16": 2, "mixed_bf16x6": 3}
 STATUS = {0: "ok", -1: "bad argument", -2: "bad input", -3: "unsupported", -4: "device error", -5: "out of memory"}
 
+BLOCK_FIELDS = 12
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
 
@@ -55,6 +56,7 @@ def lib():
         L.qw_walkers_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.qw_walkers_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.qw_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(MethodDesc), _dp, _dp]
+        L.qw_walkers_random.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _dp]
         L.qw_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp,
                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _lib = L
@@ -66,7 +68,8 @@ def exported_symbols():
     L = lib()
     names = []
     for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
-              "qw_method_read", "qw_walkers_set", "qw_walkers_get", "qw_run", "qw_kernel_times"]:
+              "qw_method_read", "qw_walkers_set", "qw_walkers_get", "qw_walkers_random", "qw_run",
+              "qw_kernel_times"]:
         if hasattr(L, n):
             names.append(n)
     return names
@@ -139,8 +142,15 @@ class Engine:
         self._check(lib().qw_walkers_get(self.h, nw, _p(pos), _p(w)))
         return pos, w
 
+    def random_walkers(self, nw, seed=1, inirange=3.0):
+        pos = np.zeros((nw, self.ne, 3))
+        self._check(lib().qw_walkers_random(self.h, nw, seed, inirange, _p(pos)))
+        return pos
+
     def run(self, m):
-        eb = np.zeros((max(m.nblock, 1), 5))
+        """eblock[b] = E, kinetic, Coulomb, nonlocal, weight sum, points, var(E), var(KE),
+        var(V), var(NL), mean weight, var(weight) (QW_BLOCK_FIELDS)."""
+        eb = np.zeros((max(m.nblock, 1), BLOCK_FIELDS))
         stats = np.zeros(4)
         self._check(lib().qw_run(self.h, ctypes.byref(m), _p(eb), _p(stats)))
         return dict(eblock=eb[:m.nblock], acceptance=stats[0], moves=stats[1], seconds=stats[2],

@@ -1,0 +1,48 @@
+"""Drop-in check: the reference's own H-atom test (tests/h/run_test.py) with this engine in
+place of the qwalk binary.  The qwalk_amd driver runs the VMC and DMC decks, writes QWalk's
+block log, and the reference's gosling (compiled from its sources under oracle/_ref) reblocks
+it; the energies must match GAMESS HF (-0.4998098113) and the exact -0.5 within 3 sigma, with
+error bars below 0.1 (tests/qwtest.py:28-36)."""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+from tests._util import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(ROOT, "mainline_amd", "qwalk_amd")
+GOSLING = os.path.join(ROOT, "oracle", "_ref", "gosling")
+
+
+def _run_case(tmp_path, method_line, ref, sigmas=3.0):
+    d = os.path.join(GOLDEN, "decks", "h_atom")
+    for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb"]:
+        shutil.copy(os.path.join(d, f), tmp_path)
+    deck = tmp_path / "qw.run"
+    deck.write_text("%s\ninclude qw.sys\ntrialfunc { include qw.slater }\n" % method_line)
+    print("drop-in:", method_line, flush=True)
+    out = subprocess.run([DRIVER, str(deck)], cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    js = subprocess.run([GOSLING, "-json", "qw.run.log"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert js.returncode == 0, js.stderr
+    props = json.loads(js.stdout)["properties"]
+    v, e = props["total_energy"]["value"][0], props["total_energy"]["error"][0]
+    print("gosling total_energy", v, "+/-", e, flush=True)
+    assert e < 0.1
+    assert abs(v - ref) / e <= sigmas
+
+
+@pytest.mark.skipif(not os.path.exists(GOSLING), reason="reference gosling not built (oracle/Makefile.ref)")
+def test_h_atom_vmc_through_driver_and_gosling(tmp_path):
+    _run_case(tmp_path, "method { VMC nstep 1000 }", -0.4998098113)
+
+
+@pytest.mark.skipif(not os.path.exists(GOSLING), reason="reference gosling not built (oracle/Makefile.ref)")
+def test_h_atom_dmc_through_driver_and_gosling(tmp_path):
+    _run_case(tmp_path, "method { DMC timestep 0.005 nblock 80 }", -0.5)
