@@ -299,6 +299,92 @@ __device__ inline void basis_val(const DevSys& S, const double p[3], int sh0, in
   }
 }
 
+// Exclusive prefix sum across the wave; *total receives the wave sum.
+__device__ inline int wave_excl_scan(int v, int lane, int* total) {
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int t = __shfl_up(incl, o);
+    if (lane >= o) incl += t;
+  }
+  *total = __shfl(incl, kWave - 1);
+  return incl - v;
+}
+
+// Sparse basis evaluation: only shells inside their cutoff contribute (the functions the
+// reference's CUTOFF_MO loop visits, src/orbitals/MO_matrix_cutoff.h:516-530).  Active
+// functions of shells [sh0, sh1) are packed in increasing function order:
+// chi[c*ld + i] (i < returned count) and flist[i] = global function index.
+template <typename T>
+__device__ inline int basis_lap_sparse(const DevSys& S, const double p[3], int sh0, int sh1, T* chi, int ld,
+                                       int* flist, int lane) {
+  int base = 0;
+  for (int s0 = sh0; s0 < sh1; s0 += kWave) {
+    const int sh = s0 + lane;
+    ShellDesc d;
+    double x = 0, y = 0, z = 0, r = 0;
+    bool act = false;
+    if (sh < sh1) {
+      d = S.shells[sh];
+      const double* ap = S.atom_pos + 3 * d.atom;
+      x = p[0] - ap[0];
+      y = p[1] - ap[1];
+      z = p[2] - ap[2];
+      r = sqrt(x * x + y * y + z * z);
+      act = r < d.rcut && r < d.objthr;
+    }
+    int tot;
+    const int at = base + wave_excl_scan(act ? d.nf : 0, lane, &tot);
+    if (act) {
+      int iv = (int)(r * S.tab_inv[d.table]);
+      iv = iv < 0 ? 0 : (iv >= S.nint ? S.nint - 1 : iv);
+      const double* c = S.spline + ((size_t)d.table * S.nint + iv) * 4;
+      double h = r - iv * S.tab_sp[d.table];
+      double f = c[0] + h * (c[1] + h * (c[2] + h * c[3]));
+      double fd = (c[1] + h * (2 * c[2] + h * (3 * c[3]))) / r;
+      double f2 = 2 * c[2] + 6 * h * c[3];
+      shell_lap<T>(d.sym, f, fd, f2, x, y, z, chi + at, ld);
+      for (int i = 0; i < d.nf; i++) flist[at + i] = d.f0 + i;
+    }
+    base += tot;
+  }
+  return base;
+}
+
+template <typename T>
+__device__ inline int basis_val_sparse(const DevSys& S, const double p[3], int sh0, int sh1, T* chi, int* flist,
+                                       int lane) {
+  int base = 0;
+  for (int s0 = sh0; s0 < sh1; s0 += kWave) {
+    const int sh = s0 + lane;
+    ShellDesc d;
+    double x = 0, y = 0, z = 0, r = 0;
+    bool act = false;
+    if (sh < sh1) {
+      d = S.shells[sh];
+      const double* ap = S.atom_pos + 3 * d.atom;
+      x = p[0] - ap[0];
+      y = p[1] - ap[1];
+      z = p[2] - ap[2];
+      r = sqrt(x * x + y * y + z * z);
+      act = r < d.rcut && r < d.objthr;
+    }
+    int tot;
+    const int at = base + wave_excl_scan(act ? d.nf : 0, lane, &tot);
+    if (act) {
+      int iv = (int)(r * S.tab_inv[d.table]);
+      iv = iv < 0 ? 0 : (iv >= S.nint ? S.nint - 1 : iv);
+      const double* c = S.spline + ((size_t)d.table * S.nint + iv) * 4;
+      double h = r - iv * S.tab_sp[d.table];
+      double f = c[0] + h * (c[1] + h * (c[2] + h * c[3]));
+      shell_val<T>(d.sym, f, x, y, z, chi + at);
+      for (int i = 0; i < d.nf; i++) flist[at + i] = d.f0 + i;
+    }
+    base += tot;
+  }
+  return base;
+}
+
 // ---------------------------------------------------------------- Jastrow radial objects
 // CUTOFF_CUSP (src/basis_function/Cutoff_cusp.cpp:122-154) and POLYPADE
 // (src/basis_function/Poly_pade_function.cpp:183-237), contracted with coefficients cf.

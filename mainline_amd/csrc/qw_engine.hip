@@ -245,9 +245,14 @@ struct qw_engine {
   }
 
   // ---- kernels
-  size_t lds_rows() const { return 5 * (size_t)S.chunk_nf * sizeof(double); }
-  size_t lds_sweep() const { return (3 * (size_t)S.ne + 5 * (size_t)S.chunk_nf + 2 * (size_t)ns) * sizeof(double); }
-  size_t lds_local() const { return (7 * (size_t)S.ne + 5 * (size_t)S.chunk_nf) * sizeof(double); }
+  // LDS: tiles of basis values (5 x chunk_nf doubles) and active-function indices (chunk_nf ints)
+  size_t lds_rows() const { return 5 * (size_t)S.chunk_nf * sizeof(double) + (size_t)S.chunk_nf * sizeof(int); }
+  size_t lds_sweep() const {
+    return (3 * (size_t)S.ne + 5 * (size_t)S.chunk_nf + 2 * (size_t)ns) * sizeof(double) + (size_t)S.chunk_nf * sizeof(int);
+  }
+  size_t lds_local() const {
+    return (7 * (size_t)S.ne + 5 * (size_t)S.chunk_nf) * sizeof(double) + (size_t)S.chunk_nf * sizeof(int);
+  }
 
   void run_mo_rows() {
     launch("k_mo_rows", [&] {

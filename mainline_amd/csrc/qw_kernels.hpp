@@ -34,22 +34,22 @@ struct DmcCtl {
 // Lane j < n accumulates MO j: phi[c] = sum_b C[b][j] chi[c][b] over all basis chunks.
 // chi is an LDS tile of 5 x S.chunk_nf doubles.  Block-wide barriers separate the tile
 // producer (basis_lap) from the consumer.
-__device__ inline void lcao_lap(const DevSys& S, int s, const double p[3], double* chi, double phi[5], int lane) {
+// flist: LDS tile of S.chunk_nf ints (indices of the active functions).
+__device__ inline void lcao_lap(const DevSys& S, int s, const double p[3], double* chi, int* flist, double phi[5],
+                                int lane) {
   const int n = s ? S.ndn : S.nup, ld = S.chunk_nf;
   for (int c = 0; c < 5; c++) phi[c] = 0;
   for (int ch = 0; ch < S.nchunk; ch++) {
     int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
-    int f0 = S.shells[sh0].f0, f1 = sh1 < S.nsh ? S.shells[sh1].f0 : S.nb;
     __syncthreads();
-    basis_lap<double>(S, p, sh0, sh1, f0, chi, ld, lane);
+    const int nact = basis_lap_sparse<double>(S, p, sh0, sh1, chi, ld, flist, lane);
     __syncthreads();
     if (lane < n) {
-      const double* Cp = S.C[s] + (size_t)f0 * n + lane;
+      const double* Cp = S.C[s] + lane;
       double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-      int nf = f1 - f0;
-#pragma unroll 4
-      for (int b = 0; b < nf; b++) {
-        double c = Cp[(size_t)b * n];
+#pragma unroll 8
+      for (int b = 0; b < nact; b++) {
+        double c = Cp[(size_t)flist[b] * n];
         a0 += c * chi[b];
         a1 += c * chi[ld + b];
         a2 += c * chi[2 * ld + b];
@@ -65,21 +65,19 @@ __device__ inline void lcao_lap(const DevSys& S, int s, const double p[3], doubl
   }
 }
 
-__device__ inline double lcao_val(const DevSys& S, int s, const double p[3], double* chi, int lane) {
+__device__ inline double lcao_val(const DevSys& S, int s, const double p[3], double* chi, int* flist, int lane) {
   const int n = s ? S.ndn : S.nup;
   double phi = 0;
   for (int ch = 0; ch < S.nchunk; ch++) {
     int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
-    int f0 = S.shells[sh0].f0, f1 = sh1 < S.nsh ? S.shells[sh1].f0 : S.nb;
     __syncthreads();
-    basis_val<double>(S, p, sh0, sh1, f0, chi, lane);
+    const int nact = basis_val_sparse<double>(S, p, sh0, sh1, chi, flist, lane);
     __syncthreads();
     if (lane < n) {
-      const double* Cp = S.C[s] + (size_t)f0 * n + lane;
+      const double* Cp = S.C[s] + lane;
       double a = 0;
-      int nf = f1 - f0;
-#pragma unroll 4
-      for (int b = 0; b < nf; b++) a += Cp[(size_t)b * n] * chi[b];
+#pragma unroll 8
+      for (int b = 0; b < nact; b++) a += Cp[(size_t)flist[b] * n] * chi[b];
       phi += a;
     }
   }
@@ -97,7 +95,7 @@ __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) 
   double p[3];
   for (int d = 0; d < 3; d++) p[d] = W.pos[((size_t)w * S.ne + e) * 3 + d];
   double phi[5];
-  lcao_lap(S, s, p, lds, phi, lane);
+  lcao_lap(S, s, p, lds, (int*)(lds + 5 * S.chunk_nf), phi, lane);
   if (lane < n) {
     double* row = W.mo + ((size_t)w * S.ne + e) * 5 * S.nsp;
     for (int c = 0; c < 5; c++) row[c * S.nsp + lane] = phi[c];
@@ -249,6 +247,7 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
   double* chi = pos + 3 * ne;
   double* rowb = chi + 5 * S.chunk_nf;
   double* phib = rowb + NS;
+  int* flist = (int*)(phib + NS);
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   double inv[NS];
   const double* ig = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
@@ -291,7 +290,7 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
       gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
-      lcao_lap(S, s, rn, chi, phi, lane);
+      lcao_lap(S, s, rn, chi, flist, phi, lane);
       double t0 = wsum(rk * phi[0]);
       double t1 = wsum(rk * phi[1]), t2 = wsum(rk * phi[2]), t3 = wsum(rk * phi[3]);
       double gj[3];
@@ -411,6 +410,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
   double* gD = pos + 3 * ne;
   double* lD = gD + 3 * ne;
   double* chi = lD + ne;
+  int* flist = (int*)(chi + 5 * S.chunk_nf);
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   const TM* mog = mo_cache<TM>(W) + (size_t)w * ne * 5 * nsp;
   // determinant gradient / laplacian per electron: lane k owns row k of its spin's inverse
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
             double nd[3] = {np_[0] - ap[0], np_[1] - ap[1], np_[2] - ap[2]};
             double nr = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
             double cs = (nd[0] * od[0] + nd[1] * od[1] + nd[2] * od[2]) / (nr * orr);
-            double phi = lcao_val(S, s, np_, chi, lane);
+            double phi = lcao_val(S, s, np_, chi, flist, lane);
             double Rq = wsum(ivk * phi);
             double uq = jastrow_e_val(S, ee, np_, pos, lane);
             double ratio = Rq * exp(uq - u_old) * P.w[q];

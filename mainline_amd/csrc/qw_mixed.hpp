@@ -70,6 +70,44 @@ __device__ inline void basis_lap_f32(const DevSys& S, const MixedTables& M, cons
   }
 }
 
+// Sparse FP32 basis evaluation: active functions packed in increasing order (see
+// basis_lap_sparse); returns their count.
+__device__ inline int basis_lap_sparse_f32(const DevSys& S, const MixedTables& M, const double p[3], float* chi,
+                                           int ld, int* flist, int lane) {
+  int base = 0;
+  for (int s0 = 0; s0 < S.nsh; s0 += kWave) {
+    const int sh = s0 + lane;
+    ShellDesc d;
+    float x = 0, y = 0, z = 0, r = 0;
+    bool act = false;
+    if (sh < S.nsh) {
+      d = S.shells[sh];
+      const double* ap = S.atom_pos + 3 * d.atom;
+      x = (float)(p[0] - ap[0]);
+      y = (float)(p[1] - ap[1]);
+      z = (float)(p[2] - ap[2]);
+      r = sqrtf(x * x + y * y + z * z);
+      act = r < (float)d.rcut && r < (float)d.objthr;
+    }
+    int tot;
+    const int at = base + wave_excl_scan(act ? d.nf : 0, lane, &tot);
+    if (act) {
+      float inv = (float)S.tab_inv[d.table];
+      int iv = (int)(r * inv);
+      iv = iv < 0 ? 0 : (iv >= S.nint ? S.nint - 1 : iv);
+      const float4 c = *(const float4*)(M.spline + ((size_t)d.table * S.nint + iv) * 4);
+      float h = r - iv * (float)S.tab_sp[d.table];
+      float f = c.x + h * (c.y + h * (c.z + h * c.w));
+      float fd = (c.y + h * (2.f * c.z + h * (3.f * c.w))) / r;
+      float f2 = 2.f * c.z + 6.f * h * c.w;
+      shell_lap<float>(d.sym, f, fd, f2, x, y, z, chi + at, ld);
+      for (int i = 0; i < d.nf; i++) flist[at + i] = d.f0 + i;
+    }
+    base += tot;
+  }
+  return base;
+}
+
 // FP32 Jastrow terms of electron e at p: value, gradient (and laplacian when lap != nullptr).
 __device__ inline float jobj_lap_f(const JObj& o, const double* cf, float r, float& u, float& gf, float& lap) {
   if (r >= (float)o.rcut) return 0.f;
@@ -139,21 +177,20 @@ __device__ inline float jastrow_e_f(const DevSys& S, int e, const double p[3], c
 // phis: LDS 5 x 64 floats used to move MFMA accumulators to lane = MO.
 template <int MODE>
 __device__ inline void lcao_lap_mx(const DevSys& S, const MixedTables& M, int s, const double p[3], float* chi,
-                                   float* phis, float phi[5], int lane) {
+                                   float* phis, int* flist, float phi[5], int lane) {
   const int n = s ? S.ndn : S.nup, ld = M.ldchi;
   for (int c = 0; c < 5; c++) phi[c] = 0.f;
   if (MODE == LCAO_FP32) {
     {
       __syncthreads();
-      basis_lap_f32(S, M, p, 0, S.nsh, 0, chi, ld, lane);
+      const int nact = basis_lap_sparse_f32(S, M, p, chi, ld, flist, lane);
       __syncthreads();
       if (lane < n) {
         const float* Cp = M.Cf[s] + lane;
         float a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-        const int nf = S.nb;
-#pragma unroll 4
-        for (int b = 0; b < nf; b++) {
-          float c = Cp[(size_t)b * n];
+#pragma unroll 8
+        for (int b = 0; b < nact; b++) {
+          float c = Cp[(size_t)flist[b] * n];
           a0 += c * chi[b];
           a1 += c * chi[ld + b];
           a2 += c * chi[2 * ld + b];
@@ -240,7 +277,7 @@ __global__ __launch_bounds__(64) void k_mo_rows_mx(DevSys S, MixedTables M, DevW
   double p[3];
   for (int d = 0; d < 3; d++) p[d] = W.pos[((size_t)w * S.ne + e) * 3 + d];
   float phi[5];
-  lcao_lap_mx<MODE>(S, M, s, p, ldsf, ldsf + 5 * M.ldchi, phi, lane);
+  lcao_lap_mx<MODE>(S, M, s, p, ldsf, ldsf + 5 * M.ldchi, (int*)(ldsf + 5 * M.ldchi + 5 * 64), phi, lane);
   if (lane < n) {
     float* row = W.mof + ((size_t)w * S.ne + e) * 5 * S.nsp;
     for (int c = 0; c < 5; c++) row[c * S.nsp + lane] = phi[c];
@@ -269,6 +306,7 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
   float* phis = chi + 5 * M.ldchi;
   float* rowb = phis + 5 * 64;
   float* phib = rowb + NS;
+  int* flist = (int*)(phib + NS);
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   float inv[NS];
   const float* ig = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
@@ -308,7 +346,7 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
       gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
-      lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, phi, lane);
+      lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, flist, phi, lane);
       float t0 = wsumf(rk * phi[0]);
       float t1 = wsumf(rk * phi[1]), t2 = wsumf(rk * phi[2]), t3 = wsumf(rk * phi[3]);
       float gj[3];

@@ -66,9 +66,13 @@ void mixed_setup(qw_engine& e) {
 }
 
 namespace {
-size_t lds_mx_rows(const qw_engine& e) { return (5 * (size_t)e.MT.ldchi + 5 * 64) * sizeof(float); }
+// LDS: FP32 basis tile (5 x ldchi), MFMA accumulator exchange (5 x 64), active-function list (ldchi ints)
+size_t lds_mx_rows(const qw_engine& e) {
+  return (5 * (size_t)e.MT.ldchi + 5 * 64) * sizeof(float) + (size_t)e.MT.ldchi * sizeof(int);
+}
 size_t lds_mx_sweep(const qw_engine& e) {
-  return 3 * (size_t)e.S.ne * sizeof(double) + (5 * (size_t)e.MT.ldchi + 5 * 64 + 2 * (size_t)e.ns) * sizeof(float);
+  return 3 * (size_t)e.S.ne * sizeof(double) + (5 * (size_t)e.MT.ldchi + 5 * 64 + 2 * (size_t)e.ns) * sizeof(float) +
+         (size_t)e.MT.ldchi * sizeof(int);
 }
 
 // FP64 LU recompute of the FP32 inverses of spin `only_spin` (or both when -1) from the
