@@ -50,16 +50,74 @@ struct DevWalkers {
 };
 
 // ---------------------------------------------------------------- wave reductions
+// ---------------------------------------------------------------- wave reductions
+// Called with all 64 lanes active.  Within each row of 16 lanes a DPP butterfly (quad_perm
+// xor 1, quad_perm xor 2, row_half_mirror, row_mirror) leaves the row total in every lane; the
+// four row totals are read with readlane and added in a fixed order, so every lane returns
+// the bitwise-same value.
+template <int CTRL>
+__device__ inline int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_i<CTRL>((int)(unsigned)b), hi = (unsigned)dpp_i<CTRL>((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline float readlane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
 __device__ inline double wsum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_d<kDppXor1>(v);
+  v += dpp_d<kDppXor2>(v);
+  v += dpp_d<kDppHalfMirror>(v);
+  v += dpp_d<kDppMirror>(v);
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 __device__ inline float wsumf(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_f<kDppXor1>(v);
+  v += dpp_f<kDppXor2>(v);
+  v += dpp_f<kDppHalfMirror>(v);
+  v += dpp_f<kDppMirror>(v);
+  return (readlane_f(v, 0) + readlane_f(v, 16)) + (readlane_f(v, 32) + readlane_f(v, 48));
 }
+__device__ inline int wave_isum(int v) {
+  v += dpp_i<kDppXor1>(v);
+  v += dpp_i<kDppXor2>(v);
+  v += dpp_i<kDppHalfMirror>(v);
+  v += dpp_i<kDppMirror>(v);
+  return (__builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16)) +
+         (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
+}
+__device__ inline double wave_maxd(double v) {
+  v = fmax(v, dpp_d<kDppXor1>(v));
+  v = fmax(v, dpp_d<kDppXor2>(v));
+  v = fmax(v, dpp_d<kDppHalfMirror>(v));
+  v = fmax(v, dpp_d<kDppMirror>(v));
+  return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
+}
+__device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
+// Lane holding the largest v (the highest such lane on ties); -1 when no lane compares
+// equal to the maximum (NaN inputs).
+__device__ inline int wave_argmax_last(double v, int lane) {
+  const double m = wave_maxd(v);
+  const unsigned long long b = __ballot(v == m);
+  return b ? 63 - __clzll((long long)b) : -1;
+}
+// Value of v in lane l (l uniform).
+__device__ inline double wave_bcast(double v, int l) { return readlane_d(v, l); }
 
 // ---------------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC'11).
@@ -299,16 +357,18 @@ __device__ inline void basis_val(const DevSys& S, const double p[3], int sh0, in
   }
 }
 
-// Exclusive prefix sum across the wave; *total receives the wave sum.
+// Exclusive prefix sum across the wave of small counts (0 <= v < 16, the function count of one
+// shell) from four ballots; *total receives the wave sum.
 __device__ inline int wave_excl_scan(int v, int lane, int* total) {
-  int incl = v;
+  int ex = 0, tot = 0;
 #pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    int t = __shfl_up(incl, o);
-    if (lane >= o) incl += t;
+  for (int b = 0; b < 4; b++) {
+    const unsigned long long m = __ballot((v >> b) & 1);
+    ex += (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u)) << b;
+    tot += __popcll(m) << b;
   }
-  *total = __shfl(incl, kWave - 1);
-  return incl - v;
+  *total = tot;
+  return ex;
 }
 
 // Sparse basis evaluation: only shells inside their cutoff contribute (the functions the

@@ -259,25 +259,27 @@ struct qw_engine {
       hipLaunchKernelGGL(k_mo_rows, dim3((unsigned)((size_t)nw * S.ne)), dim3(64), lds_rows(), stream, S, W, nw);
     });
   }
-  template <typename TI>
-  void run_lu_t(TI* out) {
+  // inverse, log|det| and sign of spin `only_spin` (both spins when -1) from the MO value rows
+  template <typename TIN, typename TI>
+  void run_inv(const TIN* mo, TI* out, int only_spin) {
     auto go = [&](auto kern) {
-      launch("k_lu", [&] {
-        hipLaunchKernelGGL(kern, dim3(2 * nw), dim3(64), 0, stream, S, (const double*)W.mo, out, W.logdet, W.sgn, nw, -1);
+      launch("k_inv", [&] {
+        hipLaunchKernelGGL(kern, dim3(only_spin < 0 ? 2 * nw : nw), dim3(64), 0, stream, S, mo, out, W.logdet, W.sgn,
+                           nw, only_spin);
       });
     };
     switch (ns) {
-      case 4: go(k_lu<4, TI>); break;
-      case 8: go(k_lu<8, TI>); break;
-      case 16: go(k_lu<16, TI>); break;
-      case 32: go(k_lu<32, TI>); break;
-      default: go(k_lu<64, TI>); break;
+      case 4: go(k_inv<4, TIN, TI>); break;
+      case 8: go(k_inv<8, TIN, TI>); break;
+      case 16: go(k_inv<16, TIN, TI>); break;
+      case 32: go(k_inv<32, TIN, TI>); break;
+      default: go(k_inv<64, TIN, TI>); break;
     }
   }
   void refresh_all() {
     if (precision == QW_PREC_FP64) {
       run_mo_rows();
-      run_lu_t<double>(W.inv);
+      run_inv<double, double>(W.mo, W.inv, -1);
     } else {
       mixed_refresh_all(*this);
     }

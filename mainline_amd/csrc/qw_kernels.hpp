@@ -2,7 +2,7 @@
 //
 //   k_mo_rows    basis + LCAO for every electron of every walker (walker refresh, fixed-config
 //                evaluation); src/orbitals/MO_matrix_cutoff.h:480-560
-//   k_lu<NS>     pivoted LU, log|det|, sign and transposed inverse per walker and spin;
+//   k_inv<NS>    pivoted inversion, log|det|, sign and transposed inverse per walker and spin;
 //                src/program_utils/MatrixAlgebrac.cpp:108-224,316-421
 //   k_sweep<NS,D> single-electron moves of one spin channel: drift, proposal, ratio, Jastrow,
 //                acceptance, Sherman-Morrison update with the inverse held in registers;
@@ -102,17 +102,29 @@ __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) 
   }
 }
 
-// ---------------------------------------------------------------- k_lu
-// Right-looking LU with partial pivoting and implicit row scaling (the same pivot choice as
-// the reference's Crout ludcmp: largest vv[i]*|a[i][j]|, last index on ties), log|det| and
-// sign, then the transposed inverse: lane j solves A x = e_j and stores x as row j.
-template <int NS, typename TI>
-__global__ __launch_bounds__(64) void k_lu(DevSys S, const double* mo, TI* inv_out, double* logdet_out,
-                                           double* sgn_out, int nw, int only_spin) {
-  __shared__ double a[NS * NS];
-  __shared__ double vv[NS];
-  __shared__ int indx[NS], perm[NS];
-  __shared__ int singular;
+// ---------------------------------------------------------------- k_inv
+// Pivoted Gauss-Jordan inversion with the rows in registers: lane i holds row i of the
+// Slater matrix A[i][j] = phi_j(r_i) of spin s (TIN: FP64 or FP32 MO cache; arithmetic FP64).
+// The pivot of column j is the unpivoted row with the largest vv[i]*|a[i][j]| (vv = 1/max|row|,
+// last row on ties): the pivot sequence of the reference's Crout ludcmp, whose candidates are
+// the same Schur-complement entries (src/program_utils/MatrixAlgebrac.cpp:108-198).  Rows are
+// never exchanged.  With A' the row-permuted matrix (A'[j] = A[p_j]), lane p_j ends holding
+// row j of inv(A') in place; the transposed inverse out[k][i] = inv(A)[i][k] is then
+// inv(A')[i][q_k] with q = p^-1, so lane p_i stores its register m at out[p_m][i].
+// A pivot row is left unscaled in its lane with a per-lane scale (1/pivot), so each
+// elimination step is one FMA per element.  The row is held as vectors of 16 doubles so the
+// column j (uniform, dynamic) is read and written with register-relative moves.
+// log|det| = sum log|pivot|; the sign comes from the pivots and the permutation parity; an
+// exactly zero pivot flags the matrix singular as the reference does (log|det| = -1e201,
+// inverse zero).
+template <int NS, typename TIN, typename TI>
+__global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                            double* sgn_out, int nw, int only_spin) {
+  constexpr int VW = NS < 16 ? NS : 16, NB = NS / VW;
+  typedef double dvec __attribute__((ext_vector_type(VW)));
+  __shared__ double prow2[2][NS];
+  __shared__ double pvs[NS];
+  __shared__ int pl[NS];
   const int lane = threadIdx.x;
   const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
   const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
@@ -126,54 +138,68 @@ __global__ __launch_bounds__(64) void k_lu(DevSys S, const double* mo, TI* inv_o
     }
     return;
   }
-  for (int k = 0; k < n; k++)
-    if (lane < n) a[k * NS + lane] = mo[((size_t)w * S.ne + off + k) * 5 * nsp + lane];
-  if (lane == 0) singular = 0;
-  __syncthreads();
-  if (lane < n) {
-    double big = 0;
-    for (int j = 0; j < n; j++) big = fmax(big, fabs(a[lane * NS + j]));
-    if (big == 0.0) singular = 1;
-    vv[lane] = big == 0.0 ? 0.0 : 1.0 / big;
-  }
-  __syncthreads();
-  double sign = 1;
+  const bool row = lane < n;
+  dvec a[NB];
+  const TIN* rp = mo + ((size_t)w * S.ne + off + (row ? lane : 0)) * 5 * nsp;
+  double big = 0;
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int k = 0; k < VW; k++) {
+      const int m = b * VW + k;
+      a[b][k] = (row && m < n) ? (double)rp[m] : 0.0;
+      big = fmax(big, fabs(a[b][k]));
+    }
+  bool singular = wave_any(row && big == 0.0);
+  const double vv = big == 0.0 ? 0.0 : 1.0 / big;
+  bool pivoted = !row;
+  double sc = 1.0;
+  int qi = 0;
   for (int j = 0; j < n && !singular; j++) {
-    double dum = (lane >= j && lane < n) ? vv[lane] * fabs(a[lane * NS + j]) : -1.0;
-    int idx = lane;
-    for (int o = 32; o > 0; o >>= 1) {
-      double od = __shfl_xor(dum, o);
-      int oi = __shfl_xor(idx, o);
-      if (od > dum || (od == dum && oi > idx)) {
-        dum = od;
-        idx = oi;
-      }
+    const int jb = j / VW, jo = j % VW;
+    double aj = 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+      if (b == jb) aj = a[b][jo];
+    const double cand = pivoted ? -1.0 : vv * fabs(aj);
+    const int p = wave_argmax_last(cand, lane);
+    // no candidate (NaN entries) or a zero pivot: singular
+    if (p < 0 || !(wave_bcast(cand, p) >= 0.0)) {
+      singular = true;
+      break;
     }
-    const int imax = idx;
-    if (imax != j) {
-      for (int k = lane; k < n; k += kWave) {
-        double t = a[imax * NS + k];
-        a[imax * NS + k] = a[j * NS + k];
-        a[j * NS + k] = t;
-      }
-      sign = -sign;
+    const double pv = wave_bcast(aj, p);
+    if (pv == 0.0) {
+      singular = true;
+      break;
     }
-    __syncthreads();
     if (lane == 0) {
-      if (imax != j) vv[imax] = vv[j];
-      indx[j] = imax;
-      if (a[j * NS + j] == 0.0) singular = 1;
+      pl[j] = p;
+      pvs[j] = pv;
+    }
+    double* prow = prow2[j & 1];
+    const bool me = lane == p;
+    if (me) {
+#pragma unroll
+      for (int b = 0; b < NB; b++)
+#pragma unroll
+        for (int k = 0; k < VW; k++) prow[b * VW + k] = a[b][k];
     }
     __syncthreads();
-    if (singular) break;
-    double inv_p = 1.0 / a[j * NS + j];
-    if (lane > j && lane < n) a[lane * NS + j] *= inv_p;
-    __syncthreads();
-    if (lane > j && lane < n) {
-      double ujk = a[j * NS + lane];
-      for (int i = j + 1; i < n; i++) a[i * NS + lane] -= a[i * NS + j] * ujk;
+    const double g = me ? 0.0 : aj * (1.0 / pv);
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int k = 0; k < VW; k++) a[b][k] = fma(-g, prow[b * VW + k], a[b][k]);
+    const double nv = me ? 1.0 : -g;
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+      if (b == jb) a[b][jo] = nv;
+    if (me) {
+      sc = 1.0 / pv;
+      qi = j;
+      pivoted = true;
     }
-    __syncthreads();
   }
   if (singular) {
     if (lane == 0) {
@@ -183,48 +209,30 @@ __global__ __launch_bounds__(64) void k_lu(DevSys S, const double* mo, TI* inv_o
     for (int k = lane; k < n * nsp; k += kWave) out[k] = (TI)0;
     return;
   }
-  if (lane == 0) {
-    double ld = 0;
-    for (int i = 0; i < n; i++) {
-      ld += log(fabs(a[i * NS + i]));
-      if (a[i * NS + i] < 0) sign = -sign;
-    }
-    logdet_out[2 * w + s] = ld;
-    sgn_out[2 * w + s] = sign;
-    for (int i = 0; i < n; i++) perm[i] = i;
-    for (int i = 0; i < n; i++) {
-      int t = perm[i];
-      perm[i] = perm[indx[i]];
-      perm[indx[i]] = t;
-    }
-  }
   __syncthreads();
-  double x[NS];
-#pragma unroll
-  for (int i = 0; i < NS; i++) x[i] = (i < n && perm[i] == lane) ? 1.0 : 0.0;
-#pragma unroll
-  for (int i = 1; i < NS; i++) {
-    if (i < n) {
-      double acc = x[i];
-#pragma unroll
-      for (int k = 0; k < i; k++) acc -= a[i * NS + k] * x[k];
-      x[i] = acc;
-    }
+  // log|det|, negative pivots and the inversion count of the pivot sequence
+  double lg = 0;
+  int neg = 0, inv_cnt = 0;
+  for (int j = lane; j < n; j += kWave) {
+    lg += log(fabs(pvs[j]));
+    neg += pvs[j] < 0;
+    const int pj = pl[j];
+    for (int m = j + 1; m < n; m++) inv_cnt += pl[m] < pj;
   }
-#pragma unroll
-  for (int i = NS - 1; i >= 0; i--) {
-    if (i < n) {
-      double acc = x[i];
-#pragma unroll
-      for (int k = i + 1; k < NS; k++)
-        if (k < n) acc -= a[i * NS + k] * x[k];
-      x[i] = acc / a[i * NS + i];
-    }
+  lg = wsum(lg);
+  const int par = wave_isum(neg + inv_cnt);
+  if (lane == 0) {
+    logdet_out[2 * w + s] = lg;
+    sgn_out[2 * w + s] = (par & 1) ? -1.0 : 1.0;
   }
-  if (lane < n) {
+  if (row) {
 #pragma unroll
-    for (int i = 0; i < NS; i++)
-      if (i < n) out[(size_t)lane * nsp + i] = (TI)x[i];
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int k = 0; k < VW; k++) {
+        const int m = b * VW + k;
+        if (m < n) out[(size_t)pl[m] * nsp + qi] = (TI)(a[b][k] * sc);
+      }
   }
 }
 
@@ -329,13 +337,12 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
     }
     if (accepted) {
       // Sherman-Morrison on the transposed inverse for the replaced row k
-      if (lane < n) phib[lane] = phi[0];
+      if (lane < NS) phib[lane] = lane < n ? phi[0] : 0.0;
       __syncthreads();
       if (lane < n) {
         double tk = 0;
 #pragma unroll
-        for (int j = 0; j < NS; j++)
-          if (j < n) tk += inv[j] * phib[j];
+        for (int j = 0; j < NS; j++) tk += inv[j] * phib[j];
         if (lane == k) {
           const double f = 1.0 / R;
 #pragma unroll
@@ -343,8 +350,7 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
         } else {
           const double pk = tk * (-1.0 / R);
 #pragma unroll
-          for (int j = 0; j < NS; j++)
-            if (j < n) inv[j] += rowb[j] * pk;
+          for (int j = 0; j < NS; j++) inv[j] += rowb[j] * pk;
         }
         for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + lane] = phi[c];
       }

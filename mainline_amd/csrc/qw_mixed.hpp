@@ -284,16 +284,6 @@ __global__ __launch_bounds__(64) void k_mo_rows_mx(DevSys S, MixedTables M, DevW
   }
 }
 
-// Copies the FP32 MO value rows of spin s into the FP64 staging rows read by the LU kernel.
-__global__ __launch_bounds__(256) void k_widen_values(DevSys S, DevWalkers W, int nw) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t per = (size_t)S.ne * S.nsp;
-  if (i >= (size_t)nw * per) return;
-  const size_t w = i / per, r = i % per, e = r / S.nsp, j = r % S.nsp;
-  const size_t idx = ((w * S.ne + e) * 5) * S.nsp + j;
-  W.mo[idx] = (double)W.mof[idx];
-}
-
 // ---------------------------------------------------------------- k_sweep_mx
 template <int NS, int DEPTH, int MODE>
 __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W, SweepArgs A) {
@@ -382,13 +372,12 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
       }
     }
     if (accepted) {
-      if (lane < n) phib[lane] = phi[0];
+      if (lane < NS) phib[lane] = lane < n ? phi[0] : 0.f;
       __syncthreads();
       if (lane < n) {
         float tk = 0;
 #pragma unroll
-        for (int j = 0; j < NS; j++)
-          if (j < n) tk += inv[j] * phib[j];
+        for (int j = 0; j < NS; j++) tk += inv[j] * phib[j];
         if (lane == k) {
           const float f = 1.f / R;
 #pragma unroll
@@ -396,8 +385,7 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
         } else {
           const float pk = tk * (-1.f / R);
 #pragma unroll
-          for (int j = 0; j < NS; j++)
-            if (j < n) inv[j] += rowb[j] * pk;
+          for (int j = 0; j < NS; j++) inv[j] += rowb[j] * pk;
         }
         for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + lane] = phi[c];
       }

@@ -75,27 +75,9 @@ size_t lds_mx_sweep(const qw_engine& e) {
          (size_t)e.MT.ldchi * sizeof(int);
 }
 
-// FP64 LU recompute of the FP32 inverses of spin `only_spin` (or both when -1) from the
-// current FP32 MO value rows.
-void mixed_lu(qw_engine& e, int only_spin) {
-  const size_t tot = (size_t)e.nw * e.S.ne * e.S.nsp;
-  e.launch("k_widen", [&] {
-    hipLaunchKernelGGL(k_widen_values, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, e.stream, e.S, e.W, e.nw);
-  });
-  auto go = [&](auto kern) {
-    e.launch("k_lu", [&] {
-      hipLaunchKernelGGL(kern, dim3(only_spin < 0 ? 2 * e.nw : e.nw), dim3(64), 0, e.stream, e.S, (const double*)e.W.mo,
-                         e.W.invf, e.W.logdet, e.W.sgn, e.nw, only_spin);
-    });
-  };
-  switch (e.ns) {
-    case 4: go(k_lu<4, float>); break;
-    case 8: go(k_lu<8, float>); break;
-    case 16: go(k_lu<16, float>); break;
-    case 32: go(k_lu<32, float>); break;
-    default: go(k_lu<64, float>); break;
-  }
-}
+// FP64 recompute of the FP32 inverses of spin `only_spin` (or both when -1) from the current
+// FP32 MO value rows: log|det| and sign in FP64.
+void mixed_lu(qw_engine& e, int only_spin) { e.run_inv<float, float>(e.W.mof, e.W.invf, only_spin); }
 
 template <int MODE>
 void mixed_rows_t(qw_engine& e) {
