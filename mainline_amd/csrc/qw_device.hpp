@@ -10,6 +10,41 @@ namespace qw {
 
 constexpr int kWave = 64;
 
+// Read-only tables read at wave-uniform addresses go through the constant address space so
+// the compiler can use scalar loads and hoist them out of loops; the tables are never
+// written while a kernel runs.
+#define QW_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ inline const QW_CONST T* cptr(const T* p) {
+  return (const QW_CONST T*)p;
+}
+
+// Compact shell record for LDS staging: rc = min(shell cutoff, object cutoff);
+// meta = atom | sym << 10 | nf << 13 | table << 17.
+struct ShellLds {
+  double rc;
+  int meta;
+  int f0;
+};
+__device__ inline int shl_atom(int m) { return m & 1023; }
+__device__ inline int shl_sym(int m) { return (m >> 10) & 7; }
+__device__ inline int shl_nf(int m) { return (m >> 13) & 15; }
+__device__ inline int shl_table(int m) { return m >> 17; }
+
+// Device form of a Jastrow radial object (CUTOFF_CUSP kind 0, POLYPADE kind 1) with the
+// constants of its evaluation folded in: R = double (FP64 path) or float (mixed path).
+//   kind 0: ccl/ccu = c*cusp (like/unlike), scale = rcut, gamma, pade0
+//   kind 1: beta[i], cl/cu[i] = c_i, cbl/cbu[i] = c_i*(beta_i + 1)
+template <typename R>
+struct JRad {
+  int kind, nf;
+  R rcut, rcinv, rcinv2, gamma, pade0;
+  R beta[kMaxJF];
+  R cl[kMaxJF], cu[kMaxJF];
+  R cbl[kMaxJF], cbu[kMaxJF];
+  int nz[kMaxJF];  // coefficient i is non-zero for like or unlike pairs
+};
+
 // Read-only system tables in device memory.
 struct DevSys {
   int nup, ndn, ne, na, nb, nsh, ntab, nint;
@@ -23,9 +58,20 @@ struct DevSys {
   const double* tab_sp;   // [ntab] spacing (height within the interval)
   const double* C[2];     // [nb][nspin]
   const float* Cf[2];     // FP32 copies
-  const JObj* ee;
-  const JObj* ei;
-  const int* atom_ei0;
+  const JRad<double>* ee;   // [nee] electron-electron objects
+  const JRad<float>* eef;
+  // electron-ion Jastrow by atom type (atoms with identical object lists): atoms of type t
+  // are eit_a0[t] .. eit_a0[t+1]-1 in type order (positions eit_pos, charges eit_Z), their
+  // objects eit[eit_o0[t] .. eit_o0[t+1]-1]
+  int n_eit;
+  const JRad<double>* eit;
+  const JRad<float>* eitf;
+  const int* eit_a0;
+  const int* eit_o0;
+  const double* eit_pos;  // [na][3] in type order
+  const double* eit_Z;    // [na] in type order
+  const ShellLds* shl;    // [nsh] compact shell records
+  const double* tabs;     // [ntab][2] 1/spacing, spacing
   const PPType* pp;
   const int* atom_pp;
   double vii;
@@ -47,6 +93,7 @@ struct DevWalkers {
   double* elast;   // [nw][4] last kinetic, Coulomb, nonlocal, total
   int* acc;        // [nw] accepted moves since the last reset
   double* bsum;    // [nw][16] block sums (see kBsum)
+  double* dgl;     // [nw][ne][4] determinant gradient / laplacian ratios from the last sweep
 };
 
 // ---------------------------------------------------------------- wave reductions
@@ -446,108 +493,210 @@ __device__ inline int basis_val_sparse(const DevSys& S, const double p[3], int s
 }
 
 // ---------------------------------------------------------------- Jastrow radial objects
+// Reciprocal and square root from the hardware estimates refined by Newton-Raphson steps
+// (two for FP64: relative error near the double epsilon; the FP32 estimate is used as is).
+__device__ inline double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ inline float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
+// r = |d| and 1/r from d2 = |d|^2 (r = 0, 1/r = inf for coincident points, as sqrt and a
+// division give).
+__device__ inline void dist_inv(double d2, double& r, double& ri) {
+  double y = __builtin_amdgcn_rsq(d2);
+  double h = 0.5 * d2;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  const bool z = d2 == 0.0;
+  ri = z ? __builtin_inf() : y;
+  r = z ? 0.0 : d2 * y;
+}
+__device__ inline void dist_inv(float d2, float& r, float& ri) {
+  const float y = __builtin_amdgcn_rsqf(d2);
+  const bool z = d2 == 0.f;
+  ri = z ? __builtin_inff() : y;
+  r = z ? 0.f : d2 * y;
+}
+
 // CUTOFF_CUSP (src/basis_function/Cutoff_cusp.cpp:122-154) and POLYPADE
-// (src/basis_function/Poly_pade_function.cpp:183-237), contracted with coefficients cf.
-// Accumulates value u, gradient factor gfac (gradient = gfac * (x,y,z)) and laplacian.
-__device__ inline void jobj_lap(const JObj& o, const double* cf, double r, double& u, double& gfac, double& lap) {
+// (src/basis_function/Poly_pade_function.cpp:183-237), contracted with the like-spin /
+// one-body coefficients (cl) or the unlike-spin ones (cu), chosen per lane.  The object is
+// read at a wave-uniform address (scalar loads).  The POLYPADE sum is regrouped as
+//   u = zpp1 S0,  gfac = -zppd/(r rcut) S1,  d2 = -(zppdd2 S1 - zppd^2 S2)/rcut^2,
+//   lap = 2 (d2 + gfac),  S0 = sum c_i p_i, S1 = sum c_i (b_i+1) p_i^2, S2 = sum c_i (b_i+1) b_i p_i^3,
+// p_i = 1/(1 + b_i zpp): the reference's per-function terms summed in another order.
+// Accumulates value u, gradient factor gfac (gradient = gfac * (x,y,z)) and, if LAP, the
+// laplacian.
+template <typename R, bool LAP>
+__device__ inline void jrad_lap(const QW_CONST JRad<R>& o, bool unlike, R r, R ri, R& u, R& gfac, R& lap) {
   if (r >= o.rcut) return;
+  const R zz = r * o.rcinv, zz2 = zz * zz;
   if (o.kind == 0) {
-    double zz = r * o.rcutinv, zz2 = zz * zz, pp = zz - zz2 + zz * zz2 / 3;
-    double pade = 1. / (1 + o.gamma * pp), pade2 = pade * pade;
-    double ppd = 1. - 2. * zz + zz2, ppdd = -2. + 2. * zz;
-    double dadr = ppd * pade2 / r;
-    double dadr2 = ppdd * pade2 * o.rcutinv - 2. * o.gamma * ppd * ppd * pade2 * pade * o.rcutinv + 2. * dadr;
-    double c = cf[0];
-    u += c * (o.cusp * o.rcut * (pp * pade - o.pade0));
-    gfac += c * (o.cusp * dadr);
-    lap += c * (o.cusp * dadr2);
+    const R pp = zz - zz2 + zz * zz2 * (R)(1.0 / 3.0);
+    const R pade = rcp_nr((R)1 + o.gamma * pp), pade2 = pade * pade;
+    const R ppd = (R)1 - (R)2 * zz + zz2;
+    const R dadr = ppd * pade2 * ri;
+    const R c = unlike ? o.cu[0] : o.cl[0];
+    u += c * (o.rcut * (pp * pade - o.pade0));
+    gfac += c * dadr;
+    if (LAP) {
+      const R ppdd = (R)-2 + (R)2 * zz;
+      const R dadr2 = ppdd * pade2 * o.rcinv - (R)2 * o.gamma * ppd * ppd * pade2 * pade * o.rcinv + (R)2 * dadr;
+      lap += c * dadr2;
+    }
     return;
   }
-  double zz = r / o.rcut, zz2 = zz * zz, zpp = zz2 * (6. - 8. * zz + 3. * zz2), zpp1 = 1 - zpp;
-  double zppd = 12. * zz * (1. - 2. * zz + zz2), zppdd2 = 6. - 24. * zz + 18 * zz2;
-  for (int i = 0; i < o.nf; i++) {
-    double c = cf[i];
-    if (c == 0.0) continue;
-    double b = o.beta[i];
-    double zpade = 1. / (1 + b * zpp), zpade2 = zpade * zpade;
-    double crsd = -zppd * zpade2 * (b + 1) / (r * o.rcut);
-    double crsdd2 = -zpade2 * (zppdd2 - b * zppd * zppd * zpade) * (b + 1) / (o.rcut * o.rcut);
-    u += c * (zpp1 * zpade);
-    gfac += c * crsd;
-    lap += c * (2 * (crsdd2 + crsd));
+  const R zpp = zz2 * ((R)6 - (R)8 * zz + (R)3 * zz2), zpp1 = (R)1 - zpp;
+  const R zppd = (R)12 * zz * ((R)1 - (R)2 * zz + zz2);
+  R s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxJF; i++) {
+    if (i >= o.nf || !o.nz[i]) continue;
+    const R b = o.beta[i];
+    const R pz = rcp_nr((R)1 + b * zpp);
+    const R c = unlike ? o.cu[i] : o.cl[i], cb = unlike ? o.cbu[i] : o.cbl[i];
+    s0 += c * pz;
+    const R t = cb * pz * pz;
+    s1 += t;
+    if (LAP) s2 += t * b * pz;
+  }
+  u += zpp1 * s0;
+  const R g = -zppd * ri * o.rcinv * s1;
+  gfac += g;
+  if (LAP) {
+    const R zppdd2 = (R)6 - (R)24 * zz + (R)18 * zz2;
+    const R d2 = -o.rcinv2 * (zppdd2 * s1 - zppd * zppd * s2);
+    lap += (R)2 * (d2 + g);
   }
 }
-__device__ inline void jobj_val(const JObj& o, const double* cf, double r, double& u) {
+template <typename R>
+__device__ inline void jrad_val(const QW_CONST JRad<R>& o, bool unlike, R r, R& u) {
   if (r >= o.rcut) return;
+  const R zz = r * o.rcinv, zz2 = zz * zz;
   if (o.kind == 0) {
-    double zz = r * o.rcutinv, zz2 = zz * zz, pp = zz - zz2 + zz * zz2 / 3;
-    double pade = 1. / (1 + o.gamma * pp);
-    u += cf[0] * (o.cusp * o.rcut * (pp * pade - o.pade0));
+    const R pp = zz - zz2 + zz * zz2 * (R)(1.0 / 3.0);
+    const R pade = rcp_nr((R)1 + o.gamma * pp);
+    u += (unlike ? o.cu[0] : o.cl[0]) * (o.rcut * (pp * pade - o.pade0));
     return;
   }
-  double zz = r / o.rcut, zz2 = zz * zz, zpp = zz2 * (6. - 8. * zz + 3. * zz2), zpp1 = 1 - zpp;
-  for (int i = 0; i < o.nf; i++)
-    if (cf[i] != 0.0) u += cf[i] * (zpp1 / (1 + o.beta[i] * zpp));
+  const R zpp = zz2 * ((R)6 - (R)8 * zz + (R)3 * zz2), zpp1 = (R)1 - zpp;
+  R s0 = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxJF; i++) {
+    if (i >= o.nf || !o.nz[i]) continue;
+    s0 += (unlike ? o.cu[i] : o.cl[i]) * rcp_nr((R)1 + o.beta[i] * zpp);
+  }
+  u += zpp1 * s0;
 }
 
 // Jastrow terms involving electron e placed at p (others at pos): value, gradient and,
-// when lap != nullptr, laplacian with respect to r_e.  Wave-cooperative; result in all lanes.
+// when lap != nullptr, laplacian with respect to r_e.  epos: atom positions in e-i type
+// order with stride estride (3 in device memory, 4 in the LDS copy).  Wave-cooperative; the
+// result is in all lanes.  R: arithmetic type (double, or float on the mixed path with
+// distances from FP64 positions).
+template <typename R, bool LAP>
+__device__ inline R jastrow_e_t(const DevSys& S, const JRad<R>* ee, const JRad<R>* eit, int e, const double p[3],
+                                const double* pos, R g[3], R* lap, int lane, const double* epos, int estride) {
+  R u = 0, gx = 0, gy = 0, gz = 0, l = 0;
+  // object loops outermost: each object's parameters are loop-invariant over the pairs
+  const QW_CONST int* ea0 = cptr(S.eit_a0);
+  const QW_CONST int* eo0 = cptr(S.eit_o0);
+  for (int t = 0; t < S.n_eit; t++) {
+    const int a0 = ea0[t], a1 = ea0[t + 1], o1 = eo0[t + 1];
+    for (int o = eo0[t]; o < o1; o++) {
+      const QW_CONST JRad<R>& ob = cptr(eit)[o];
+      for (int k = a0 + lane; k < a1; k += kWave) {
+        const double* ap = epos + estride * k;
+        const R x = (R)(p[0] - ap[0]), y = (R)(p[1] - ap[1]), z = (R)(p[2] - ap[2]);
+        R r, ri;
+        dist_inv(x * x + y * y + z * z, r, ri);
+        R gf = 0;
+        jrad_lap<R, LAP>(ob, false, r, ri, u, gf, l);
+        gx += gf * x;
+        gy += gf * y;
+        gz += gf * z;
+      }
+    }
+  }
+  const bool eup = e < S.nup;
+  for (int o = 0; o < S.nee; o++) {
+    const QW_CONST JRad<R>& ob = cptr(ee)[o];
+    for (int j = lane; j < S.ne; j += kWave) {
+      if (j == e) continue;
+      const R x = (R)(p[0] - pos[3 * j]), y = (R)(p[1] - pos[3 * j + 1]), z = (R)(p[2] - pos[3 * j + 2]);
+      R r, ri;
+      dist_inv(x * x + y * y + z * z, r, ri);
+      const bool unlike = (j < S.nup) != eup;
+      R gf = 0;
+      jrad_lap<R, LAP>(ob, unlike, r, ri, u, gf, l);
+      gx += gf * x;
+      gy += gf * y;
+      gz += gf * z;
+    }
+  }
+  if constexpr (sizeof(R) == sizeof(double)) {
+    u = wsum(u);
+    g[0] = wsum(gx);
+    g[1] = wsum(gy);
+    g[2] = wsum(gz);
+    if (LAP) *lap = wsum(l);
+  } else {
+    u = wsumf(u);
+    g[0] = wsumf(gx);
+    g[1] = wsumf(gy);
+    g[2] = wsumf(gz);
+    if (LAP) *lap = wsumf(l);
+  }
+  return u;
+}
+__device__ inline double jastrow_e(const DevSys& S, int e, const double p[3], const double* pos, double g[3],
+                                   double* lap, int lane, const double* epos, int estride) {
+  if (lap) return jastrow_e_t<double, true>(S, S.ee, S.eit, e, p, pos, g, lap, lane, epos, estride);
+  return jastrow_e_t<double, false>(S, S.ee, S.eit, e, p, pos, g, nullptr, lane, epos, estride);
+}
 __device__ inline double jastrow_e(const DevSys& S, int e, const double p[3], const double* pos, double g[3],
                                    double* lap, int lane) {
-  double u = 0, gx = 0, gy = 0, gz = 0, l = 0;
-  for (int a = lane; a < S.na; a += kWave) {
-    const double* ap = S.atom_pos + 3 * a;
-    double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
-    double r = sqrt(x * x + y * y + z * z);
-    double gf = 0;
-    for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_lap(S.ei[o], S.ei[o].cl, r, u, gf, l);
-    gx += gf * x;
-    gy += gf * y;
-    gz += gf * z;
-  }
-  bool eup = e < S.nup;
-  for (int j = lane; j < S.ne; j += kWave) {
-    if (j == e) continue;
-    double x = p[0] - pos[3 * j], y = p[1] - pos[3 * j + 1], z = p[2] - pos[3 * j + 2];
-    double r = sqrt(x * x + y * y + z * z);
-    bool unlike = (j < S.nup) != eup;
-    double gf = 0;
-    for (int o = 0; o < S.nee; o++) jobj_lap(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, u, gf, l);
-    gx += gf * x;
-    gy += gf * y;
-    gz += gf * z;
-  }
-  u = wsum(u);
-  g[0] = wsum(gx);
-  g[1] = wsum(gy);
-  g[2] = wsum(gz);
-  if (lap) *lap = wsum(l);
-  return u;
+  return jastrow_e(S, e, p, pos, g, lap, lane, S.eit_pos, 3);
 }
 
 // Value-only Jastrow terms of electron e at p (pseudopotential quadrature).
 __device__ inline double jastrow_e_val(const DevSys& S, int e, const double p[3], const double* pos, int lane) {
   double u = 0;
-  for (int a = lane; a < S.na; a += kWave) {
-    const double* ap = S.atom_pos + 3 * a;
-    double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
-    double r = sqrt(x * x + y * y + z * z);
-    for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_val(S.ei[o], S.ei[o].cl, r, u);
+  const QW_CONST int* ea0 = cptr(S.eit_a0);
+  const QW_CONST int* eo0 = cptr(S.eit_o0);
+  for (int t = 0; t < S.n_eit; t++) {
+    const int a0 = ea0[t], a1 = ea0[t + 1], o1 = eo0[t + 1];
+    for (int o = eo0[t]; o < o1; o++) {
+      const QW_CONST JRad<double>& ob = cptr(S.eit)[o];
+      for (int k = a0 + lane; k < a1; k += kWave) {
+        const double* ap = S.eit_pos + 3 * k;
+        double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
+        double r, ri;
+        dist_inv(x * x + y * y + z * z, r, ri);
+        jrad_val<double>(ob, false, r, u);
+      }
+    }
   }
-  bool eup = e < S.nup;
-  for (int j = lane; j < S.ne; j += kWave) {
-    if (j == e) continue;
-    double x = p[0] - pos[3 * j], y = p[1] - pos[3 * j + 1], z = p[2] - pos[3 * j + 2];
-    double r = sqrt(x * x + y * y + z * z);
-    bool unlike = (j < S.nup) != eup;
-    for (int o = 0; o < S.nee; o++) jobj_val(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, u);
+  const bool eup = e < S.nup;
+  for (int o = 0; o < S.nee; o++) {
+    const QW_CONST JRad<double>& ob = cptr(S.ee)[o];
+    for (int j = lane; j < S.ne; j += kWave) {
+      if (j == e) continue;
+      double x = p[0] - pos[3 * j], y = p[1] - pos[3 * j + 1], z = p[2] - pos[3 * j + 2];
+      double r, ri;
+      dist_inv(x * x + y * y + z * z, r, ri);
+      jrad_val<double>(ob, (j < S.nup) != eup, r, u);
+    }
   }
   return wsum(u);
 }
 
 // Semilocal pseudopotential channel values v_l(r) (src/basis_function/Rgaussian_function.cpp:180-202,
 // src/system/Pseudopotential.cpp:305-318).
-__device__ inline void pp_radial(const PPType& P, double Z, double r, double* vl) {
+__device__ inline void pp_radial(const QW_CONST PPType& P, double Z, double r, double* vl) {
   for (int l = 0; l < P.L; l++) {
     double v = 0;
     for (int j = 0; j < P.nterm[l]; j++) {

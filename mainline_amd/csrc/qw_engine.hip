@@ -42,6 +42,38 @@ int pick_ns(int n) {
 
 }  // namespace
 
+// Device forms of the Jastrow radial objects (constants folded, see JRad).
+template <typename R>
+std::vector<JRad<R>> jrad_table(const std::vector<JObj>& objs) {
+  std::vector<JRad<R>> out;
+  for (const JObj& o : objs) {
+    JRad<R> d{};
+    d.kind = o.kind;
+    d.nf = o.nf;
+    d.rcut = (R)o.rcut;
+    d.rcinv = (R)o.rcutinv;
+    d.rcinv2 = (R)(o.rcutinv * o.rcutinv);
+    d.gamma = (R)o.gamma;
+    d.pade0 = (R)o.pade0;
+    for (int i = 0; i < kMaxJF; i++) {
+      d.beta[i] = (R)o.beta[i];
+      if (o.kind == 0) {
+        d.cl[i] = (R)(o.cl[i] * o.cusp);
+        d.cu[i] = (R)(o.cu[i] * o.cusp);
+      } else {
+        d.cl[i] = (R)o.cl[i];
+        d.cu[i] = (R)o.cu[i];
+      }
+      d.cbl[i] = (R)(o.cl[i] * (o.beta[i] + 1));
+      d.cbu[i] = (R)(o.cu[i] * (o.beta[i] + 1));
+      d.nz[i] = (o.cl[i] != 0.0 || o.cu[i] != 0.0) ? 1 : 0;
+    }
+    out.push_back(d);
+  }
+  if (out.empty()) out.push_back(JRad<R>{});
+  return out;
+}
+
 struct qw_engine {
   std::string err;
   int device = 0, precision = 0, refresh = 32;
@@ -194,9 +226,65 @@ struct qw_engine {
       std::vector<float> cf(T.C[s].begin(), T.C[s].end());
       S.Cf[s] = upload(cf);
     }
-    S.ee = upload(T.ee);
-    S.ei = upload(T.ei);
-    S.atom_ei0 = upload(T.atom_ei0);
+    S.ee = upload(jrad_table<double>(T.ee));
+    S.eef = upload(jrad_table<float>(T.ee));
+    {
+      // electron-ion Jastrow objects grouped by atom type: atoms whose object lists are equal
+      std::vector<std::vector<int>> members;
+      std::vector<int> rep;
+      auto same = [&](int a, int b) {
+        const int na0 = T.atom_ei0[a], nb0 = T.atom_ei0[b];
+        const int n1 = T.atom_ei0[a + 1] - na0, n2 = T.atom_ei0[b + 1] - nb0;
+        return n1 == n2 && (n1 == 0 || std::memcmp(&T.ei[na0], &T.ei[nb0], sizeof(JObj) * n1) == 0);
+      };
+      for (int a = 0; a < T.na; a++) {
+        size_t t = 0;
+        while (t < rep.size() && !same(rep[t], a)) t++;
+        if (t == rep.size()) {
+          rep.push_back(a);
+          members.emplace_back();
+        }
+        members[t].push_back(a);
+      }
+      std::vector<JObj> eit;
+      std::vector<int> a0{0}, o0{0};
+      std::vector<double> epos, eZ;
+      for (size_t t = 0; t < rep.size(); t++) {
+        for (int o = T.atom_ei0[rep[t]]; o < T.atom_ei0[rep[t] + 1]; o++) eit.push_back(T.ei[o]);
+        o0.push_back((int)eit.size());
+        for (int a : members[t]) {
+          for (int d = 0; d < 3; d++) epos.push_back(T.atom_pos[3 * a + d]);
+          eZ.push_back(T.atom_Z[a]);
+        }
+        a0.push_back(a0.back() + (int)members[t].size());
+      }
+      if (eit.empty()) eit.push_back(JObj{});
+      S.n_eit = (int)rep.size();
+      S.eit = upload(jrad_table<double>(eit));
+      S.eitf = upload(jrad_table<float>(eit));
+      S.eit_a0 = upload(a0);
+      S.eit_o0 = upload(o0);
+      S.eit_pos = upload(epos);
+      S.eit_Z = upload(eZ);
+    }
+    {
+      if (T.na > 1023 || T.ntab >= (1 << 14)) throw std::runtime_error("too many atoms or spline tables");
+      std::vector<ShellLds> shl(T.shells.size());
+      for (size_t i = 0; i < T.shells.size(); i++) {
+        const ShellDesc& d = T.shells[i];
+        shl[i].rc = std::min(d.rcut, d.objthr);
+        shl[i].meta = d.atom | (d.sym << 10) | (d.nf << 13) | (d.table << 17);
+        shl[i].f0 = d.f0;
+      }
+      S.shl = upload(shl);
+      std::vector<double> tabs;
+      for (int t = 0; t < T.ntab; t++) {
+        tabs.push_back(T.tab_invspacing[t]);
+        tabs.push_back(T.tab_spacing[t]);
+      }
+      if (tabs.empty()) tabs.assign(2, 0.0);
+      S.tabs = upload(tabs);
+    }
     S.pp = upload(T.pptypes);
     S.atom_pp = upload(T.atom_pp);
     dctl = upload(std::vector<DmcCtl>(1));
@@ -233,6 +321,7 @@ struct qw_engine {
     W.elast = walloc<double>((size_t)n * 4);
     W.acc = walloc<int>(n);
     W.bsum = walloc<double>((size_t)n * kBsum);
+    W.dgl = walloc<double>((size_t)n * ne * 4);
     int P = 1;
     while (P < n) P <<= 1;
     keys = walloc<unsigned long long>(P);
@@ -247,9 +336,7 @@ struct qw_engine {
   // ---- kernels
   // LDS: tiles of basis values (5 x chunk_nf doubles) and active-function indices (chunk_nf ints)
   size_t lds_rows() const { return 5 * (size_t)S.chunk_nf * sizeof(double) + (size_t)S.chunk_nf * sizeof(int); }
-  size_t lds_sweep() const {
-    return (3 * (size_t)S.ne + 5 * (size_t)S.chunk_nf + 2 * (size_t)ns) * sizeof(double) + (size_t)S.chunk_nf * sizeof(int);
-  }
+  size_t lds_sweep() const { return sweep_lds_bytes(S.ne, S.na, S.nsh, S.ntab, ns, kCapF64, sizeof(double)); }
   size_t lds_local() const {
     return (7 * (size_t)S.ne + 5 * (size_t)S.chunk_nf) * sizeof(double) + (size_t)S.chunk_nf * sizeof(int);
   }
@@ -284,8 +371,8 @@ struct qw_engine {
       mixed_refresh_all(*this);
     }
   }
-  void run_sweep(int s, int k0, int k1, double tau, unsigned sweep, int depth, int restrict_nodes) {
-    SweepArgs A{s, k0, k1, tau, seed, step, sweep, restrict_nodes};
+  void run_sweep(int s, int k0, int k1, double tau, unsigned sweep, int depth, int restrict_nodes, int dets) {
+    SweepArgs A{s, k0, k1, tau, seed, step, sweep, restrict_nodes, dets};
     if (precision != QW_PREC_FP64) {
       mixed_sweep(*this, A, depth);
       return;
@@ -645,11 +732,12 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
             for (int s = 0; s < 2; s++) {
               int n = s ? e->T.ndn : e->T.nup;
               if (n == 0) continue;
-              e->run_sweep(s, 0, n, tau, (unsigned)sw, dmc ? 1 : 2, dmc ? 1 : 0);
+              e->run_sweep(s, 0, n, tau, (unsigned)sw, dmc ? 1 : 2, dmc ? 1 : 0, sw == nsw - 1);
             }
           }
           moves += (long long)nw * ne * nsw;
           A.mode = dmc ? 2 : 1;
+          A.use_dgl = 1;
           A.step = e->step;
           e->run_local(A);
           e->step++;

@@ -14,7 +14,7 @@
 //                src/method/Dmc_method.cpp:776-797,961-995
 //   k_copy       walker state copies decided by k_branch
 #pragma once
-#include "qw_device.hpp"
+#include "qw_move.hpp"
 
 namespace qw {
 
@@ -243,20 +243,57 @@ struct SweepArgs {
   unsigned long long seed;
   unsigned step, sweep;
   int restrict_nodes;
+  int dets;  // write the determinant gradient / laplacian ratios of spin s to W.dgl at the end
 };
 
+// Determinant parts of the local energy for lane k's electron of spin s from the inverse row
+// in registers: dgl[c] = sum_j M_c[e_k][j] inv(k, j), c = d/dx, d/dy, d/dz, laplacian
+// (src/wavefunction/Slat_wf.h:1021-1080, getDetLap).  inv[j] is zero for j >= n, so the row index is
+// clamped instead of branched on.
+template <int NS, typename TI, typename TM>
+__device__ inline void write_det_parts(const DevSys& S, const DevWalkers& W, int w, int s, const TI* inv, const TM* mog,
+                                       int lane) {
+  const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
+  if (lane >= n) return;
+  const int e = off + lane;
+  const TM* mr = mog + (size_t)e * 5 * nsp;
+  double t[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const TM* row = mr + (size_t)(c + 1) * nsp;
+#pragma unroll
+    for (int j = 0; j < NS; j++) t[c] += (double)row[min(j, n - 1)] * (double)inv[j];
+  }
+  double* o = W.dgl + ((size_t)w * S.ne + e) * 4;
+  o[0] = t[0];
+  o[1] = t[1];
+  o[2] = t[2];
+  o[3] = t[3];
+}
+
+// Basis functions per LDS tile batch in the FP64 sweep.
+constexpr int kCapF64 = 128;
+
+__host__ __device__ inline size_t sweep_lds_bytes(int ne, int na, int nsh, int ntab, int ns, int cap, int tsize) {
+  return (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15) + ((sys_lds_bytes(na, nsh, ntab) + 15) & ~(size_t)15) +
+         (size_t)5 * cap * tsize + (size_t)cap * sizeof(int) + (size_t)2 * ns * tsize + 16;
+}
+
 template <int NS, int DEPTH>
-__global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs A) {
+__global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepArgs A) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, w = blockIdx.x;
   if (w >= W.nw) return;
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
   double* pos = lds;
-  double* chi = pos + 3 * ne;
-  double* rowb = chi + 5 * S.chunk_nf;
+  SysLds V;
+  char* q = sys_lds_carve(S, (char*)lds + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
+  double* chi = (double*)q;
+  int* roff = (int*)(chi + 5 * kCapF64);
+  double* rowb = (double*)(roff + kCapF64);
   double* phib = rowb + NS;
-  int* flist = (int*)(phib + NS);
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  stage_sys(S, V, lane);
   double inv[NS];
   const double* ig = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
 #pragma unroll
@@ -265,6 +302,14 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
   int nacc = 0;
   double* mog = W.mo + (size_t)w * ne * 5 * nsp;
   const double tau = A.tau, stau = sqrt(tau);
+  // MO gradient row of the electron about to move, loaded one move ahead
+  double m1 = 0, m2 = 0, m3 = 0;
+  if (lane < n && A.k0 < A.k1) {
+    const size_t e0 = off + A.k0;
+    m1 = mog[(e0 * 5 + 1) * nsp + lane];
+    m2 = mog[(e0 * 5 + 2) * nsp + lane];
+    m3 = mog[(e0 * 5 + 3) * nsp + lane];
+  }
   __syncthreads();
   for (int k = A.k0; k < A.k1; k++) {
     const int e = off + k;
@@ -275,16 +320,10 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
     }
     __syncthreads();
     const double rk = lane < n ? rowb[lane] : 0.0;
-    double m1 = 0, m2 = 0, m3 = 0;
-    if (lane < n) {
-      m1 = mog[((size_t)e * 5 + 1) * nsp + lane];
-      m2 = mog[((size_t)e * 5 + 2) * nsp + lane];
-      m3 = mog[((size_t)e * 5 + 3) * nsp + lane];
-    }
     // drift at the old position: determinant part from the cached MO row, Jastrow part fresh
     double d0[3], gj0[3];
     const double gdx = wsum(rk * m1), gdy = wsum(rk * m2), gdz = wsum(rk * m3);
-    const double u0 = jastrow_e(S, e, r0, pos, gj0, nullptr, lane);
+    const double u0 = jastrow_e(S, e, r0, pos, gj0, nullptr, lane, V.epos, 4);
     d0[0] = gdx + gj0[0];
     d0[1] = gdy + gj0[1];
     d0[2] = gdz + gj0[2];
@@ -298,11 +337,11 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
       gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
-      lcao_lap(S, s, rn, chi, flist, phi, lane);
+      mo_lap_point<double, double, 4>(S, V, S.spline, S.C[s], n, rn, chi, roff, kCapF64, phi, lane);
       double t0 = wsum(rk * phi[0]);
       double t1 = wsum(rk * phi[1]), t2 = wsum(rk * phi[2]), t3 = wsum(rk * phi[3]);
       double gj[3];
-      double un = jastrow_e(S, e, rn, pos, gj, nullptr, lane);
+      double un = jastrow_e(S, e, rn, pos, gj, nullptr, lane, V.epos, 4);
       dn[0] = t1 / t0 + gj[0];
       dn[1] = t2 / t0 + gj[1];
       dn[2] = t3 / t0 + gj[2];
@@ -359,6 +398,12 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
       nacc++;
       if (lane < 3) pos[3 * e + lane] = r1[lane];
     }
+    if (lane < n && k + 1 < A.k1) {
+      const size_t e1 = e + 1;
+      m1 = mog[(e1 * 5 + 1) * nsp + lane];
+      m2 = mog[(e1 * 5 + 2) * nsp + lane];
+      m3 = mog[(e1 * 5 + 3) * nsp + lane];
+    }
     __syncthreads();
   }
   double* igw = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
@@ -367,6 +412,7 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
     for (int j = 0; j < NS; j++)
       if (j < n) igw[(size_t)lane * nsp + j] = inv[j];
   }
+  if (A.dets) write_det_parts<NS>(S, W, w, s, inv, (const double*)mog, lane);
   for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
   if (lane == 0) {
     W.logdet[2 * w + s] = logdet;
@@ -378,6 +424,7 @@ __global__ __launch_bounds__(64) void k_sweep(DevSys S, DevWalkers W, SweepArgs 
 // ---------------------------------------------------------------- k_local
 struct LocalArgs {
   int mode;              // 0 = evaluate (write outputs), 1 = VMC sums, 2 = DMC weight + sums, 3 = init (elast only)
+  int use_dgl;           // determinant parts from W.dgl (written by the last sweep) instead of the inverse
   int pp_mode;           // 0 deterministic, 1 stochastic
   const double* rot;     // [9] fixed rotation (mode 0) or null
   const double* pp_rand; // [nw][na][ne] uniforms (mode 0, stochastic) or null
@@ -420,7 +467,16 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   const TM* mog = mo_cache<TM>(W) + (size_t)w * ne * 5 * nsp;
   // determinant gradient / laplacian per electron: lane k owns row k of its spin's inverse
-  for (int s = 0; s < 2; s++) {
+  if (A.use_dgl) {
+    for (int e = lane; e < ne; e += kWave) {
+      const double* d = W.dgl + ((size_t)w * ne + e) * 4;
+      gD[3 * e] = d[0];
+      gD[3 * e + 1] = d[1];
+      gD[3 * e + 2] = d[2];
+      lD[e] = d[3];
+    }
+  }
+  for (int s = 0; s < 2 && !A.use_dgl; s++) {
     const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0;
     const TM* ig = inv_store<TM>(W) + ((size_t)w * 2 + s) * nsp * nsp;
     for (int k = lane; k < n; k += kWave) {
@@ -445,30 +501,56 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
   for (int i = lane; i < ne; i += kWave) {
     const double* pi = pos + 3 * i;
     double gx = 0, gy = 0, gz = 0, l = 0, u = 0;
-    for (int a = 0; a < S.na; a++) {
-      const double* ap = S.atom_pos + 3 * a;
-      double x = pi[0] - ap[0], y = pi[1] - ap[1], z = pi[2] - ap[2];
-      double r = sqrt(x * x + y * y + z * z);
-      vei += S.atom_Z[a] / r;
-      double gf = 0;
-      for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_lap(S.ei[o], S.ei[o].cl, r, u, gf, l);
-      gx += gf * x;
-      gy += gf * y;
-      gz += gf * z;
+    // object loops outermost (object parameters loop-invariant over the partners)
+    const QW_CONST int* ea0 = cptr(S.eit_a0);
+    const QW_CONST int* eo0 = cptr(S.eit_o0);
+    const QW_CONST double* epos = cptr(S.eit_pos);
+    for (int t = 0; t < S.n_eit; t++) {
+      const int a0 = ea0[t], a1 = ea0[t + 1], o0 = eo0[t], o1 = eo0[t + 1];
+      for (int k = a0; k < a1; k++) {
+        const QW_CONST double* ap = epos + 3 * k;
+        double x = pi[0] - ap[0], y = pi[1] - ap[1], z = pi[2] - ap[2];
+        double r, ri;
+        dist_inv(x * x + y * y + z * z, r, ri);
+        vei += cptr(S.eit_Z)[k] * ri;
+      }
+      for (int o = o0; o < o1; o++) {
+        const QW_CONST JRad<double>& ob = cptr(S.eit)[o];
+        for (int k = a0; k < a1; k++) {
+          const QW_CONST double* ap = epos + 3 * k;
+          double x = pi[0] - ap[0], y = pi[1] - ap[1], z = pi[2] - ap[2];
+          double r, ri;
+          dist_inv(x * x + y * y + z * z, r, ri);
+          double gf = 0;
+          jrad_lap<double, true>(ob, false, r, ri, u, gf, l);
+          gx += gf * x;
+          gy += gf * y;
+          gz += gf * z;
+        }
+      }
     }
     const bool iup = i < S.nup;
-    for (int j = 0; j < ne; j++) {
-      if (j == i) continue;
+    for (int j = 0; j < i; j++) {
       double x = pi[0] - pos[3 * j], y = pi[1] - pos[3 * j + 1], z = pi[2] - pos[3 * j + 2];
-      double r = sqrt(x * x + y * y + z * z);
-      if (j < i) vee += 1.0 / r;
-      bool unlike = (j < S.nup) != iup;
-      double gf = 0, upair = 0;
-      for (int o = 0; o < S.nee; o++) jobj_lap(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, upair, gf, l);
-      if (j > i) u += upair;
-      gx += gf * x;
-      gy += gf * y;
-      gz += gf * z;
+      double r, ri;
+      dist_inv(x * x + y * y + z * z, r, ri);
+      vee += ri;
+    }
+    for (int o = 0; o < S.nee; o++) {
+      const QW_CONST JRad<double>& ob = cptr(S.ee)[o];
+      for (int j = 0; j < ne; j++) {
+        if (j == i) continue;
+        double x = pi[0] - pos[3 * j], y = pi[1] - pos[3 * j + 1], z = pi[2] - pos[3 * j + 2];
+        double r, ri;
+        dist_inv(x * x + y * y + z * z, r, ri);
+        const bool unlike = (j < S.nup) != iup;
+        double gf = 0, upair = 0;
+        jrad_lap<double, true>(ob, unlike, r, ri, upair, gf, l);
+        if (j > i) u += upair;
+        gx += gf * x;
+        gy += gf * y;
+        gz += gf * z;
+      }
     }
     utot += u;
     double dot = gD[3 * i] * gx + gD[3 * i + 1] * gy + gD[3 * i + 2] * gz;
@@ -497,10 +579,10 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
       random_rotation(A.seed, (unsigned)w, A.step, rot);
     }
     for (int a = 0; a < S.na; a++) {
-      const int pk = S.atom_pp[a];
+      const int pk = cptr(S.atom_pp)[a];
       if (pk < 0) continue;
-      const PPType& P = S.pp[pk];
-      const double* ap = S.atom_pos + 3 * a;
+      const QW_CONST PPType& P = cptr(S.pp)[pk];
+      const QW_CONST double* ap = cptr(S.atom_pos) + 3 * a;
       for (int e0 = 0; e0 < ne; e0 += kWave) {
         const int e = e0 + lane;
         bool want = false;
@@ -508,7 +590,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
         if (e < ne) {
           double x = pos[3 * e] - ap[0], y = pos[3 * e + 1] - ap[1], z = pos[3 * e + 2] - ap[2];
           double r = sqrt(x * x + y * y + z * z);
-          pp_radial(P, S.atom_Z[a], r, vl);
+          pp_radial(P, cptr(S.atom_Z)[a], r, vl);
           if (A.pp_mode == 0) {
             want = r < P.cutoff;
           } else {
@@ -531,7 +613,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
           double od[3] = {re[0] - ap[0], re[1] - ap[1], re[2] - ap[2]};
           double orr = sqrt(od[0] * od[0] + od[1] * od[1] + od[2] * od[2]);
           double v[kMaxPPL];
-          pp_radial(P, S.atom_Z[a], orr, v);
+          pp_radial(P, cptr(S.atom_Z)[a], orr, v);
           if (A.pp_mode == 1) {
             double strength = 0;
             for (int l = 0; l < P.L - 1; l++) strength += 10.0 * (2 * l + 1) * fabs(v[l]);

@@ -108,68 +108,14 @@ __device__ inline int basis_lap_sparse_f32(const DevSys& S, const MixedTables& M
   return base;
 }
 
-// FP32 Jastrow terms of electron e at p: value, gradient (and laplacian when lap != nullptr).
-__device__ inline float jobj_lap_f(const JObj& o, const double* cf, float r, float& u, float& gf, float& lap) {
-  if (r >= (float)o.rcut) return 0.f;
-  if (o.kind == 0) {
-    float rci = (float)o.rcutinv, g = (float)o.gamma;
-    float zz = r * rci, zz2 = zz * zz, pp = zz - zz2 + zz * zz2 * (1.f / 3.f);
-    float pade = 1.f / (1.f + g * pp), pade2 = pade * pade;
-    float ppd = 1.f - 2.f * zz + zz2, ppdd = -2.f + 2.f * zz;
-    float dadr = ppd * pade2 / r;
-    float dadr2 = ppdd * pade2 * rci - 2.f * g * ppd * ppd * pade2 * pade * rci + 2.f * dadr;
-    float c = (float)cf[0], cu = (float)o.cusp;
-    u += c * (cu * (float)o.rcut * (pp * pade - (float)o.pade0));
-    gf += c * (cu * dadr);
-    lap += c * (cu * dadr2);
-    return 0.f;
-  }
-  float rc = (float)o.rcut;
-  float zz = r / rc, zz2 = zz * zz, zpp = zz2 * (6.f - 8.f * zz + 3.f * zz2), zpp1 = 1.f - zpp;
-  float zppd = 12.f * zz * (1.f - 2.f * zz + zz2), zppdd2 = 6.f - 24.f * zz + 18.f * zz2;
-  float irr = 1.f / (r * rc), irc2 = 1.f / (rc * rc);
-  for (int i = 0; i < o.nf; i++) {
-    float c = (float)cf[i];
-    if (c == 0.f) continue;
-    float b = (float)o.beta[i];
-    float zpade = 1.f / (1.f + b * zpp), zpade2 = zpade * zpade;
-    float crsd = -zppd * zpade2 * (b + 1.f) * irr;
-    float crsdd2 = -zpade2 * (zppdd2 - b * zppd * zppd * zpade) * (b + 1.f) * irc2;
-    u += c * (zpp1 * zpade);
-    gf += c * crsd;
-    lap += c * (2.f * (crsdd2 + crsd));
-  }
-  return 0.f;
+// FP32 Jastrow value and gradient of electron e at p (distances from FP64 positions);
+// epos: atom positions in e-i type order with stride estride.
+__device__ inline float jastrow_e_f(const DevSys& S, int e, const double p[3], const double* pos, float g[3], int lane,
+                                    const double* epos, int estride) {
+  return jastrow_e_t<float, false>(S, S.eef, S.eitf, e, p, pos, g, nullptr, lane, epos, estride);
 }
-
 __device__ inline float jastrow_e_f(const DevSys& S, int e, const double p[3], const double* pos, float g[3], int lane) {
-  float u = 0, gx = 0, gy = 0, gz = 0, l = 0;
-  for (int a = lane; a < S.na; a += kWave) {
-    const double* ap = S.atom_pos + 3 * a;
-    float x = (float)(p[0] - ap[0]), y = (float)(p[1] - ap[1]), z = (float)(p[2] - ap[2]);
-    float r = sqrtf(x * x + y * y + z * z);
-    float gf = 0;
-    for (int o = S.atom_ei0[a]; o < S.atom_ei0[a + 1]; o++) jobj_lap_f(S.ei[o], S.ei[o].cl, r, u, gf, l);
-    gx += gf * x;
-    gy += gf * y;
-    gz += gf * z;
-  }
-  const bool eup = e < S.nup;
-  for (int j = lane; j < S.ne; j += kWave) {
-    if (j == e) continue;
-    float x = (float)(p[0] - pos[3 * j]), y = (float)(p[1] - pos[3 * j + 1]), z = (float)(p[2] - pos[3 * j + 2]);
-    float r = sqrtf(x * x + y * y + z * z);
-    bool unlike = (j < S.nup) != eup;
-    float gf = 0;
-    for (int o = 0; o < S.nee; o++) jobj_lap_f(S.ee[o], unlike ? S.ee[o].cu : S.ee[o].cl, r, u, gf, l);
-    gx += gf * x;
-    gy += gf * y;
-    gz += gf * z;
-  }
-  g[0] = wsumf(gx);
-  g[1] = wsumf(gy);
-  g[2] = wsumf(gz);
-  return wsumf(u);
+  return jastrow_e_f(S, e, p, pos, g, lane, S.eit_pos, 3);
 }
 
 // LCAO of one point in the mixed path.  On return lane j < n holds phi[c] (c = value,
@@ -285,19 +231,35 @@ __global__ __launch_bounds__(64) void k_mo_rows_mx(DevSys S, MixedTables M, DevW
 }
 
 // ---------------------------------------------------------------- k_sweep_mx
+// Basis functions per LDS tile batch in the FP32 sweep.
+constexpr int kCapF32 = 256;
+
+// LDS of k_sweep_mx: positions | system tables | basis tile (FP32 sparse: 5 x cap floats +
+// cap row offsets; MFMA: 5 x ldchi floats + 5 x 64 accumulator exchange + ldchi ints) |
+// inverse row and value row broadcast (2 x NS floats).
+__host__ __device__ inline size_t sweep_mx_lds_bytes(int ne, int na, int nsh, int ntab, int ns, int mode, int ldchi) {
+  const size_t tile = mode == LCAO_FP32 ? (size_t)5 * kCapF32 * 4 + (size_t)kCapF32 * 4
+                                        : (size_t)(5 * ldchi + 5 * 64) * 4 + (size_t)ldchi * 4;
+  return (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15) + ((sys_lds_bytes(na, nsh, ntab) + 15) & ~(size_t)15) +
+         tile + (size_t)2 * ns * 4 + 16;
+}
+
 template <int NS, int DEPTH, int MODE>
-__global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W, SweepArgs A) {
+__global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W, SweepArgs A) {
   extern __shared__ double ldsd[];
   const int lane = threadIdx.x, w = blockIdx.x;
   if (w >= W.nw) return;
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
   double* pos = ldsd;
-  float* chi = (float*)(pos + 3 * ne);
-  float* phis = chi + 5 * M.ldchi;
-  float* rowb = phis + 5 * 64;
+  SysLds V;
+  char* q = sys_lds_carve(S, (char*)ldsd + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
+  float* chi = (float*)q;
+  float* phis = chi + 5 * (MODE == LCAO_FP32 ? kCapF32 : M.ldchi);  // MFMA modes only
+  int* flist = (int*)(MODE == LCAO_FP32 ? phis : phis + 5 * 64);
+  float* rowb = (float*)(flist + (MODE == LCAO_FP32 ? kCapF32 : M.ldchi));
   float* phib = rowb + NS;
-  int* flist = (int*)(phib + NS);
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  stage_sys(S, V, lane);
   float inv[NS];
   const float* ig = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
 #pragma unroll
@@ -306,6 +268,13 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
   int nacc = 0;
   float* mog = W.mof + (size_t)w * ne * 5 * nsp;
   const double tau = A.tau, stau = sqrt(tau);
+  float m1 = 0, m2 = 0, m3 = 0;
+  if (lane < n && A.k0 < A.k1) {
+    const size_t e0 = off + A.k0;
+    m1 = mog[(e0 * 5 + 1) * nsp + lane];
+    m2 = mog[(e0 * 5 + 2) * nsp + lane];
+    m3 = mog[(e0 * 5 + 3) * nsp + lane];
+  }
   __syncthreads();
   for (int k = A.k0; k < A.k1; k++) {
     const int e = off + k;
@@ -316,14 +285,8 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
     }
     __syncthreads();
     const float rk = lane < n ? rowb[lane] : 0.f;
-    float m1 = 0, m2 = 0, m3 = 0;
-    if (lane < n) {
-      m1 = mog[((size_t)e * 5 + 1) * nsp + lane];
-      m2 = mog[((size_t)e * 5 + 2) * nsp + lane];
-      m3 = mog[((size_t)e * 5 + 3) * nsp + lane];
-    }
     float gj0[3];
-    const float u0 = jastrow_e_f(S, e, r0, pos, gj0, lane);
+    const float u0 = jastrow_e_f(S, e, r0, pos, gj0, lane, V.epos, 4);
     double d0[3] = {(double)(wsumf(rk * m1) + gj0[0]), (double)(wsumf(rk * m2) + gj0[1]), (double)(wsumf(rk * m3) + gj0[2])};
     float phi[5];
     double r1[3], d1[3], ln1 = 0;
@@ -336,11 +299,14 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
       gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
-      lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, flist, phi, lane);
+      if constexpr (MODE == LCAO_FP32)
+        mo_lap_point<float, float, 8>(S, V, M.spline, M.Cf[s], n, rn, chi, flist, kCapF32, phi, lane);
+      else
+        lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, flist, phi, lane);
       float t0 = wsumf(rk * phi[0]);
       float t1 = wsumf(rk * phi[1]), t2 = wsumf(rk * phi[2]), t3 = wsumf(rk * phi[3]);
       float gj[3];
-      float un = jastrow_e_f(S, e, rn, pos, gj, lane);
+      float un = jastrow_e_f(S, e, rn, pos, gj, lane, V.epos, 4);
       float it0 = 1.f / t0;
       dn[0] = (double)(t1 * it0 + gj[0]);
       dn[1] = (double)(t2 * it0 + gj[1]);
@@ -394,6 +360,12 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
       nacc++;
       if (lane < 3) pos[3 * e + lane] = r1[lane];
     }
+    if (lane < n && k + 1 < A.k1) {
+      const size_t e1 = e + 1;
+      m1 = mog[(e1 * 5 + 1) * nsp + lane];
+      m2 = mog[(e1 * 5 + 2) * nsp + lane];
+      m3 = mog[(e1 * 5 + 3) * nsp + lane];
+    }
     __syncthreads();
   }
   float* igw = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
@@ -402,6 +374,7 @@ __global__ __launch_bounds__(64) void k_sweep_mx(DevSys S, MixedTables M, DevWal
     for (int j = 0; j < NS; j++)
       if (j < n) igw[(size_t)lane * nsp + j] = inv[j];
   }
+  if (A.dets) write_det_parts<NS>(S, W, w, s, inv, (const float*)mog, lane);
   for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
   if (lane == 0) {
     W.logdet[2 * w + s] = logdet;

@@ -70,9 +70,8 @@ namespace {
 size_t lds_mx_rows(const qw_engine& e) {
   return (5 * (size_t)e.MT.ldchi + 5 * 64) * sizeof(float) + (size_t)e.MT.ldchi * sizeof(int);
 }
-size_t lds_mx_sweep(const qw_engine& e) {
-  return 3 * (size_t)e.S.ne * sizeof(double) + (5 * (size_t)e.MT.ldchi + 5 * 64 + 2 * (size_t)e.ns) * sizeof(float) +
-         (size_t)e.MT.ldchi * sizeof(int);
+size_t lds_mx_sweep(const qw_engine& e, int mode) {
+  return sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, mode, e.MT.ldchi);
 }
 
 // FP64 recompute of the FP32 inverses of spin `only_spin` (or both when -1) from the current
@@ -90,7 +89,9 @@ void mixed_rows_t(qw_engine& e) {
 template <int DEPTH, int MODE>
 void mixed_sweep_t(qw_engine& e, const SweepArgs& A) {
   auto go = [&](auto kern) {
-    e.launch("k_sweep", [&] { hipLaunchKernelGGL(kern, dim3(e.nw), dim3(64), lds_mx_sweep(e), e.stream, e.S, e.MT, e.W, A); });
+    e.launch("k_sweep", [&] {
+      hipLaunchKernelGGL(kern, dim3(e.nw), dim3(64), lds_mx_sweep(e, MODE), e.stream, e.S, e.MT, e.W, A);
+    });
   };
   switch (e.ns) {
     case 4: go(k_sweep_mx<4, DEPTH, MODE>); break;
@@ -119,6 +120,7 @@ void mixed_sweep(qw_engine& e, const SweepArgs& A0, int depth) {
     SweepArgs A = A0;
     A.k0 = k0;
     A.k1 = std::min(A0.k1, k0 + e.refresh);
+    A.dets = A0.dets && A.k1 == A0.k1;
     if (depth == 1) {
       if (mode == LCAO_FP32) mixed_sweep_t<1, LCAO_FP32>(e, A);
       else if (mode == LCAO_BF16X6) mixed_sweep_t<1, LCAO_BF16X6>(e, A);
