@@ -121,6 +121,50 @@ qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* eblock, double* 
 qw_status qw_kernel_times(const qw_engine* e, char* buf, int buflen, double* avg_ms, int* count,
                           int n, int* nk);
 
+
+/* Device bytes the engine allocates for a population of nw walkers (walker state, MO caches,
+ * inverses), excluding the system tables already resident.  Used for admission by HBM
+ * budget.  No reference counterpart: the reference sizes nothing on a device. */
+qw_status qw_engine_walker_bytes(const qw_engine* e, int nw, unsigned long long* bytes);
+
+/* ---------------------------------------------------------------- serving loop
+ * A request loop over the engine: VMC/DMC jobs (decks of different sizes) are submitted at
+ * any time, admitted in arrival order while the sum of their device bytes fits the HBM budget
+ * (and fewer than max_active run), and run concurrently, each on its own engine, stream and
+ * host thread; a finished job frees its memory and the next pending job is admitted.  A job's
+ * results depend only on its own deck, seed and method, so they are the same alone and in a
+ * mix.  No reference counterpart: QWalk runs one method per process
+ * (src/program_utils/ooqmc.cpp:120-168); this is the batching/serving layer above it. */
+typedef struct qw_server qw_server;
+
+typedef struct qw_server_opts {
+  int device;
+  unsigned long long hbm_budget;  /* bytes of walker state admitted at once (0 = 64 GiB) */
+  int max_active;                 /* jobs running at once (0 = 8) */
+} qw_server_opts;
+
+typedef struct qw_job_desc {
+  int precision;                  /* QW_PREC_* */
+  unsigned long long seed;
+  int nwalkers;
+  int equil_steps;                /* VMC steps from the random guess before the method (0 = none) */
+  qw_method_desc method;          /* nconfig is ignored (nwalkers is used) */
+} qw_job_desc;
+
+/* stats of a finished job (seconds on the server's clock, from qw_server_create) */
+#define QW_JOB_STATS 8   /* 0 submit, 1 admitted, 2 finished, 3 latency (finished - submit),
+                            4 electron moves, 5 run seconds, 6 electron-moves/s, 7 device bytes */
+
+qw_status qw_server_create(const qw_server_opts* opts, qw_server** out);
+/* Queue a job; the deck is read when the job is admitted.  *job_id >= 1. */
+qw_status qw_server_submit(qw_server* s, const char* deck_path, const qw_job_desc* job, long long* job_id);
+/* Block until the job has finished; eblock[method.nblock][QW_BLOCK_FIELDS], stats[QW_JOB_STATS].
+ * Returns the job's own status (its error message from qw_server_error). */
+qw_status qw_server_wait(qw_server* s, long long job_id, double* eblock, double* stats);
+const char* qw_server_error(qw_server* s, long long job_id);
+/* Waits for all jobs, then releases the server. */
+void qw_server_destroy(qw_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,348 @@
+// Walker kernels for more than 64 electrons of one spin (up to 256): one 256-thread block
+// (four wavefronts) per walker.  The inverse stays in device memory (too large for one
+// wavefront's registers) and each Sherman-Morrison update streams it: thread i owns row i of
+// the transposed inverse, as lane i does in the register-resident kernels.  Wavefront 0
+// evaluates the basis (two-pass sparse, into the LDS tile) and the Jastrow sums; all threads
+// contract the tile with their MO's coefficients (thread j = MO j).
+//
+//   k_sweep_big<TI, R>  moves of one spin channel (src/method/Split_sample.cpp:302-540,
+//                       src/program_utils/MatrixAlgebrac.cpp:766-838); TI = inverse / MO cache
+//                       element type, R = Jastrow arithmetic
+//   k_mo_rows_big<T>    basis + LCAO of every electron (src/orbitals/MO_matrix_cutoff.h:480-560)
+//   k_lu_gather / k_lu_logdet / k_lu_scatter
+//                       around the batched LU and inverse of the device solver library: the
+//                       Slater matrices are gathered in column-major order so that the inverse
+//                       comes back as the transposed inverse the kernels use
+//                       (src/program_utils/MatrixAlgebrac.cpp:316-421)
+#pragma once
+#include "qw_kernels.hpp"
+
+namespace qw {
+
+constexpr int kBigThreads = 256;
+constexpr int kBigMaxSpin = 256;
+
+// Spline table element type that goes with an inverse / MO cache element type.
+template <typename TI>
+struct SplineOf {
+  using type = double;
+};
+template <>
+struct SplineOf<float> {
+  using type = float;
+};
+template <typename TI>
+using TS_SPLINE_T = typename SplineOf<TI>::type;
+
+// Jastrow objects of the arithmetic type R.
+template <typename R>
+__device__ inline const JRad<R>* jee(const DevSys& S);
+template <>
+__device__ inline const JRad<double>* jee<double>(const DevSys& S) { return S.ee; }
+template <>
+__device__ inline const JRad<float>* jee<float>(const DevSys& S) { return S.eef; }
+template <typename R>
+__device__ inline const JRad<R>* jei(const DevSys& S);
+template <>
+__device__ inline const JRad<double>* jei<double>(const DevSys& S) { return S.eit; }
+template <>
+__device__ inline const JRad<float>* jei<float>(const DevSys& S) { return S.eitf; }
+
+// Sum of NV values over the block (256 threads), the same in every thread.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* red, int t) {
+  const int wv = t >> 6, lane = t & 63;
+#pragma unroll
+  for (int c = 0; c < NV; c++) v[c] = wsum(v[c]);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < NV; c++) red[wv * 8 + c] = v[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NV; c++) v[c] = (red[c] + red[8 + c]) + (red[16 + c] + red[24 + c]);
+  __syncthreads();
+}
+
+__host__ __device__ inline size_t sweep_big_lds_bytes(int ne, int na, int nsh, int ntab, int nsp, int cap, int tsize) {
+  return (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15) + ((sys_lds_bytes(na, nsh, ntab) + 15) & ~(size_t)15) +
+         (size_t)5 * cap * tsize + (size_t)cap * sizeof(int) + (size_t)2 * nsp * tsize + 48 * sizeof(double) + 64;
+}
+
+// Pointers of one precision for the big kernels.
+template <typename TI>
+struct BigPtrs {
+  TI* inv;           // [nw][2][nsp][nsp]
+  TI* mo;            // [nw][ne][5][nsp]
+  const TI* C[2];    // [nb][nspin] coefficients
+  const void* spline;  // double or float table, matching TI
+};
+
+template <typename TI, typename R, int DEPTH>
+__global__ __launch_bounds__(256) void k_sweep_big(DevSys S, DevWalkers W, BigPtrs<TI> P, SweepArgs A, int cap) {
+  extern __shared__ double lds[];
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63, w = blockIdx.x;
+  if (w >= W.nw) return;
+  const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
+  double* pos = lds;
+  SysLds V;
+  char* q = sys_lds_carve(S, (char*)lds + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
+  TI* chi = (TI*)q;
+  int* roff = (int*)(chi + 5 * cap);
+  TI* rowk = (TI*)(roff + cap);
+  TI* phi0 = rowk + nsp;
+  double* red = (double*)(((size_t)(phi0 + nsp) - (size_t)lds + 15) / 16 * 16 + (char*)lds);
+  double* bc = red + 32;  // broadcast slots: nact, take, nfun, Jastrow u and gradient
+  for (int i = t; i < 3 * ne; i += kBigThreads) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  if (wv == 0) stage_sys(S, V, lane);
+  TI* ig = P.inv + ((size_t)w * 2 + s) * nsp * nsp;
+  TI* mog = P.mo + (size_t)w * ne * 5 * nsp;
+  const TS_SPLINE_T<TI>* spline = (const TS_SPLINE_T<TI>*)P.spline;
+  double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
+  int nacc = 0;
+  const double tau = A.tau, stau = sqrt(tau);
+  __syncthreads();
+  for (int k = A.k0; k < A.k1; k++) {
+    const int e = off + k;
+    const double r0[3] = {pos[3 * e], pos[3 * e + 1], pos[3 * e + 2]};
+    const double rk = t < n ? (double)ig[(size_t)k * nsp + t] : 0.0;
+    double gd[3] = {0, 0, 0};
+    if (t < n)
+      for (int c = 0; c < 3; c++) gd[c] = rk * (double)mog[((size_t)e * 5 + 1 + c) * nsp + t];
+    block_sum<3>(gd, red, t);
+    if (wv == 0) {
+      R g[3];
+      const R u = jastrow_e_t<R, false>(S, jee<R>(S), jei<R>(S), e, r0, pos, g, nullptr, lane, V.epos, 4);
+      if (lane == 0) {
+        bc[3] = (double)u;
+        bc[4] = (double)g[0];
+        bc[5] = (double)g[1];
+        bc[6] = (double)g[2];
+      }
+    }
+    __syncthreads();
+    const double u0 = bc[3];
+    const double d0[3] = {gd[0] + bc[4], gd[1] + bc[5], gd[2] + bc[6]};
+    __syncthreads();
+    double phi[5];
+    double r1[3], d1[3], ln1 = 0, Rt = 0;
+    bool accepted = false;
+    double acc_first = 0;
+    for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
+      const double tt = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
+      double g[3], dr[3], rn[3], dn[3];
+      gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      lim_drift(d0, tt, dr);
+      for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
+      // basis (wavefront 0) and contraction (thread = MO)
+      TI acc[5] = {0, 0, 0, 0, 0};
+      if (wv == 0) {
+        const int na_ = basis_active(S, V, rn, lane);
+        if (lane == 0) bc[0] = na_;
+      }
+      __syncthreads();
+      const int nact = (int)bc[0];
+      for (int i0 = 0; i0 < nact;) {
+        if (wv == 0) {
+          int nfun;
+          const int take = basis_batch_lap<TI, TS_SPLINE_T<TI>>(S, V, spline, rn, i0, nact, n, chi, cap, roff, cap,
+                                                                &nfun, lane);
+          if (lane == 0) {
+            bc[1] = take;
+            bc[2] = nfun;
+          }
+        }
+        __syncthreads();
+        const int take = (int)bc[1], nfun = (int)bc[2];
+        if (t < n) lcao_batch<TI, 4>(P.C[s], roff, chi, cap, nfun, acc, t);
+        __syncthreads();
+        i0 += take;
+      }
+      for (int c = 0; c < 5; c++) phi[c] = (double)acc[c];
+      double tv[4] = {rk * phi[0], rk * phi[1], rk * phi[2], rk * phi[3]};
+      block_sum<4>(tv, red, t);
+      const double t0 = tv[0];
+      if (wv == 0) {
+        R g2[3];
+        const R u = jastrow_e_t<R, false>(S, jee<R>(S), jei<R>(S), e, rn, pos, g2, nullptr, lane, V.epos, 4);
+        if (lane == 0) {
+          bc[3] = (double)u;
+          bc[4] = (double)g2[0];
+          bc[5] = (double)g2[1];
+          bc[6] = (double)g2[2];
+        }
+      }
+      __syncthreads();
+      const double un = bc[3];
+      dn[0] = tv[1] / t0 + bc[4];
+      dn[1] = tv[2] / t0 + bc[5];
+      dn[2] = tv[3] / t0 + bc[6];
+      __syncthreads();
+      const double lnn = log(fabs(t0)) + (un - u0);
+      const bool neg = t0 < 0;
+      double acc_p;
+      if (dep == 1) {
+        const double rr = exp(lnn);
+        acc_p = (A.restrict_nodes && neg) ? 0.0
+                                          : fmin(1.0, rr * rr * exp(trans(rn, dn, r0, tt) - trans(r0, d0, rn, tt)));
+        acc_first = acc_p;
+        for (int d = 0; d < 3; d++) r1[d] = rn[d], d1[d] = dn[d];
+        ln1 = lnn;
+      } else {
+        const double rr = exp(lnn);
+        const double pt =
+            (trans(rn, dn, r1, tau) - trans(r0, d0, r1, tau)) + (trans(rn, dn, r0, tt) - trans(r0, d0, rn, tt));
+        const double r21 = exp(ln1 - lnn);
+        const double a21 = fmin(1.0, r21 * r21 * exp(trans(r1, d1, rn, tau) - trans(rn, dn, r1, tau)));
+        const double num = 1 - a21, den = 1 - acc_first;
+        if (A.restrict_nodes && neg) acc_p = 0;
+        else if (fabs(den) < 1e-10) acc_p = fabs(num) < 1e-10 ? 0.0 : 1.0;
+        else acc_p = fmin(1.0, rr * rr * exp(pt) * num / den);
+      }
+      const double uu =
+          uniform(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
+      if (acc_p + uu > 1.0) {
+        accepted = true;
+        Rt = t0;
+        for (int d = 0; d < 3; d++) r1[d] = rn[d];
+      }
+    }
+    if (accepted) {
+      // Sherman-Morrison on the transposed inverse in device memory: thread i owns row i
+      if (t < nsp) {
+        phi0[t] = t < n ? (TI)phi[0] : (TI)0;
+        rowk[t] = t < n ? (TI)rk : (TI)0;
+      }
+      __syncthreads();
+      if (t < n) {
+        TI* row = ig + (size_t)t * nsp;
+        if (t == k) {
+          const double f = 1.0 / Rt;
+          for (int j = 0; j < n; j++) row[j] = (TI)((double)rowk[j] * f);
+        } else {
+          double tk = 0;
+          for (int j = 0; j < n; j++) tk += (double)row[j] * (double)phi0[j];
+          const double pk = tk * (-1.0 / Rt);
+          for (int j = 0; j < n; j++) row[j] = (TI)((double)row[j] + (double)rowk[j] * pk);
+        }
+        for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + t] = (TI)phi[c];
+      }
+      logdet += log(fabs(Rt));
+      if (Rt < 0) sgn = -sgn;
+      nacc++;
+      if (t < 3) pos[3 * e + t] = r1[t];
+    }
+    // the device-memory inverse rows written above are read by other threads next move
+    __threadfence_block();
+    __syncthreads();
+  }
+  for (int i = t; i < 3 * ne; i += kBigThreads) W.pos[(size_t)w * ne * 3 + i] = pos[i];
+  if (t == 0) {
+    W.logdet[2 * w + s] = logdet;
+    W.sgn[2 * w + s] = sgn;
+    W.acc[w] += nacc;
+  }
+  if (A.dets && t < n) {
+    // determinant parts of the local energy for this thread's electron
+    const TI* row = ig + (size_t)t * nsp;
+    const TI* mr = mog + (size_t)(off + t) * 5 * nsp;
+    double dv[4] = {0, 0, 0, 0};
+    for (int j = 0; j < n; j++) {
+      const double iv = (double)row[j];
+      for (int c = 0; c < 4; c++) dv[c] += (double)mr[(size_t)(c + 1) * nsp + j] * iv;
+    }
+    double* o = W.dgl + ((size_t)w * ne + off + t) * 4;
+    for (int c = 0; c < 4; c++) o[c] = dv[c];
+  }
+}
+
+// Basis + LCAO of electron e of walker w (one 256-thread block per electron): wavefront 0
+// evaluates the active functions chunk by chunk, every thread accumulates its MO.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mo_rows_big(DevSys S, const double* wpos, const T* C0, const T* C1,
+                                                     T* mo_out, int nw) {
+  extern __shared__ double lds[];
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  const long blk = blockIdx.x;
+  const int w = (int)(blk / S.ne), e = (int)(blk % S.ne);
+  if (w >= nw) return;
+  const int s = e < S.nup ? 0 : 1, n = s ? S.ndn : S.nup, ld = S.chunk_nf, nsp = S.nsp;
+  const T* C = s ? C1 : C0;
+  T* chi = (T*)lds;
+  int* flist = (int*)(chi + 5 * ld);
+  int* nb = flist + ld;
+  double p[3];
+  for (int d = 0; d < 3; d++) p[d] = wpos[((size_t)w * S.ne + e) * 3 + d];
+  T acc[5] = {0, 0, 0, 0, 0};
+  for (int ch = 0; ch < S.nchunk; ch++) {
+    const int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
+    __syncthreads();
+    if (wv == 0) {
+      const int na_ = basis_lap_sparse<T>(S, p, sh0, sh1, chi, ld, flist, lane);
+      if (lane == 0) *nb = na_;
+    }
+    __syncthreads();
+    const int nact = *nb;
+    if (t < n) {
+      const T* Cp = C + t;
+      for (int b = 0; b < nact; b++) {
+        const T c = Cp[(size_t)flist[b] * n];
+        acc[0] += c * chi[b];
+        acc[1] += c * chi[ld + b];
+        acc[2] += c * chi[2 * ld + b];
+        acc[3] += c * chi[3 * ld + b];
+        acc[4] += c * chi[4 * ld + b];
+      }
+    }
+  }
+  if (t < n) {
+    T* row = mo_out + ((size_t)w * S.ne + e) * 5 * nsp;
+    for (int c = 0; c < 5; c++) row[(size_t)c * nsp + t] = acc[c];
+  }
+}
+
+// Column-major Slater matrices of spin s for the batched solver: work[w][c][r] = phi_c(r_r)
+// (value rows of the MO cache), so that the solver's inverse, read row-major, is the
+// transposed inverse inv(k, i) = A^-1(i, k).
+template <typename TIN>
+__global__ __launch_bounds__(256) void k_lu_gather(DevSys S, const TIN* mo, double* work, int nw, int s) {
+  const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
+  const size_t per = (size_t)n * n, i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nw * per) return;
+  const size_t w = i / per, rem = i % per, c = rem / n, r = rem % n;
+  work[i] = (double)mo[((w * S.ne + off + r) * 5) * nsp + c];
+}
+
+// log|det| and sign from the LU factors (diagonal of U, 1-based pivots); singular matrices
+// (solver info > 0) get the reference's -1e201.
+__global__ __launch_bounds__(256) void k_lu_logdet(const double* work, const int* ipiv, const int* info, double* logdet,
+                                                   double* sgn, int nw, int s, int n) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  if (info[w] != 0) {
+    logdet[2 * w + s] = -1e201;
+    sgn[2 * w + s] = 1;
+    return;
+  }
+  const double* a = work + (size_t)w * n * n;
+  double ld = 0;
+  int odd = 0;
+  for (int i = 0; i < n; i++) {
+    const double d = a[(size_t)i * n + i];
+    ld += log(fabs(d));
+    odd ^= (d < 0) ^ (ipiv[(size_t)w * n + i] != i + 1);
+  }
+  logdet[2 * w + s] = ld;
+  sgn[2 * w + s] = odd ? -1.0 : 1.0;
+}
+
+template <typename TI>
+__global__ __launch_bounds__(256) void k_lu_scatter(DevSys S, const double* work, const int* info, TI* inv, int nw,
+                                                    int s) {
+  const int n = s ? S.ndn : S.nup, nsp = S.nsp;
+  const size_t per = (size_t)n * n, i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nw * per) return;
+  const size_t w = i / per, rem = i % per, k = rem / n, c = rem % n;
+  inv[((w * 2 + s) * nsp + k) * nsp + c] = info[w] ? (TI)0 : (TI)work[i];
+}
+
+}  // namespace qw

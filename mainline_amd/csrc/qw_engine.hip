@@ -12,7 +12,10 @@
 #include <string>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "../../include/qwalk_amd.h"
+#include "qw_big.hpp"
 #include "qw_kernels.hpp"
 #include "qw_mixed.hpp"
 #include "qw_setup.hpp"
@@ -106,9 +109,18 @@ struct qw_engine {
   std::vector<double> ktime;
   std::vector<int> kcount;
 
+  // more than 64 electrons of one spin: 256-thread walker blocks, solver-library refresh
+  bool big = false;
+  rocblas_handle rb = nullptr;
+  double* luw = nullptr;
+  int* lupiv = nullptr;
+  int* luinfo = nullptr;
+  int lucap = 0;
+
   ~qw_engine() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    if (rb) (void)rocblas_destroy_handle(rb);
     for (void* p : sys_allocs) (void)hipFree(p);
     for (void* p : walker_allocs) (void)hipFree(p);
     for (auto& e : evs) pool.push_back(e.a), pool.push_back(e.b);
@@ -291,6 +303,22 @@ struct qw_engine {
     dred = upload(std::vector<double>(kBsum + 1));
   }
 
+  // device bytes of ensure_walkers(n) (keep in step with it)
+  unsigned long long walker_bytes(int n) const {
+    const unsigned long long ne = S.ne, nsp = S.nsp, N = n;
+    const unsigned long long es = precision == QW_PREC_FP64 ? 8 : 4;
+    unsigned long long b = N * ne * 3 * 8 + N * 2 * nsp * nsp * es + N * ne * 5 * nsp * es;
+    b += N * (2 * 8 + 2 * 8 + 8 + 8 + 4 * 8 + 4 + kBsum * 8) + N * ne * 4 * 8;
+    unsigned long long P = 1;
+    while (P < N) P <<= 1;
+    b += P * (8 + 4) + 2 * (N / 2 + 1) * 4;
+    if (big) {
+      const unsigned long long nm = std::max(T.nup, T.ndn);
+      b += N * nm * nm * 8 + N * nm * 4 + N * 4;
+    }
+    return b;
+  }
+
   void ensure_walkers(int n) {
     if (n <= cap) {
       W.nw = n;
@@ -300,6 +328,10 @@ struct qw_engine {
     HIPCHK(hipStreamSynchronize(stream));
     for (void* p : walker_allocs) (void)hipFree(p);
     walker_allocs.clear();
+    luw = nullptr;  // the solver buffers are walker allocations too
+    lupiv = nullptr;
+    luinfo = nullptr;
+    lucap = 0;
     const size_t ne = S.ne, nsp = S.nsp;
     W.pos = walloc<double>((size_t)n * ne * 3);
     if (precision == QW_PREC_FP64) {
@@ -308,9 +340,9 @@ struct qw_engine {
       W.invf = nullptr;
       W.mof = nullptr;
     } else {
-      // the FP64 MO value matrix is kept for the periodic FP64 LU recompute
-      W.mo = walloc<double>((size_t)n * ne * 5 * nsp);
-      W.inv = walloc<double>((size_t)n * 2 * nsp * nsp);
+      // FP32 caches only: the FP64 inverse recompute reads the FP32 MO values directly
+      W.mo = nullptr;
+      W.inv = nullptr;
       W.invf = walloc<float>((size_t)n * 2 * nsp * nsp);
       W.mof = walloc<float>((size_t)n * ne * 5 * nsp);
     }
@@ -364,6 +396,10 @@ struct qw_engine {
     }
   }
   void refresh_all() {
+    if (big) {
+      big_refresh_all();
+      return;
+    }
     if (precision == QW_PREC_FP64) {
       run_mo_rows();
       run_inv<double, double>(W.mo, W.inv, -1);
@@ -373,6 +409,10 @@ struct qw_engine {
   }
   void run_sweep(int s, int k0, int k1, double tau, unsigned sweep, int depth, int restrict_nodes, int dets) {
     SweepArgs A{s, k0, k1, tau, seed, step, sweep, restrict_nodes, dets};
+    if (big) {
+      big_sweep(A, depth);
+      return;
+    }
     if (precision != QW_PREC_FP64) {
       mixed_sweep(*this, A, depth);
       return;
@@ -405,6 +445,87 @@ struct qw_engine {
     }
     launch("k_local", [&] { hipLaunchKernelGGL(k_local<double>, dim3(nw), dim3(64), lds_local(), stream, S, W, A); });
   }
+  // ---- more than 64 electrons per spin
+  void lu_buffers() {
+    const int n = std::max(T.nup, T.ndn);
+    if (lucap >= nw && luw) return;
+    if (!rb) {
+      if (rocblas_create_handle(&rb) != rocblas_status_success) throw DevErr("rocblas_create_handle failed");
+      if (rocblas_set_stream(rb, stream) != rocblas_status_success) throw DevErr("rocblas_set_stream failed");
+    }
+    luw = walloc<double>((size_t)nw * n * n);
+    lupiv = walloc<int>((size_t)nw * n);
+    luinfo = walloc<int>(nw);
+    lucap = nw;
+  }
+  // FP64 LU + inverse of spin s from the MO value rows (FP64 or FP32 cache)
+  void lu_refresh(int s) {
+    const int n = s ? T.ndn : T.nup;
+    if (n == 0) return;
+    lu_buffers();
+    const size_t tot = (size_t)nw * n * n;
+    const unsigned g = (unsigned)((tot + 255) / 256);
+    launch("k_inv", [&] {
+      if (precision == QW_PREC_FP64)
+        hipLaunchKernelGGL(k_lu_gather<double>, dim3(g), dim3(256), 0, stream, S, (const double*)W.mo, luw, nw, s);
+      else
+        hipLaunchKernelGGL(k_lu_gather<float>, dim3(g), dim3(256), 0, stream, S, (const float*)W.mof, luw, nw, s);
+      if (rocsolver_dgetrf_strided_batched(rb, n, n, luw, n, (rocblas_stride)n * n, lupiv, n, luinfo, nw) !=
+          rocblas_status_success)
+        throw DevErr("rocsolver_dgetrf_strided_batched failed");
+      hipLaunchKernelGGL(k_lu_logdet, dim3((nw + 255) / 256), dim3(256), 0, stream, (const double*)luw,
+                         (const int*)lupiv, (const int*)luinfo, W.logdet, W.sgn, nw, s, n);
+      if (rocsolver_dgetri_strided_batched(rb, n, luw, n, (rocblas_stride)n * n, lupiv, n, luinfo, nw) !=
+          rocblas_status_success)
+        throw DevErr("rocsolver_dgetri_strided_batched failed");
+      if (precision == QW_PREC_FP64)
+        hipLaunchKernelGGL(k_lu_scatter<double>, dim3(g), dim3(256), 0, stream, S, (const double*)luw,
+                           (const int*)luinfo, W.inv, nw, s);
+      else
+        hipLaunchKernelGGL(k_lu_scatter<float>, dim3(g), dim3(256), 0, stream, S, (const double*)luw,
+                           (const int*)luinfo, W.invf, nw, s);
+    });
+  }
+  void big_refresh_all() {
+    const size_t lds = 5 * (size_t)S.chunk_nf * sizeof(double) + (size_t)S.chunk_nf * sizeof(int) + 16;
+    launch("k_mo_rows", [&] {
+      if (precision == QW_PREC_FP64)
+        hipLaunchKernelGGL(k_mo_rows_big<double>, dim3((unsigned)((size_t)nw * S.ne)), dim3(256), lds, stream, S,
+                           (const double*)W.pos, S.C[0], S.C[1], W.mo, nw);
+      else
+        hipLaunchKernelGGL(k_mo_rows_big<float>, dim3((unsigned)((size_t)nw * S.ne)), dim3(256), lds, stream, S,
+                           (const double*)W.pos, MT.Cf[0], MT.Cf[1], W.mof, nw);
+    });
+    lu_refresh(0);
+    lu_refresh(1);
+  }
+  template <typename TI, typename R>
+  void big_sweep_t(const SweepArgs& A, int depth, const BigPtrs<TI>& P, int cap) {
+    const size_t lds = sweep_big_lds_bytes(S.ne, S.na, S.nsh, S.ntab, S.nsp, cap, sizeof(TI));
+    launch("k_sweep", [&] {
+      if (depth == 1)
+        hipLaunchKernelGGL((k_sweep_big<TI, R, 1>), dim3(nw), dim3(kBigThreads), lds, stream, S, W, P, A, cap);
+      else
+        hipLaunchKernelGGL((k_sweep_big<TI, R, 2>), dim3(nw), dim3(kBigThreads), lds, stream, S, W, P, A, cap);
+    });
+  }
+  void big_sweep(const SweepArgs& A0, int depth) {
+    if (precision == QW_PREC_FP64) {
+      BigPtrs<double> P{W.inv, W.mo, {S.C[0], S.C[1]}, S.spline};
+      big_sweep_t<double, double>(A0, depth, P, kCapF64);
+      return;
+    }
+    BigPtrs<float> P{W.invf, W.mof, {MT.Cf[0], MT.Cf[1]}, MT.spline};
+    for (int k0 = A0.k0; k0 < A0.k1; k0 += refresh) {
+      SweepArgs A = A0;
+      A.k0 = k0;
+      A.k1 = std::min(A0.k1, k0 + refresh);
+      A.dets = A0.dets && A.k1 == A0.k1;
+      big_sweep_t<float, float>(A, depth, P, kCapF32);
+      lu_refresh(A.s);
+    }
+  }
+
   void reduce(double* host, int reset) {
     launch("k_reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, stream, W, dred, reset); });
     HIPCHK(hipMemcpyAsync(host, dred, (kBsum + 1) * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -458,9 +579,22 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
     g_create_error = x.what();
     return unsup ? QW_ERR_UNSUPPORTED : QW_ERR_INPUT;
   }
-  if (std::max(e->T.nup, e->T.ndn) > 64) {
-    g_create_error = "more than 64 electrons of one spin are not supported by this engine build";
+  const int nmax = std::max(e->T.nup, e->T.ndn);
+  if (nmax > kBigMaxSpin) {
+    g_create_error = "more than 256 electrons of one spin are not supported by this engine build";
     return QW_ERR_UNSUPPORTED;
+  }
+  if (nmax > 64) {
+    // the block-per-walker path: no pseudopotential quadrature, no MFMA LCAO
+    if (!e->T.pptypes.empty()) {
+      g_create_error = "pseudopotentials with more than 64 electrons of one spin are not supported";
+      return QW_ERR_UNSUPPORTED;
+    }
+    if (e->precision == QW_PREC_MIXED_BF16 || e->precision == QW_PREC_MIXED_BF16X6) {
+      g_create_error = "the BF16 LCAO modes support up to 64 electrons of one spin";
+      return QW_ERR_UNSUPPORTED;
+    }
+    e->big = true;
   }
   try {
     int ndev = 0;
@@ -480,6 +614,12 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
 }
 
 extern "C" void qw_engine_destroy(qw_engine* e) { delete e; }
+
+extern "C" qw_status qw_engine_walker_bytes(const qw_engine* e, int nw, unsigned long long* bytes) {
+  if (!e || !bytes || nw < 0) return QW_ERR_ARG;
+  *bytes = e->walker_bytes(nw);
+  return QW_OK;
+}
 
 extern "C" const char* qw_last_error(const qw_engine* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
 

@@ -453,6 +453,52 @@ __device__ inline const double* inv_store<double>(const DevWalkers& W) { return 
 template <>
 __device__ inline const float* inv_store<float>(const DevWalkers& W) { return W.invf; }
 
+// Jastrow of electron i for the local energy: value (pairs i<j and its one-body terms),
+// gradient and laplacian with respect to r_i, in arithmetic R (distances from FP64 positions).
+// Object loops outermost so each object's parameters stay in scalar registers.
+template <typename R>
+__device__ inline void jastrow_local(const DevSys& S, const JRad<R>* ee, const JRad<R>* eit, const double* pos, int i,
+                                     R& u, R& gx, R& gy, R& gz, R& l) {
+  const double* pi = pos + 3 * i;
+  const QW_CONST int* ea0 = cptr(S.eit_a0);
+  const QW_CONST int* eo0 = cptr(S.eit_o0);
+  const QW_CONST double* epos = cptr(S.eit_pos);
+  for (int t = 0; t < S.n_eit; t++) {
+    const int a0 = ea0[t], a1 = ea0[t + 1], o1 = eo0[t + 1];
+    for (int o = eo0[t]; o < o1; o++) {
+      const QW_CONST JRad<R>& ob = cptr(eit)[o];
+      for (int k = a0; k < a1; k++) {
+        const QW_CONST double* ap = epos + 3 * k;
+        const R x = (R)(pi[0] - ap[0]), y = (R)(pi[1] - ap[1]), z = (R)(pi[2] - ap[2]);
+        R r, ri;
+        dist_inv(x * x + y * y + z * z, r, ri);
+        R gf = 0;
+        jrad_lap<R, true>(ob, false, r, ri, u, gf, l);
+        gx += gf * x;
+        gy += gf * y;
+        gz += gf * z;
+      }
+    }
+  }
+  const bool iup = i < S.nup;
+  for (int o = 0; o < S.nee; o++) {
+    const QW_CONST JRad<R>& ob = cptr(ee)[o];
+    for (int j = 0; j < S.ne; j++) {
+      if (j == i) continue;
+      const R x = (R)(pi[0] - pos[3 * j]), y = (R)(pi[1] - pos[3 * j + 1]), z = (R)(pi[2] - pos[3 * j + 2]);
+      R r, ri;
+      dist_inv(x * x + y * y + z * z, r, ri);
+      const bool unlike = (j < S.nup) != iup;
+      R gf = 0, upair = 0;
+      jrad_lap<R, true>(ob, unlike, r, ri, upair, gf, l);
+      if (j > i) u += upair;
+      gx += gf * x;
+      gy += gf * y;
+      gz += gf * z;
+    }
+  }
+}
+
 template <typename TM>
 __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs A) {
   extern __shared__ double lds[];
@@ -501,56 +547,32 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
   for (int i = lane; i < ne; i += kWave) {
     const double* pi = pos + 3 * i;
     double gx = 0, gy = 0, gz = 0, l = 0, u = 0;
-    // object loops outermost (object parameters loop-invariant over the partners)
-    const QW_CONST int* ea0 = cptr(S.eit_a0);
-    const QW_CONST int* eo0 = cptr(S.eit_o0);
+    // Coulomb (FP64)
     const QW_CONST double* epos = cptr(S.eit_pos);
-    for (int t = 0; t < S.n_eit; t++) {
-      const int a0 = ea0[t], a1 = ea0[t + 1], o0 = eo0[t], o1 = eo0[t + 1];
-      for (int k = a0; k < a1; k++) {
-        const QW_CONST double* ap = epos + 3 * k;
-        double x = pi[0] - ap[0], y = pi[1] - ap[1], z = pi[2] - ap[2];
-        double r, ri;
-        dist_inv(x * x + y * y + z * z, r, ri);
-        vei += cptr(S.eit_Z)[k] * ri;
-      }
-      for (int o = o0; o < o1; o++) {
-        const QW_CONST JRad<double>& ob = cptr(S.eit)[o];
-        for (int k = a0; k < a1; k++) {
-          const QW_CONST double* ap = epos + 3 * k;
-          double x = pi[0] - ap[0], y = pi[1] - ap[1], z = pi[2] - ap[2];
-          double r, ri;
-          dist_inv(x * x + y * y + z * z, r, ri);
-          double gf = 0;
-          jrad_lap<double, true>(ob, false, r, ri, u, gf, l);
-          gx += gf * x;
-          gy += gf * y;
-          gz += gf * z;
-        }
-      }
+    for (int k = 0; k < S.na; k++) {
+      const QW_CONST double* ap = epos + 3 * k;
+      double x = pi[0] - ap[0], y = pi[1] - ap[1], z = pi[2] - ap[2];
+      double r, ri;
+      dist_inv(x * x + y * y + z * z, r, ri);
+      vei += cptr(S.eit_Z)[k] * ri;
     }
-    const bool iup = i < S.nup;
     for (int j = 0; j < i; j++) {
       double x = pi[0] - pos[3 * j], y = pi[1] - pos[3 * j + 1], z = pi[2] - pos[3 * j + 2];
       double r, ri;
       dist_inv(x * x + y * y + z * z, r, ri);
       vee += ri;
     }
-    for (int o = 0; o < S.nee; o++) {
-      const QW_CONST JRad<double>& ob = cptr(S.ee)[o];
-      for (int j = 0; j < ne; j++) {
-        if (j == i) continue;
-        double x = pi[0] - pos[3 * j], y = pi[1] - pos[3 * j + 1], z = pi[2] - pos[3 * j + 2];
-        double r, ri;
-        dist_inv(x * x + y * y + z * z, r, ri);
-        const bool unlike = (j < S.nup) != iup;
-        double gf = 0, upair = 0;
-        jrad_lap<double, true>(ob, unlike, r, ri, upair, gf, l);
-        if (j > i) u += upair;
-        gx += gf * x;
-        gy += gf * y;
-        gz += gf * z;
-      }
+    // Jastrow: FP64 on the FP64 path, FP32 pair terms on the mixed path
+    if constexpr (sizeof(TM) == sizeof(double)) {
+      jastrow_local<double>(S, S.ee, S.eit, pos, i, u, gx, gy, gz, l);
+    } else {
+      float uf = 0, gxf = 0, gyf = 0, gzf = 0, lf = 0;
+      jastrow_local<float>(S, S.eef, S.eitf, pos, i, uf, gxf, gyf, gzf, lf);
+      u = uf;
+      gx = gxf;
+      gy = gyf;
+      gz = gzf;
+      l = lf;
     }
     utot += u;
     double dot = gD[3 * i] * gx + gD[3 * i + 1] * gy + gD[3 * i + 2] * gz;

@@ -32,6 +32,18 @@ class MethodDesc(ctypes.Structure):
                 ("pp_stochastic", ctypes.c_int)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_ulonglong), ("max_active", ctypes.c_int)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("nwalkers", ctypes.c_int),
+                ("equil_steps", ctypes.c_int), ("method", MethodDesc)]
+
+
+JOB_STATS = 8
+
+
 class QwError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__("%s (%s)" % (msg, STATUS.get(status, status)))
@@ -59,6 +71,14 @@ def lib():
         L.qw_walkers_random.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _dp]
         L.qw_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp,
                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.qw_engine_walker_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.qw_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(ctypes.c_void_p)]
+        L.qw_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(JobDesc),
+                                       ctypes.POINTER(ctypes.c_longlong)]
+        L.qw_server_wait.argtypes = [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp]
+        L.qw_server_error.restype = ctypes.c_char_p
+        L.qw_server_error.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
+        L.qw_server_destroy.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -69,7 +89,8 @@ def exported_symbols():
     names = []
     for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
               "qw_method_read", "qw_walkers_set", "qw_walkers_get", "qw_walkers_random", "qw_run",
-              "qw_kernel_times"]:
+              "qw_kernel_times", "qw_engine_walker_bytes", "qw_server_create", "qw_server_submit",
+              "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
         if hasattr(L, n):
             names.append(n)
     return names
@@ -164,3 +185,43 @@ class Engine:
         self._check(lib().qw_kernel_times(self.h, buf, 4096, _p(avg), cnt, 64, ctypes.byref(nk)))
         names = buf.value.decode().split(",") if nk.value else []
         return {n: (avg[i], cnt[i]) for i, n in enumerate(names)}
+
+
+class Server:
+    """The serving loop (qw_server_*): jobs run concurrently on one device, admitted by HBM
+    budget in arrival order."""
+
+    def __init__(self, device=0, hbm_budget=0, max_active=0):
+        h = ctypes.c_void_p()
+        o = ServerOpts(device, hbm_budget, max_active)
+        st = lib().qw_server_create(ctypes.byref(o), ctypes.byref(h))
+        if st:
+            raise QwError(st, "server create failed")
+        self.h = h
+        self.nblock = {}
+
+    def submit(self, deck, m, nwalkers, precision="fp64", seed=1, equil_steps=0):
+        j = JobDesc(PREC[precision], seed, nwalkers, equil_steps, m)
+        jid = ctypes.c_longlong()
+        st = lib().qw_server_submit(self.h, deck.encode(), ctypes.byref(j), ctypes.byref(jid))
+        if st:
+            raise QwError(st, "submit failed")
+        self.nblock[jid.value] = m.nblock
+        return jid.value
+
+    def wait(self, jid):
+        """Block records of the job and its stats: submit, admitted, finished, latency (s),
+        electron moves, run seconds, electron-moves/s, device bytes."""
+        eb = np.zeros((self.nblock[jid], BLOCK_FIELDS))
+        stats = np.zeros(JOB_STATS)
+        st = lib().qw_server_wait(self.h, jid, _p(eb), _p(stats))
+        if st:
+            raise QwError(st, lib().qw_server_error(self.h, jid).decode())
+        return eb, stats
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qw_server_destroy(self.h)
+            self.h = None
+
+    __del__ = close

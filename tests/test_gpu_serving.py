@@ -1,0 +1,76 @@
+"""Serving loop (qw_server_*): concurrent jobs of different sizes on one device, admission by
+HBM budget, and each job's result the same alone and in a mix."""
+import numpy as np
+import pytest
+
+from mainline_amd import engine as E
+from tests._util import synth_deck
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs():
+    h4, _ = synth_deck("h4")
+    pp16, _ = synth_deck("pp16")
+    ae32, _ = synth_deck("ae32")
+    return [
+        (h4, E.method("VMC", 256, nblock=3, nstep=5, timestep=0.5), 256, "fp64", 11),
+        (pp16, E.method("DMC", 128, nblock=2, nstep=5, timestep=0.01), 128, "mixed_fp32", 12),
+        (ae32, E.method("DMC", 64, nblock=2, nstep=3, timestep=0.01), 64, "fp64", 13),
+        (h4, E.method("DMC", 512, nblock=2, nstep=10, timestep=0.02), 512, "mixed_fp32", 14),
+        (pp16, E.method("VMC", 96, nblock=2, nstep=4, timestep=0.3), 96, "fp64", 15),
+        (ae32, E.method("VMC", 32, nblock=2, nstep=2, timestep=0.3), 32, "mixed_bf16x6", 16),
+    ]
+
+
+def test_each_job_is_the_same_alone_and_in_the_mix():
+    jobs = _jobs()
+    alone = []
+    for deck, m, nw, prec, seed in jobs:
+        s = E.Server(max_active=1)
+        jid = s.submit(deck, m, nw, precision=prec, seed=seed, equil_steps=3)
+        eb, st = s.wait(jid)
+        s.close()
+        alone.append(eb)
+        print("alone", prec, nw, eb[:, 0], "latency %.3f s" % st[3], flush=True)
+    s = E.Server(max_active=6)
+    ids = [s.submit(deck, m, nw, precision=prec, seed=seed, equil_steps=3) for deck, m, nw, prec, seed in jobs]
+    res = [s.wait(j) for j in ids]
+    s.close()
+    admitted = [st[1] for _, st in res]
+    finished = [st[2] for _, st in res]
+    # several jobs were on the device at once
+    overlap = sum(1 for i in range(len(res)) for j in range(len(res))
+                  if i < j and admitted[j] < finished[i] and admitted[i] < finished[j])
+    print("overlapping job pairs", overlap, flush=True)
+    assert overlap >= 1
+    for a, (eb, st) in zip(alone, res):
+        assert np.array_equal(a, eb)
+        assert st[4] > 0 and st[6] > 0
+
+
+def test_admission_respects_the_hbm_budget():
+    deck, _ = synth_deck("pp16")
+    m = E.method("VMC", 256, nblock=1, nstep=3, timestep=0.3)
+    probe = E.Engine(deck, "fp64")
+    b = ctypes_bytes(probe, 256)
+    probe.close()
+    s = E.Server(hbm_budget=int(b * 1.5), max_active=8)  # room for one job at a time
+    ids = [s.submit(deck, m, 256, seed=20 + i) for i in range(3)]
+    res = [s.wait(j)[1] for j in ids]
+    s.close()
+    for prev, nxt in zip(res, res[1:]):
+        assert nxt[1] >= prev[2] - 1e-6  # admitted only after the previous one finished
+    big = E.Server(hbm_budget=int(b // 2))
+    jid = big.submit(deck, m, 256)
+    with pytest.raises(E.QwError) as ex:
+        big.wait(jid)
+    big.close()
+    assert ex.value.status == -5
+
+
+def ctypes_bytes(eng, nw):
+    import ctypes
+    v = ctypes.c_ulonglong()
+    assert E.lib().qw_engine_walker_bytes(eng.h, nw, ctypes.byref(v)) == 0
+    return v.value
