@@ -128,6 +128,26 @@ def serving_leg(precision, njobs, rate, seed, tmp):
     return res
 
 
+def copy_probe(nbytes):
+    """Three device-to-device copies of nbytes through the HIP runtime (its copy kernel):
+    the reference reading for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log("copied %d bytes x3" % nbytes)
+
+
 def cpu_baseline(deck, mol, seconds_target):
     """The oracle's FP64 CPU engine (same algorithm, threads over walkers) on a bounded sample."""
     from oracle import oracle as O
@@ -159,7 +179,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-blocks", type=int, default=8, help="0 disables the energy-agreement leg")
     ap.add_argument("--serving-jobs", type=int, default=100, help="0 disables the serving leg")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only copy this many bytes device-to-device (counter calibration run) and exit")
     a = ap.parse_args()
+    if a.copy_probe > 0:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the walker engine runs on a single MI355X in this build; "
               "multi-GPU sharding is described in DESIGN.md and not run here.", file=sys.stderr)

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_sweep_big(DevSys S, DevWalkers W, BigPt
     for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
       const double tt = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
       double g[3], dr[3], rn[3], dn[3];
-      gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      gauss3(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, tt, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
       // basis (wavefront 0) and contraction (thread = MO)
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_sweep_big(DevSys S, DevWalkers W, BigPt
         else acc_p = fmin(1.0, rr * rr * exp(pt) * num / den);
       }
       const double uu =
-          uniform(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
+          uniform(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
       if (acc_p + uu > 1.0) {
         accepted = true;
         Rt = t0;

@@ -523,73 +523,82 @@ __device__ inline void dist_inv(float d2, float& r, float& ri) {
 
 // CUTOFF_CUSP (src/basis_function/Cutoff_cusp.cpp:122-154) and POLYPADE
 // (src/basis_function/Poly_pade_function.cpp:183-237), contracted with the like-spin /
-// one-body coefficients (cl) or the unlike-spin ones (cu), chosen per lane.  The object is
-// read at a wave-uniform address (scalar loads).  The POLYPADE sum is regrouped as
+// one-body coefficients (cl) or the unlike-spin ones (cu), chosen per lane by uf = 0 / 1
+// (c = uf*cu + (1-uf)*cl, exact).  The POLYPADE sum is regrouped as
 //   u = zpp1 S0,  gfac = -zppd/(r rcut) S1,  d2 = -(zppdd2 S1 - zppd^2 S2)/rcut^2,
 //   lap = 2 (d2 + gfac),  S0 = sum c_i p_i, S1 = sum c_i (b_i+1) p_i^2, S2 = sum c_i (b_i+1) b_i p_i^3,
 // p_i = 1/(1 + b_i zpp): the reference's per-function terms summed in another order.
-// Accumulates value u, gradient factor gfac (gradient = gfac * (x,y,z)) and, if LAP, the
-// laplacian.
+// The evaluators are branch-free (the cutoff masks the result) and specialised on the
+// object's function count, so every parameter load is unconditional and loop-invariant in
+// the pair loops that call them: the compiler keeps the parameters in scalar registers.
 template <typename R, bool LAP>
-__device__ inline void jrad_lap(const QW_CONST JRad<R>& o, bool unlike, R r, R ri, R& u, R& gfac, R& lap) {
-  if (r >= o.rcut) return;
-  const R zz = r * o.rcinv, zz2 = zz * zz;
-  if (o.kind == 0) {
+struct CuspEval {
+  const QW_CONST JRad<R>& o;
+  __device__ inline void operator()(R uf, R r, R ri, R& u, R& gfac, R& lap) const {
+    const R zz = r * o.rcinv, zz2 = zz * zz;
     const R pp = zz - zz2 + zz * zz2 * (R)(1.0 / 3.0);
     const R pade = rcp_nr((R)1 + o.gamma * pp), pade2 = pade * pade;
     const R ppd = (R)1 - (R)2 * zz + zz2;
     const R dadr = ppd * pade2 * ri;
-    const R c = unlike ? o.cu[0] : o.cl[0];
-    u += c * (o.rcut * (pp * pade - o.pade0));
-    gfac += c * dadr;
+    const R c = fma(uf, o.cu[0], ((R)1 - uf) * o.cl[0]);
+    const bool in = r < o.rcut;
+    u += in ? c * (o.rcut * (pp * pade - o.pade0)) : (R)0;
+    gfac += in ? c * dadr : (R)0;
     if (LAP) {
       const R ppdd = (R)-2 + (R)2 * zz;
       const R dadr2 = ppdd * pade2 * o.rcinv - (R)2 * o.gamma * ppd * ppd * pade2 * pade * o.rcinv + (R)2 * dadr;
-      lap += c * dadr2;
+      lap += in ? c * dadr2 : (R)0;
     }
-    return;
   }
-  const R zpp = zz2 * ((R)6 - (R)8 * zz + (R)3 * zz2), zpp1 = (R)1 - zpp;
-  const R zppd = (R)12 * zz * ((R)1 - (R)2 * zz + zz2);
-  R s0 = 0, s1 = 0, s2 = 0;
+};
+template <typename R, bool LAP, int NF>
+struct PolyEval {
+  const QW_CONST JRad<R>& o;
+  __device__ inline void operator()(R uf, R r, R ri, R& u, R& gfac, R& lap) const {
+    const R zz = r * o.rcinv, zz2 = zz * zz;
+    const R zpp = zz2 * ((R)6 - (R)8 * zz + (R)3 * zz2), zpp1 = (R)1 - zpp;
+    const R zppd = (R)12 * zz * ((R)1 - (R)2 * zz + zz2);
+    const R omu = (R)1 - uf;
+    R s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
-  for (int i = 0; i < kMaxJF; i++) {
-    if (i >= o.nf || !o.nz[i]) continue;
-    const R b = o.beta[i];
-    const R pz = rcp_nr((R)1 + b * zpp);
-    const R c = unlike ? o.cu[i] : o.cl[i], cb = unlike ? o.cbu[i] : o.cbl[i];
-    s0 += c * pz;
-    const R t = cb * pz * pz;
-    s1 += t;
-    if (LAP) s2 += t * b * pz;
+    for (int i = 0; i < NF; i++) {
+      const R b = o.beta[i];
+      const R pz = rcp_nr((R)1 + b * zpp);
+      const R c = fma(uf, o.cu[i], omu * o.cl[i]), cb = fma(uf, o.cbu[i], omu * o.cbl[i]);
+      s0 += c * pz;
+      const R t = cb * pz * pz;
+      s1 += t;
+      if (LAP) s2 += t * b * pz;
+    }
+    const bool in = r < o.rcut;
+    const R g = -zppd * ri * o.rcinv * s1;
+    u += in ? zpp1 * s0 : (R)0;
+    gfac += in ? g : (R)0;
+    if (LAP) {
+      const R zppdd2 = (R)6 - (R)24 * zz + (R)18 * zz2;
+      const R d2 = -o.rcinv2 * (zppdd2 * s1 - zppd * zppd * s2);
+      lap += in ? (R)2 * (d2 + g) : (R)0;
+    }
   }
-  u += zpp1 * s0;
-  const R g = -zppd * ri * o.rcinv * s1;
-  gfac += g;
-  if (LAP) {
-    const R zppdd2 = (R)6 - (R)24 * zz + (R)18 * zz2;
-    const R d2 = -o.rcinv2 * (zppdd2 * s1 - zppd * zppd * s2);
-    lap += (R)2 * (d2 + g);
-  }
-}
-template <typename R>
-__device__ inline void jrad_val(const QW_CONST JRad<R>& o, bool unlike, R r, R& u) {
-  if (r >= o.rcut) return;
-  const R zz = r * o.rcinv, zz2 = zz * zz;
+};
+
+// Runs f(evaluator) with the evaluator specialised for object o (kind and function count).
+template <typename R, bool LAP, typename F>
+__device__ inline void with_jrad(const QW_CONST JRad<R>& o, F&& f) {
   if (o.kind == 0) {
-    const R pp = zz - zz2 + zz * zz2 * (R)(1.0 / 3.0);
-    const R pade = rcp_nr((R)1 + o.gamma * pp);
-    u += (unlike ? o.cu[0] : o.cl[0]) * (o.rcut * (pp * pade - o.pade0));
+    f(CuspEval<R, LAP>{o});
     return;
   }
-  const R zpp = zz2 * ((R)6 - (R)8 * zz + (R)3 * zz2), zpp1 = (R)1 - zpp;
-  R s0 = 0;
-#pragma unroll
-  for (int i = 0; i < kMaxJF; i++) {
-    if (i >= o.nf || !o.nz[i]) continue;
-    s0 += (unlike ? o.cu[i] : o.cl[i]) * rcp_nr((R)1 + o.beta[i] * zpp);
+  switch (o.nf) {
+    case 1: f(PolyEval<R, LAP, 1>{o}); break;
+    case 2: f(PolyEval<R, LAP, 2>{o}); break;
+    case 3: f(PolyEval<R, LAP, 3>{o}); break;
+    case 4: f(PolyEval<R, LAP, 4>{o}); break;
+    case 5: f(PolyEval<R, LAP, 5>{o}); break;
+    case 6: f(PolyEval<R, LAP, 6>{o}); break;
+    case 7: f(PolyEval<R, LAP, 7>{o}); break;
+    default: f(PolyEval<R, LAP, kMaxJF>{o}); break;
   }
-  u += zpp1 * s0;
 }
 
 // Jastrow terms involving electron e placed at p (others at pos): value, gradient and,
@@ -607,35 +616,37 @@ __device__ inline R jastrow_e_t(const DevSys& S, const JRad<R>* ee, const JRad<R
   for (int t = 0; t < S.n_eit; t++) {
     const int a0 = ea0[t], a1 = ea0[t + 1], o1 = eo0[t + 1];
     for (int o = eo0[t]; o < o1; o++) {
-      const QW_CONST JRad<R>& ob = cptr(eit)[o];
-      for (int k = a0 + lane; k < a1; k += kWave) {
-        const double* ap = epos + estride * k;
-        const R x = (R)(p[0] - ap[0]), y = (R)(p[1] - ap[1]), z = (R)(p[2] - ap[2]);
-        R r, ri;
-        dist_inv(x * x + y * y + z * z, r, ri);
-        R gf = 0;
-        jrad_lap<R, LAP>(ob, false, r, ri, u, gf, l);
-        gx += gf * x;
-        gy += gf * y;
-        gz += gf * z;
-      }
+      with_jrad<R, LAP>(cptr(eit)[o], [&](const auto& ev) {
+        for (int k = a0 + lane; k < a1; k += kWave) {
+          const double* ap = epos + estride * k;
+          const R x = (R)(p[0] - ap[0]), y = (R)(p[1] - ap[1]), z = (R)(p[2] - ap[2]);
+          R r, ri;
+          dist_inv(x * x + y * y + z * z, r, ri);
+          R gf = 0;
+          ev((R)0, r, ri, u, gf, l);
+          gx += gf * x;
+          gy += gf * y;
+          gz += gf * z;
+        }
+      });
     }
   }
   const bool eup = e < S.nup;
   for (int o = 0; o < S.nee; o++) {
-    const QW_CONST JRad<R>& ob = cptr(ee)[o];
-    for (int j = lane; j < S.ne; j += kWave) {
-      if (j == e) continue;
-      const R x = (R)(p[0] - pos[3 * j]), y = (R)(p[1] - pos[3 * j + 1]), z = (R)(p[2] - pos[3 * j + 2]);
-      R r, ri;
-      dist_inv(x * x + y * y + z * z, r, ri);
-      const bool unlike = (j < S.nup) != eup;
-      R gf = 0;
-      jrad_lap<R, LAP>(ob, unlike, r, ri, u, gf, l);
-      gx += gf * x;
-      gy += gf * y;
-      gz += gf * z;
-    }
+    with_jrad<R, LAP>(cptr(ee)[o], [&](const auto& ev) {
+      for (int j = lane; j < S.ne; j += kWave) {
+        if (j == e) continue;
+        const R x = (R)(p[0] - pos[3 * j]), y = (R)(p[1] - pos[3 * j + 1]), z = (R)(p[2] - pos[3 * j + 2]);
+        R r, ri;
+        dist_inv(x * x + y * y + z * z, r, ri);
+        const R uf = ((j < S.nup) != eup) ? (R)1 : (R)0;
+        R gf = 0;
+        ev(uf, r, ri, u, gf, l);
+        gx += gf * x;
+        gy += gf * y;
+        gz += gf * z;
+      }
+    });
   }
   if constexpr (sizeof(R) == sizeof(double)) {
     u = wsum(u);
@@ -664,34 +675,8 @@ __device__ inline double jastrow_e(const DevSys& S, int e, const double p[3], co
 
 // Value-only Jastrow terms of electron e at p (pseudopotential quadrature).
 __device__ inline double jastrow_e_val(const DevSys& S, int e, const double p[3], const double* pos, int lane) {
-  double u = 0;
-  const QW_CONST int* ea0 = cptr(S.eit_a0);
-  const QW_CONST int* eo0 = cptr(S.eit_o0);
-  for (int t = 0; t < S.n_eit; t++) {
-    const int a0 = ea0[t], a1 = ea0[t + 1], o1 = eo0[t + 1];
-    for (int o = eo0[t]; o < o1; o++) {
-      const QW_CONST JRad<double>& ob = cptr(S.eit)[o];
-      for (int k = a0 + lane; k < a1; k += kWave) {
-        const double* ap = S.eit_pos + 3 * k;
-        double x = p[0] - ap[0], y = p[1] - ap[1], z = p[2] - ap[2];
-        double r, ri;
-        dist_inv(x * x + y * y + z * z, r, ri);
-        jrad_val<double>(ob, false, r, u);
-      }
-    }
-  }
-  const bool eup = e < S.nup;
-  for (int o = 0; o < S.nee; o++) {
-    const QW_CONST JRad<double>& ob = cptr(S.ee)[o];
-    for (int j = lane; j < S.ne; j += kWave) {
-      if (j == e) continue;
-      double x = p[0] - pos[3 * j], y = p[1] - pos[3 * j + 1], z = p[2] - pos[3 * j + 2];
-      double r, ri;
-      dist_inv(x * x + y * y + z * z, r, ri);
-      jrad_val<double>(ob, (j < S.nup) != eup, r, u);
-    }
-  }
-  return wsum(u);
+  double g[3];
+  return jastrow_e_t<double, false>(S, S.ee, S.eit, e, p, pos, g, nullptr, lane, S.eit_pos, 3);
 }
 
 // Semilocal pseudopotential channel values v_l(r) (src/basis_function/Rgaussian_function.cpp:180-202,

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -99,6 +100,12 @@ struct qw_engine {
   MixedTables MT{};
   // per-kernel timing with events on the engine stream
   bool timing = true;
+  bool capturing = false;  // launches are being recorded into a graph (no timing events)
+  int graph_mode = -1;     // QW_GRAPHS: 0 never, 1 always, otherwise for small step work
+  unsigned* dstep = nullptr;  // device step counter read by the kernels
+  hipGraphExec_t gx = nullptr;
+  int g_np = 0, g_nw = 0;
+  double g_tau = 0;
   struct Ev {
     int kind;
     hipEvent_t a, b;
@@ -121,6 +128,7 @@ struct qw_engine {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     if (rb) (void)rocblas_destroy_handle(rb);
+    if (gx) (void)hipGraphExecDestroy(gx);
     for (void* p : sys_allocs) (void)hipFree(p);
     for (void* p : walker_allocs) (void)hipFree(p);
     for (auto& e : evs) pool.push_back(e.a), pool.push_back(e.b);
@@ -167,7 +175,7 @@ struct qw_engine {
   // Launch wrapper: records start/stop events around the launch on the engine stream.
   template <typename F>
   void launch(const char* name, F f) {
-    if (!timing) {
+    if (!timing || capturing) {
       f();
       HIPCHK(hipGetLastError());
       return;
@@ -300,6 +308,7 @@ struct qw_engine {
     S.pp = upload(T.pptypes);
     S.atom_pp = upload(T.atom_pp);
     dctl = upload(std::vector<DmcCtl>(1));
+    dstep = upload(std::vector<unsigned>(1));
     dred = upload(std::vector<double>(kBsum + 1));
   }
 
@@ -408,7 +417,7 @@ struct qw_engine {
     }
   }
   void run_sweep(int s, int k0, int k1, double tau, unsigned sweep, int depth, int restrict_nodes, int dets) {
-    SweepArgs A{s, k0, k1, tau, seed, step, sweep, restrict_nodes, dets};
+    SweepArgs A{s, k0, k1, tau, seed, step, sweep, restrict_nodes, dets, dstep};
     if (big) {
       big_sweep(A, depth);
       return;
@@ -526,6 +535,69 @@ struct qw_engine {
     }
   }
 
+  void set_device_step() {
+    const unsigned v = step;
+    HIPCHK(hipMemcpyAsync(dstep, &v, sizeof(unsigned), hipMemcpyHostToDevice, stream));
+  }
+  // One DMC feedback interval on the stream: np steps (both spin sweeps, local energy and
+  // weights, step counter) then the trial-energy update and branching.  The kernels read the
+  // step from the device counter, so the same launches can be recorded once and replayed.
+  void dmc_interval(int np, double tau, LocalArgs A) {
+    const int ne = S.ne;
+    for (int p = 0; p < np; p++) {
+      for (int s = 0; s < 2; s++) {
+        const int n = s ? T.ndn : T.nup;
+        if (n == 0) continue;
+        run_sweep(s, 0, n, tau, 0u, 1, 1, 1);
+      }
+      A.mode = 2;
+      A.use_dgl = 1;
+      A.stepp = dstep;
+      run_local(A);
+      launch("k_step", [&] { hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, stream, dstep); });
+    }
+    launch("k_branch", [&] {
+      hipLaunchKernelGGL(k_branch, dim3(1), dim3(1024), 0, stream, W, dctl, keys, vals, psrc, pdst, nw * ne * np, 1,
+                         seed, 0u, (const unsigned*)dstep);
+    });
+    launch("k_copy", [&] {
+      hipLaunchKernelGGL(k_copy, dim3(nw / 2 + 1), dim3(256), 0, stream, S, W, (const DmcCtl*)dctl,
+                         (const int*)psrc, (const int*)pdst);
+    });
+  }
+  bool use_graphs() const {
+    if (big || graph_mode == 0) return false;
+    return graph_mode == 1 || (long long)nw * S.ne <= (1ll << 17);
+  }
+  // dmc_interval through a recorded graph (recorded again when np, tau or nw change)
+  void dmc_interval_graph(int np, double tau, const LocalArgs& A) {
+    if (!gx || g_np != np || g_tau != tau || g_nw != nw) {
+      if (gx) {
+        HIPCHK(hipGraphExecDestroy(gx));
+        gx = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      capturing = true;
+      try {
+        dmc_interval(np, tau, A);
+      } catch (...) {
+        capturing = false;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      capturing = false;
+      HIPCHK(hipStreamEndCapture(stream, &g));
+      HIPCHK(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+      g_np = np;
+      g_tau = tau;
+      g_nw = nw;
+    }
+    HIPCHK(hipGraphLaunch(gx, stream));
+  }
+
   void reduce(double* host, int reset) {
     launch("k_reduce", [&] { hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, stream, W, dred, reset); });
     HIPCHK(hipMemcpyAsync(host, dred, (kBsum + 1) * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -609,6 +681,7 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
     g_create_error = x.what();
     return QW_ERR_DEVICE;
   }
+  if (const char* g = std::getenv("QW_GRAPHS")) e->graph_mode = (g[0] == '0') ? 0 : (g[0] == '1') ? 1 : -1;
   *out = e.release();
   return QW_OK;
 }
@@ -866,35 +939,29 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
       if (!dmc) e->refresh_all();
       for (int st = 0; st < m->nstep;) {
         const int np = dmc ? std::min(fbi, m->nstep - st) : 1;
-        for (int p = 0; p < np; p++) {
-          const int nsw = dmc ? 1 : std::max(1, m->ndecorr);
+        e->set_device_step();
+        if (dmc) {
+          if (e->use_graphs()) e->dmc_interval_graph(np, tau, A);
+          else e->dmc_interval(np, tau, A);
+          moves += (long long)nw * ne * np;
+          e->step += np;
+        } else {
+          const int nsw = std::max(1, m->ndecorr);
           for (int sw = 0; sw < nsw; sw++) {
             for (int s = 0; s < 2; s++) {
               int n = s ? e->T.ndn : e->T.nup;
               if (n == 0) continue;
-              e->run_sweep(s, 0, n, tau, (unsigned)sw, dmc ? 1 : 2, dmc ? 1 : 0, sw == nsw - 1);
+              e->run_sweep(s, 0, n, tau, (unsigned)sw, 2, 0, sw == nsw - 1);
             }
           }
           moves += (long long)nw * ne * nsw;
-          A.mode = dmc ? 2 : 1;
+          A.mode = 1;
           A.use_dgl = 1;
-          A.step = e->step;
+          A.stepp = e->dstep;
           e->run_local(A);
           e->step++;
         }
         st += np;
-        if (dmc) {
-          e->launch("k_branch", [&] {
-            hipLaunchKernelGGL(k_branch, dim3(1), dim3(1024), 0, e->stream, e->W, e->dctl, e->keys, e->vals, e->psrc,
-                               e->pdst, nw * ne * np, 1, e->seed, e->step);
-          });
-          e->launch("k_copy", [&] {
-            hipLaunchKernelGGL(k_copy, dim3(nw / 2 + 1), dim3(256), 0, e->stream, e->S, e->W, (const DmcCtl*)e->dctl,
-                               (const int*)e->psrc, (const int*)e->pdst);
-          });
-        } else {
-          // acceptance bookkeeping for VMC
-        }
       }
       e->reduce(red, 1);
       double sw = red[0], np_ = red[5];

@@ -244,7 +244,14 @@ struct SweepArgs {
   unsigned step, sweep;
   int restrict_nodes;
   int dets;  // write the determinant gradient / laplacian ratios of spin s to W.dgl at the end
+  const unsigned* stepp;  // device step counter (replaces `step` when set; graph replays)
 };
+__device__ inline unsigned step_of(unsigned step, const unsigned* stepp) { return stepp ? *stepp : step; }
+
+// Advances the device step counter (one thread).
+__global__ void k_step_advance(unsigned* stepp) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *stepp += 1;
+}
 
 // Determinant parts of the local energy for lane k's electron of spin s from the inverse row
 // in registers: dgl[c] = sum_j M_c[e_k][j] inv(k, j), c = d/dx, d/dy, d/dz, laplacian
@@ -334,7 +341,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
     for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
       const double t = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
       double g[3], dr[3], rn[3], dn[3];
-      gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      gauss3(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
       mo_lap_point<double, double, 4>(S, V, S.spline, S.C[s], n, rn, chi, roff, kCapF64, phi, lane);
@@ -367,7 +374,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
         else if (fabs(den) < 1e-10) acc = fabs(num) < 1e-10 ? 0.0 : 1.0;
         else acc = fmin(1.0, rr * rr * exp(pt) * num / den);
       }
-      double u = uniform(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
+      double u = uniform(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
       if (acc + u > 1.0) {
         accepted = true;
         R = t0;
@@ -379,9 +386,11 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
       if (lane < NS) phib[lane] = lane < n ? phi[0] : 0.0;
       __syncthreads();
       if (lane < n) {
-        double tk = 0;
+        // four independent partial sums: the dot product is not one serial FMA chain
+        double tq[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int j = 0; j < NS; j++) tk += inv[j] * phib[j];
+        for (int j = 0; j < NS; j++) tq[j & 3] += inv[j] * phib[j];
+        const double tk = (tq[0] + tq[1]) + (tq[2] + tq[3]);
         if (lane == k) {
           const double f = 1.0 / R;
 #pragma unroll
@@ -425,6 +434,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
 struct LocalArgs {
   int mode;              // 0 = evaluate (write outputs), 1 = VMC sums, 2 = DMC weight + sums, 3 = init (elast only)
   int use_dgl;           // determinant parts from W.dgl (written by the last sweep) instead of the inverse
+  const unsigned* stepp; // device step counter (replaces `step` when set)
   int pp_mode;           // 0 deterministic, 1 stochastic
   const double* rot;     // [9] fixed rotation (mode 0) or null
   const double* pp_rand; // [nw][na][ne] uniforms (mode 0, stochastic) or null
@@ -466,36 +476,38 @@ __device__ inline void jastrow_local(const DevSys& S, const JRad<R>* ee, const J
   for (int t = 0; t < S.n_eit; t++) {
     const int a0 = ea0[t], a1 = ea0[t + 1], o1 = eo0[t + 1];
     for (int o = eo0[t]; o < o1; o++) {
-      const QW_CONST JRad<R>& ob = cptr(eit)[o];
-      for (int k = a0; k < a1; k++) {
-        const QW_CONST double* ap = epos + 3 * k;
-        const R x = (R)(pi[0] - ap[0]), y = (R)(pi[1] - ap[1]), z = (R)(pi[2] - ap[2]);
-        R r, ri;
-        dist_inv(x * x + y * y + z * z, r, ri);
-        R gf = 0;
-        jrad_lap<R, true>(ob, false, r, ri, u, gf, l);
-        gx += gf * x;
-        gy += gf * y;
-        gz += gf * z;
-      }
+      with_jrad<R, true>(cptr(eit)[o], [&](const auto& ev) {
+        for (int k = a0; k < a1; k++) {
+          const QW_CONST double* ap = epos + 3 * k;
+          const R x = (R)(pi[0] - ap[0]), y = (R)(pi[1] - ap[1]), z = (R)(pi[2] - ap[2]);
+          R r, ri;
+          dist_inv(x * x + y * y + z * z, r, ri);
+          R gf = 0;
+          ev((R)0, r, ri, u, gf, l);
+          gx += gf * x;
+          gy += gf * y;
+          gz += gf * z;
+        }
+      });
     }
   }
   const bool iup = i < S.nup;
   for (int o = 0; o < S.nee; o++) {
-    const QW_CONST JRad<R>& ob = cptr(ee)[o];
-    for (int j = 0; j < S.ne; j++) {
-      if (j == i) continue;
-      const R x = (R)(pi[0] - pos[3 * j]), y = (R)(pi[1] - pos[3 * j + 1]), z = (R)(pi[2] - pos[3 * j + 2]);
-      R r, ri;
-      dist_inv(x * x + y * y + z * z, r, ri);
-      const bool unlike = (j < S.nup) != iup;
-      R gf = 0, upair = 0;
-      jrad_lap<R, true>(ob, unlike, r, ri, upair, gf, l);
-      if (j > i) u += upair;
-      gx += gf * x;
-      gy += gf * y;
-      gz += gf * z;
-    }
+    with_jrad<R, true>(cptr(ee)[o], [&](const auto& ev) {
+      for (int j = 0; j < S.ne; j++) {
+        if (j == i) continue;
+        const R x = (R)(pi[0] - pos[3 * j]), y = (R)(pi[1] - pos[3 * j + 1]), z = (R)(pi[2] - pos[3 * j + 2]);
+        R r, ri;
+        dist_inv(x * x + y * y + z * z, r, ri);
+        const R uf = ((j < S.nup) != iup) ? (R)1 : (R)0;
+        R gf = 0, upair = 0;
+        ev(uf, r, ri, upair, gf, l);
+        if (j > i) u += upair;
+        gx += gf * x;
+        gy += gf * y;
+        gz += gf * z;
+      }
+    });
   }
 }
 
@@ -598,7 +610,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
       if (A.rot)
         for (int i = 0; i < 9; i++) rot[i] = A.rot[i];
     } else {
-      random_rotation(A.seed, (unsigned)w, A.step, rot);
+      random_rotation(A.seed, (unsigned)w, step_of(A.step, A.stepp), rot);
     }
     for (int a = 0; a < S.na; a++) {
       const int pk = cptr(S.atom_pp)[a];
@@ -620,7 +632,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
             for (int l = 0; l < P.L - 1; l++) strength += 10.0 * (2 * l + 1) * fabs(vl[l]);
             strength = fmin(1.0, strength);
             double rr = A.mode == 0 ? A.pp_rand[((size_t)w * S.na + a) * ne + e]
-                                    : uniform(A.seed, (unsigned)w, A.step, (unsigned)(a * ne + e), TAG_PP);
+                                    : uniform(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)(a * ne + e), TAG_PP);
             want = strength > rr;
           }
           vnl += vl[P.L - 1];  // local channel for every electron
@@ -727,7 +739,9 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
 // w_big / (w_big + w_small), both slots get the mean weight.
 __global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsigned long long* keys, int* vals,
                                                  int* pair_src, int* pair_dst, int nmoves, int do_branch,
-                                                 unsigned long long seed, unsigned event) {
+                                                 unsigned long long seed, unsigned event_host,
+                                                 const unsigned* eventp) {
+  const unsigned event = eventp ? *eventp : event_host;
   __shared__ double red[1024];
   __shared__ long long redi[1024];
   const int t = threadIdx.x, nw = W.nw;

@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
     for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
       const double t = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
       double g[3], dr[3], rn[3], dn[3];
-      gauss3(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      gauss3(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
       if constexpr (MODE == LCAO_FP32)
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
         else if (fabs(den) < 1e-10) acc = fabs(num) < 1e-10 ? 0.0 : 1.0;
         else acc = fmin(1.0, rr * rr * exp(pt) * num / den);
       }
-      double u = uniform(A.seed, (unsigned)w, A.step, (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
+      double u = uniform(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
       if (acc + u > 1.0) {
         accepted = true;
         R = t0;
@@ -341,9 +341,11 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
       if (lane < NS) phib[lane] = lane < n ? phi[0] : 0.f;
       __syncthreads();
       if (lane < n) {
-        float tk = 0;
+        // four independent partial sums: the dot product is not one serial FMA chain
+        float tq[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int j = 0; j < NS; j++) tk += inv[j] * phib[j];
+        for (int j = 0; j < NS; j++) tq[j & 3] += inv[j] * phib[j];
+        const float tk = (tq[0] + tq[1]) + (tq[2] + tq[3]);
         if (lane == k) {
           const float f = 1.f / R;
 #pragma unroll

@@ -111,3 +111,24 @@ def test_h_atom_vmc_and_dmc_known_answers():
     err = e.std(ddof=1) / math.sqrt(len(e))
     _say("H DMC", e.mean(), err)
     assert abs(e.mean() - (-0.5)) < 3 * err + 2e-4
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_fp32"])
+def test_graph_replayed_dmc_equals_direct_launches(prec, monkeypatch):
+    """DMC feedback intervals recorded once into a HIP graph and replayed (several replays per
+    block, host synchronisation between blocks) give exactly the directly launched result."""
+    deck, mol = synth_deck("pp16")
+    nw = 64
+    pos = synth.random_configs(mol, nw, seed=23)
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("QW_GRAPHS", mode)
+        eng = E.Engine(deck, prec, seed=5151)
+        eng.set_walkers(pos)
+        res = eng.run(E.method("DMC", nw, nblock=2, nstep=15, timestep=0.01))
+        p, w = eng.get_walkers(nw)
+        out[mode] = (res["eblock"], p, w)
+        eng.close()
+    _say("graph vs direct", out["0"][0][:, 0], out["1"][0][:, 0])
+    for a, b in zip(out["0"], out["1"]):
+        assert np.array_equal(a, b)
