@@ -127,6 +127,12 @@ qw_status qw_kernel_times(const qw_engine* e, char* buf, int buflen, double* avg
  * budget.  No reference counterpart: the reference sizes nothing on a device. */
 qw_status qw_engine_walker_bytes(const qw_engine* e, int nw, unsigned long long* bytes);
 
+/* Launch choices the engine made (no reference counterpart): out4 = LCAO coefficient loads in
+ * flight chosen by the online tuner (0 while it is still measuring), HIP graphs used for DMC
+ * steps (0/1), block-per-walker kernels for more than 64 electrons of a spin (0/1), register
+ * tile width of the inverse. */
+qw_status qw_engine_tuning(const qw_engine* e, int* out4);
+
 /* ---------------------------------------------------------------- serving loop
  * A request loop over the engine: VMC/DMC jobs (decks of different sizes) are submitted at
  * any time, admitted in arrival order while the sum of their device bytes fits the HBM budget

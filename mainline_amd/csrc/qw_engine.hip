@@ -426,26 +426,83 @@ struct qw_engine {
       mixed_sweep(*this, A, depth);
       return;
     }
-    auto go = [&](auto kern) {
-      launch("k_sweep", [&] { hipLaunchKernelGGL(kern, dim3(nw), dim3(64), lds_sweep(), stream, S, W, A); });
-    };
+    const int u = pick_u();
     if (depth == 1) {
-      switch (ns) {
-        case 4: go(k_sweep<4, 1>); break;
-        case 8: go(k_sweep<8, 1>); break;
-        case 16: go(k_sweep<16, 1>); break;
-        case 32: go(k_sweep<32, 1>); break;
-        default: go(k_sweep<64, 1>); break;
-      }
+      if (u == 2) sweep_fp64<1, 2>(A);
+      else if (u == 8) sweep_fp64<1, 8>(A);
+      else sweep_fp64<1, 4>(A);
     } else {
-      switch (ns) {
-        case 4: go(k_sweep<4, 2>); break;
-        case 8: go(k_sweep<8, 2>); break;
-        case 16: go(k_sweep<16, 2>); break;
-        case 32: go(k_sweep<32, 2>); break;
-        default: go(k_sweep<64, 2>); break;
+      if (u == 2) sweep_fp64<2, 2>(A);
+      else if (u == 8) sweep_fp64<2, 8>(A);
+      else sweep_fp64<2, 4>(A);
+    }
+  }
+  template <int DEPTH, int U>
+  void sweep_fp64(const SweepArgs& A) {
+    auto go = [&](auto kern) {
+      tuned_launch([&] { hipLaunchKernelGGL(kern, dim3(nw), dim3(64), lds_sweep(), stream, S, W, A); });
+    };
+    switch (ns) {
+      case 4: go(k_sweep<4, DEPTH, U>); break;
+      case 8: go(k_sweep<8, DEPTH, U>); break;
+      case 16: go(k_sweep<16, DEPTH, U>); break;
+      case 32: go(k_sweep<32, DEPTH, U>); break;
+      default: go(k_sweep<64, DEPTH, U>); break;
+    }
+  }
+
+  // ---- online tuning of the LCAO load depth U (coefficient loads in flight).  The
+  // candidates do the same arithmetic in the same order, so the first sweep launches of an
+  // engine simply run with each candidate in turn (kTuneReps times), timed with events; the
+  // fastest is kept for the engine's lifetime.  QW_TUNE=0 keeps the default.
+  static constexpr int kTuneReps = 2;
+  bool tune_enabled = true;
+  int tune_u = 0;      // chosen depth, 0 while measuring
+  int tune_n = 0;      // candidate launches issued
+  int tune_cur = 0;    // candidate of the launch being issued
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> tune_ev;
+  int u_default() const { return precision == QW_PREC_FP64 ? 4 : 8; }
+  int u_cand(int i) const { return precision == QW_PREC_FP64 ? (2 << i) : (4 << i); }
+  int pick_u() {
+    if (tune_u) return tune_u;
+    if (!tune_enabled || capturing || big || (precision != QW_PREC_FP64 && precision != QW_PREC_MIXED_FP32))
+      return u_default();
+    tune_cur = u_cand(tune_n % 3);
+    return tune_cur;
+  }
+  // Launches a sweep; while U is being tuned the launch is bracketed by events.
+  template <typename F>
+  void tuned_launch(F f) {
+    const bool measuring = !tune_u && tune_enabled && !capturing && tune_cur != 0;
+    if (!measuring) {
+      launch("k_sweep", f);
+      return;
+    }
+    hipEvent_t a = ev(), b = ev();
+    HIPCHK(hipEventRecord(a, stream));
+    launch("k_sweep", f);
+    HIPCHK(hipEventRecord(b, stream));
+    tune_ev.push_back({tune_cur, {a, b}});
+    tune_cur = 0;
+    if (++tune_n < 3 * kTuneReps) return;
+    HIPCHK(hipEventSynchronize(b));
+    double best = 1e300;
+    for (int i = 0; i < 3; i++) {
+      const int u = u_cand(i);
+      double t = 1e300;
+      for (auto& x : tune_ev)
+        if (x.first == u) {
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, x.second.first, x.second.second));
+          t = std::min(t, (double)ms);
+        }
+      if (t < best) {
+        best = t;
+        tune_u = u;
       }
     }
+    for (auto& x : tune_ev) pool.push_back(x.second.first), pool.push_back(x.second.second);
+    tune_ev.clear();
   }
   void run_local(const LocalArgs& A) {
     if (precision != QW_PREC_FP64) {
@@ -682,11 +739,21 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
     return QW_ERR_DEVICE;
   }
   if (const char* g = std::getenv("QW_GRAPHS")) e->graph_mode = (g[0] == '0') ? 0 : (g[0] == '1') ? 1 : -1;
+  if (const char* t = std::getenv("QW_TUNE")) e->tune_enabled = t[0] != '0';
   *out = e.release();
   return QW_OK;
 }
 
 extern "C" void qw_engine_destroy(qw_engine* e) { delete e; }
+
+extern "C" qw_status qw_engine_tuning(const qw_engine* e, int* out4) {
+  if (!e || !out4) return QW_ERR_ARG;
+  out4[0] = e->tune_u;
+  out4[1] = e->use_graphs() ? 1 : 0;
+  out4[2] = e->big ? 1 : 0;
+  out4[3] = e->ns;
+  return QW_OK;
+}
 
 extern "C" qw_status qw_engine_walker_bytes(const qw_engine* e, int nw, unsigned long long* bytes) {
   if (!e || !bytes || nw < 0) return QW_ERR_ARG;

@@ -286,7 +286,9 @@ __host__ __device__ inline size_t sweep_lds_bytes(int ne, int na, int nsh, int n
          (size_t)5 * cap * tsize + (size_t)cap * sizeof(int) + (size_t)2 * ns * tsize + 16;
 }
 
-template <int NS, int DEPTH>
+// U: coefficient loads in flight in the LCAO (chosen by the engine's online tuner; every U
+// gives the same arithmetic in the same order).
+template <int NS, int DEPTH, int U>
 __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepArgs A) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, w = blockIdx.x;
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
       gauss3(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
-      mo_lap_point<double, double, 4>(S, V, S.spline, S.C[s], n, rn, chi, roff, kCapF64, phi, lane);
+      mo_lap_point<double, double, U>(S, V, S.spline, S.C[s], n, rn, chi, roff, kCapF64, phi, lane);
       double t0 = wsum(rk * phi[0]);
       double t1 = wsum(rk * phi[1]), t2 = wsum(rk * phi[2]), t3 = wsum(rk * phi[3]);
       double gj[3];

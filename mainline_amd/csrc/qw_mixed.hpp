@@ -244,7 +244,7 @@ __host__ __device__ inline size_t sweep_mx_lds_bytes(int ne, int na, int nsh, in
          tile + (size_t)2 * ns * 4 + 16;
 }
 
-template <int NS, int DEPTH, int MODE>
+template <int NS, int DEPTH, int MODE, int U>
 __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W, SweepArgs A) {
   extern __shared__ double ldsd[];
   const int lane = threadIdx.x, w = blockIdx.x;
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
       if constexpr (MODE == LCAO_FP32)
-        mo_lap_point<float, float, 8>(S, V, M.spline, M.Cf[s], n, rn, chi, flist, kCapF32, phi, lane);
+        mo_lap_point<float, float, U>(S, V, M.spline, M.Cf[s], n, rn, chi, flist, kCapF32, phi, lane);
       else
         lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, flist, phi, lane);
       float t0 = wsumf(rk * phi[0]);

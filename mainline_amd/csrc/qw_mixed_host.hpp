@@ -86,19 +86,31 @@ void mixed_rows_t(qw_engine& e) {
   });
 }
 
-template <int DEPTH, int MODE>
-void mixed_sweep_t(qw_engine& e, const SweepArgs& A) {
+template <int DEPTH, int MODE, int U>
+void mixed_sweep_u(qw_engine& e, const SweepArgs& A) {
   auto go = [&](auto kern) {
-    e.launch("k_sweep", [&] {
+    e.tuned_launch([&] {
       hipLaunchKernelGGL(kern, dim3(e.nw), dim3(64), lds_mx_sweep(e, MODE), e.stream, e.S, e.MT, e.W, A);
     });
   };
   switch (e.ns) {
-    case 4: go(k_sweep_mx<4, DEPTH, MODE>); break;
-    case 8: go(k_sweep_mx<8, DEPTH, MODE>); break;
-    case 16: go(k_sweep_mx<16, DEPTH, MODE>); break;
-    case 32: go(k_sweep_mx<32, DEPTH, MODE>); break;
-    default: go(k_sweep_mx<64, DEPTH, MODE>); break;
+    case 4: go(k_sweep_mx<4, DEPTH, MODE, U>); break;
+    case 8: go(k_sweep_mx<8, DEPTH, MODE, U>); break;
+    case 16: go(k_sweep_mx<16, DEPTH, MODE, U>); break;
+    case 32: go(k_sweep_mx<32, DEPTH, MODE, U>); break;
+    default: go(k_sweep_mx<64, DEPTH, MODE, U>); break;
+  }
+}
+template <int DEPTH, int MODE>
+void mixed_sweep_t(qw_engine& e, const SweepArgs& A) {
+  if (MODE != LCAO_FP32) {  // the MFMA modes read the dense coefficient fragments (no U)
+    mixed_sweep_u<DEPTH, MODE, 8>(e, A);
+    return;
+  }
+  switch (e.pick_u()) {
+    case 4: mixed_sweep_u<DEPTH, MODE, 4>(e, A); break;
+    case 16: mixed_sweep_u<DEPTH, MODE, 16>(e, A); break;
+    default: mixed_sweep_u<DEPTH, MODE, 8>(e, A); break;
   }
 }
 }  // namespace
