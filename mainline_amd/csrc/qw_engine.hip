@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -26,6 +27,11 @@ using namespace qw;
 namespace {
 
 thread_local std::string g_create_error;
+
+// Device allocation and release, and graph recording, are serialised across the engines of a
+// process: an allocation or free (which may synchronise the device) issued by one host
+// thread while another records a graph would invalidate that recording.
+std::mutex g_devmem_mu;
 
 struct DevErr : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -129,8 +135,11 @@ struct qw_engine {
     if (stream) (void)hipStreamSynchronize(stream);
     if (rb) (void)rocblas_destroy_handle(rb);
     if (gx) (void)hipGraphExecDestroy(gx);
-    for (void* p : sys_allocs) (void)hipFree(p);
-    for (void* p : walker_allocs) (void)hipFree(p);
+    {
+      std::lock_guard<std::mutex> g(g_devmem_mu);
+      for (void* p : sys_allocs) (void)hipFree(p);
+      for (void* p : walker_allocs) (void)hipFree(p);
+    }
     for (auto& e : evs) pool.push_back(e.a), pool.push_back(e.b);
     for (auto ev : pool) (void)hipEventDestroy(ev);
     if (stream) (void)hipStreamDestroy(stream);
@@ -140,17 +149,26 @@ struct qw_engine {
   T* upload(const std::vector<T>& v) {
     void* p = nullptr;
     size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
-    HIPCHK(hipMalloc(&p, n));
+    {
+      std::lock_guard<std::mutex> g(g_devmem_mu);
+      HIPCHK(hipMalloc(&p, n));
+    }
     sys_allocs.push_back(p);
-    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) {
+      HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
     return (T*)p;
   }
   template <typename T>
   T* walloc(size_t n) {
     void* p = nullptr;
-    HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    {
+      std::lock_guard<std::mutex> g(g_devmem_mu);
+      HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    }
     walker_allocs.push_back(p);
-    HIPCHK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPCHK(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), stream));
     return (T*)p;
   }
 
@@ -335,7 +353,10 @@ struct qw_engine {
       return;
     }
     HIPCHK(hipStreamSynchronize(stream));
-    for (void* p : walker_allocs) (void)hipFree(p);
+    {
+      std::lock_guard<std::mutex> g(g_devmem_mu);
+      for (void* p : walker_allocs) (void)hipFree(p);
+    }
     walker_allocs.clear();
     luw = nullptr;  // the solver buffers are walker allocations too
     lupiv = nullptr;
@@ -516,6 +537,7 @@ struct qw_engine {
     const int n = std::max(T.nup, T.ndn);
     if (lucap >= nw && luw) return;
     if (!rb) {
+      std::lock_guard<std::mutex> g(g_devmem_mu);
       if (rocblas_create_handle(&rb) != rocblas_status_success) throw DevErr("rocblas_create_handle failed");
       if (rocblas_set_stream(rb, stream) != rocblas_status_success) throw DevErr("rocblas_set_stream failed");
     }
@@ -531,6 +553,9 @@ struct qw_engine {
     lu_buffers();
     const size_t tot = (size_t)nw * n * n;
     const unsigned g = (unsigned)((tot + 255) / 256);
+    // the solver library may allocate its workspace: serialised with other engines' graph
+    // recording like every device allocation
+    std::lock_guard<std::mutex> lk(g_devmem_mu);
     launch("k_inv", [&] {
       if (precision == QW_PREC_FP64)
         hipLaunchKernelGGL(k_lu_gather<double>, dim3(g), dim3(256), 0, stream, S, (const double*)W.mo, luw, nw, s);
@@ -634,6 +659,7 @@ struct qw_engine {
         gx = nullptr;
       }
       hipGraph_t g = nullptr;
+      std::lock_guard<std::mutex> lk(g_devmem_mu);
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       capturing = true;
       try {
@@ -730,7 +756,7 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
     HIPCHK(hipGetDeviceCount(&ndev));
     if (e->device < 0 || e->device >= ndev) throw DevErr("no HIP device " + std::to_string(e->device));
     HIPCHK(hipSetDevice(e->device));
-    HIPCHK(hipStreamCreate(&e->stream));
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     e->ns = pick_ns(std::max(e->T.nup, e->T.ndn));
     e->build_device_system();
     if (e->precision != QW_PREC_FP64) mixed_setup(*e);
@@ -873,16 +899,22 @@ extern "C" qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const
     std::vector<void*> tmp;
     auto dalloc = [&](size_t bytes) {
       void* p = nullptr;
-      HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+      {
+        std::lock_guard<std::mutex> g(g_devmem_mu);
+        HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+      }
       tmp.push_back(p);
       return p;
     };
     struct Free {
       std::vector<void*>& v;
+      hipStream_t st;
       ~Free() {
+        (void)hipStreamSynchronize(st);
+        std::lock_guard<std::mutex> g(g_devmem_mu);
         for (void* p : v) (void)hipFree(p);
       }
-    } fr{tmp};
+    } fr{tmp, e->stream};
     double* d_rot = nullptr;
     double* d_rand = nullptr;
     if (rot) {
