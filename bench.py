@@ -62,36 +62,45 @@ def run_gpu(deck, mol, precision, nw, steps, warmup, tau):
                 acceptance=float(res["acceptance"]), kernels={k: [v[0], int(v[1])] for k, v in kt.items()})
 
 
-def accuracy_leg(deck, mol, precs, nw, nblock, nstep, tau):
-    """DMC of every precision from the same VMC-equilibrated walkers with the same random
-    streams: block-averaged energies, their mean and standard error per precision."""
+def accuracy_leg(deck, mol, precs, nw, nblock, nstep, tau, nequil):
+    """DMC of every precision from the same walkers with the same random streams: an
+    equilibration stretch of nequil steps, then nblock blocks of nstep steps.  Per precision:
+    block-averaged energies, their mean and standard error, and the block-by-block difference
+    from FP64 (the runs share their random numbers, so the paired difference is far more
+    precise than either error bar)."""
     from mainline_amd import engine as E
     from mainline_amd import synth
-    eq = E.Engine(deck, "fp64", seed=98)
-    eq.set_walkers(synth.random_configs(mol, nw, seed=9))
-    eq.run(E.method("VMC", nw, nblock=1, nstep=30, timestep=0.3, ndecorr=1))
-    pos, _ = eq.get_walkers(nw)
-    eq.close()
+    pos = synth.random_configs(mol, nw, seed=9)
     out = {}
     for p in precs:
         eng = E.Engine(deck, p, seed=99)
         eng.set_walkers(pos)
-        eng.run(E.method("DMC", nw, nblock=1, nstep=nstep, timestep=tau))  # equilibration block
+        eng.run(E.method("DMC", nw, nblock=1, nstep=nequil, timestep=tau))
         eb = eng.run(E.method("DMC", nw, nblock=nblock, nstep=nstep, timestep=tau))["eblock"][:, 0]
         out[p] = dict(mean=float(eb.mean()), err=float(eb.std(ddof=1) / np.sqrt(len(eb))), blocks=eb.tolist())
         log("[accuracy %s] E = %.5f +- %.5f" % (p, out[p]["mean"], out[p]["err"]))
         eng.close()
+    ref = out.get("fp64")
+    for p, v in out.items():
+        if ref is None:
+            continue
+        d = np.array(v["blocks"]) - np.array(ref["blocks"])
+        v["delta_vs_fp64"] = v["mean"] - ref["mean"]
+        v["combined_err"] = float(np.hypot(v["err"], ref["err"]))
+        v["agrees_2sigma"] = bool(abs(v["delta_vs_fp64"]) <= 2 * v["combined_err"])
+        v["paired_delta"] = float(d.mean())
+        v["paired_err"] = float(d.std(ddof=1) / np.sqrt(len(d))) if len(d) > 1 else 0.0
     return out
 
 
 def serving_leg(precision, njobs, rate, seed, tmp):
-    """Poisson stream of VMC/DMC requests (mean `rate` jobs/s) over molecules of 8..128
-    electrons (log-uniform), 1024 walkers and 1000..20000 walker-steps each, through the
-    serving loop; job latency is arrival to result."""
+    """Poisson stream of VMC/DMC requests (mean `rate` jobs/s) over molecules of 8..512
+    electrons (log-uniform, up to 1664 basis functions), 1024 walkers and 1000..20000
+    walker-steps each, through the serving loop; job latency is arrival to result."""
     from mainline_amd import engine as E
     from mainline_amd import synth
     rng = np.random.default_rng(seed)
-    sizes = [8, 12, 16, 24, 32, 48, 64, 96, 128]
+    sizes = [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512]
     decks = {}
     for ne in sizes:
         m = synth.z4_molecule(ne // 4, 20261300 + ne, name="srv%d" % ne)
@@ -101,7 +110,7 @@ def serving_leg(precision, njobs, rate, seed, tmp):
     jobs = []
     t0 = time.perf_counter()
     for i in range(njobs):
-        ne_req = float(np.exp(rng.uniform(np.log(8), np.log(128))))
+        ne_req = float(np.exp(rng.uniform(np.log(8), np.log(512))))
         ne = min(sizes, key=lambda x: abs(np.log(x) - np.log(ne_req)))
         dmc = rng.uniform() < 0.5
         nsteps = int(np.ceil(rng.uniform(1000, 20000) / 1024))
@@ -177,7 +186,7 @@ def main():
     ap.add_argument("--precision", default="auto", help="fp64, mixed_fp32, mixed_bf16, mixed_bf16x6 or auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--accuracy-blocks", type=int, default=8, help="0 disables the energy-agreement leg")
+    ap.add_argument("--accuracy-blocks", type=int, default=10, help="0 disables the energy-agreement leg")
     ap.add_argument("--serving-jobs", type=int, default=100, help="0 disables the serving leg")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only copy this many bytes device-to-device (counter calibration run) and exit")
@@ -201,14 +210,7 @@ def main():
         log("[%s] %.3e electron-moves/s, %.2f ms/step" % (p, results[p]["moves_per_s"], results[p]["ms_per_step"]))
     acc = None
     if a.accuracy_blocks > 0:
-        acc = accuracy_leg(deck, mol, list(results), a.walkers, a.accuracy_blocks, 10, tau)
-        ref = acc.get("fp64")
-        for p, v in acc.items():
-            if ref is None:
-                continue
-            v["delta_vs_fp64"] = v["mean"] - ref["mean"]
-            v["combined_err"] = float(np.hypot(v["err"], ref["err"]))
-            v["agrees_2sigma"] = bool(abs(v["delta_vs_fp64"]) <= 2 * v["combined_err"])
+        acc = accuracy_leg(deck, mol, list(results), a.walkers, a.accuracy_blocks, 10, tau, 40)
     # headline: the fastest mixed precision whose energy agrees with FP64 within two combined
     # error bars (FP64 itself when none does or no mixed mode was run)
     head = "fp64" if "fp64" in results else precs[0]

@@ -156,6 +156,23 @@ __device__ inline double wave_maxd(double v) {
   return fmax(fmax(readlane_d(v, 0), readlane_d(v, 16)), fmax(readlane_d(v, 32), readlane_d(v, 48)));
 }
 __device__ inline bool wave_any(bool p) { return __ballot(p) != 0ull; }
+// Orders the LDS accesses of one wavefront around this point (the lanes of a wave run in
+// lockstep and the LDS serves a wave's requests in issue order, so the compiler fence is all
+// that is needed when one wave owns the data).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Barrier between the LDS producer and consumer of one walker: the block barrier when a block
+// is one walker (WPB = 1), the wavefront fence when a block holds WPB walkers, one per wave.
+template <int WPB>
+__device__ inline void walker_sync() {
+  if constexpr (WPB == 1)
+    __syncthreads();
+  else
+    wave_sync();
+}
 // Lane holding the largest v (the highest such lane on ties); -1 when no lane compares
 // equal to the maximum (NaN inputs).
 __device__ inline int wave_argmax_last(double v, int lane) {

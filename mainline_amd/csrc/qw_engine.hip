@@ -398,7 +398,9 @@ struct qw_engine {
   // ---- kernels
   // LDS: tiles of basis values (5 x chunk_nf doubles) and active-function indices (chunk_nf ints)
   size_t lds_rows() const { return 5 * (size_t)S.chunk_nf * sizeof(double) + (size_t)S.chunk_nf * sizeof(int); }
-  size_t lds_sweep() const { return sweep_lds_bytes(S.ne, S.na, S.nsh, S.ntab, ns, kCapF64, sizeof(double)); }
+  size_t lds_sweep(int depth) const {
+    return sweep_lds_bytes(S.ne, S.na, S.nsh, S.ntab, ns, kCapF64, sizeof(double), depth);
+  }
   size_t lds_local() const {
     return (7 * (size_t)S.ne + 5 * (size_t)S.chunk_nf) * sizeof(double) + (size_t)S.chunk_nf * sizeof(int);
   }
@@ -422,9 +424,13 @@ struct qw_engine {
       case 8: go(k_inv<8, TIN, TI>); break;
       case 16: go(k_inv<16, TIN, TI>); break;
       case 32: go(k_inv<32, TIN, TI>); break;
-      default: go(k_inv<64, TIN, TI>); break;
+      default:
+        if (inv_rows) go(k_inv<64, TIN, TI>);
+        else go(k_inv_blk<TIN, TI>);
+        break;
     }
   }
+  bool inv_rows = false;  // 33..64 per spin: row-per-lane inversion instead of 8x8 blocks (QW_INV=rows)
   void refresh_all() {
     if (big) {
       big_refresh_all();
@@ -461,7 +467,7 @@ struct qw_engine {
   template <int DEPTH, int U>
   void sweep_fp64(const SweepArgs& A) {
     auto go = [&](auto kern) {
-      tuned_launch([&] { hipLaunchKernelGGL(kern, dim3(nw), dim3(64), lds_sweep(), stream, S, W, A); });
+      tuned_launch([&] { hipLaunchKernelGGL(kern, dim3(nw), dim3(64), lds_sweep(DEPTH), stream, S, W, A); });
     };
     switch (ns) {
       case 4: go(k_sweep<4, DEPTH, U>); break;
@@ -472,26 +478,37 @@ struct qw_engine {
     }
   }
 
-  // ---- online tuning of the LCAO load depth U (coefficient loads in flight).  The
-  // candidates do the same arithmetic in the same order, so the first sweep launches of an
-  // engine simply run with each candidate in turn (kTuneReps times), timed with events; the
-  // fastest is kept for the engine's lifetime.  QW_TUNE=0 keeps the default.
+  // ---- online tuning of the sweep launch: LCAO coefficient loads in flight (U) and, on the
+  // FP32 mixed path, whether a block stages the coefficients in LDS for several walkers
+  // (choice 100 + U).  The candidates do the same arithmetic in the same order, so the first
+  // sweep launches of an engine simply run with each candidate in turn (kTuneReps times),
+  // timed with events; the fastest is kept for the engine's lifetime.  QW_TUNE=0 keeps the
+  // default; QW_SWEEP=<choice> forces one.
   static constexpr int kTuneReps = 2;
   bool tune_enabled = true;
-  int tune_u = 0;      // chosen depth, 0 while measuring
+  int tune_u = 0;      // chosen launch, 0 while measuring
   int tune_n = 0;      // candidate launches issued
   int tune_cur = 0;    // candidate of the launch being issued
+  bool cl_ok = false;  // LDS-staged coefficients fit this device and system
+  std::vector<int> tune_cands;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> tune_ev;
   int u_default() const { return precision == QW_PREC_FP64 ? 4 : 8; }
-  int u_cand(int i) const { return precision == QW_PREC_FP64 ? (2 << i) : (4 << i); }
+  const std::vector<int>& cands() {
+    if (tune_cands.empty()) {
+      if (precision == QW_PREC_FP64) tune_cands = {2, 4, 8};
+      else tune_cands = {4, 8, 16};
+      if (cl_ok) tune_cands.insert(tune_cands.end(), {104, 108, 116});
+    }
+    return tune_cands;
+  }
   int pick_u() {
     if (tune_u) return tune_u;
     if (!tune_enabled || capturing || big || (precision != QW_PREC_FP64 && precision != QW_PREC_MIXED_FP32))
       return u_default();
-    tune_cur = u_cand(tune_n % 3);
+    tune_cur = cands()[tune_n % cands().size()];
     return tune_cur;
   }
-  // Launches a sweep; while U is being tuned the launch is bracketed by events.
+  // Launches a sweep; while the launch choice is being tuned the launch is bracketed by events.
   template <typename F>
   void tuned_launch(F f) {
     const bool measuring = !tune_u && tune_enabled && !capturing && tune_cur != 0;
@@ -505,11 +522,12 @@ struct qw_engine {
     HIPCHK(hipEventRecord(b, stream));
     tune_ev.push_back({tune_cur, {a, b}});
     tune_cur = 0;
-    if (++tune_n < 3 * kTuneReps) return;
+    const size_t nc = cands().size();
+    if (++tune_n < (int)(nc * kTuneReps)) return;
     HIPCHK(hipEventSynchronize(b));
     double best = 1e300;
-    for (int i = 0; i < 3; i++) {
-      const int u = u_cand(i);
+    for (size_t i = 0; i < nc; i++) {
+      const int u = tune_cands[i];
       double t = 1e300;
       for (auto& x : tune_ev)
         if (x.first == u) {
@@ -766,6 +784,10 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
   }
   if (const char* g = std::getenv("QW_GRAPHS")) e->graph_mode = (g[0] == '0') ? 0 : (g[0] == '1') ? 1 : -1;
   if (const char* t = std::getenv("QW_TUNE")) e->tune_enabled = t[0] != '0';
+  if (const char* v = std::getenv("QW_INV")) e->inv_rows = std::strcmp(v, "rows") == 0;
+  if (const char* c = std::getenv("QW_SWEEP"))
+    for (int x : e->cands())
+      if (x == std::atoi(c)) e->tune_u = x;
   *out = e.release();
   return QW_OK;
 }

@@ -110,18 +110,17 @@ __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) 
 // the same Schur-complement entries (src/program_utils/MatrixAlgebrac.cpp:108-198).  Rows are
 // never exchanged.  With A' the row-permuted matrix (A'[j] = A[p_j]), lane p_j ends holding
 // row j of inv(A') in place; the transposed inverse out[k][i] = inv(A)[i][k] is then
-// inv(A')[i][q_k] with q = p^-1, so lane p_i stores its register m at out[p_m][i].
-// A pivot row is left unscaled in its lane with a per-lane scale (1/pivot), so each
-// elimination step is one FMA per element.  The row is held as vectors of 16 doubles so the
-// column j (uniform, dynamic) is read and written with register-relative moves.
+// inv(A')[i][q_k] with q = p^-1, so lane p_i stores its column m at out[p_m][i].
+// The row is held as NB vectors of VW doubles.  The step loop runs over the vectors (unrolled)
+// and inside each over its columns, so column j is read and written with one register-relative
+// move in a vector known at compile time.  A pivot row is left unscaled in its lane with a
+// per-lane scale (1/pivot), so each elimination step is one FMA per element.
 // log|det| = sum log|pivot|; the sign comes from the pivots and the permutation parity; an
 // exactly zero pivot flags the matrix singular as the reference does (log|det| = -1e201,
 // inverse zero).
 template <int NS, typename TIN, typename TI>
 __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
                                             double* sgn_out, int nw, int only_spin) {
-  constexpr int VW = NS < 16 ? NS : 16, NB = NS / VW;
-  typedef double dvec __attribute__((ext_vector_type(VW)));
   __shared__ double prow2[2][NS];
   __shared__ double pvs[NS];
   __shared__ int pl[NS];
@@ -139,6 +138,8 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
     return;
   }
   const bool row = lane < n;
+  constexpr int VW = NS < 16 ? NS : 16, NB = NS / VW;
+  typedef double dvec __attribute__((ext_vector_type(VW)));
   dvec a[NB];
   const TIN* rp = mo + ((size_t)w * S.ne + off + (row ? lane : 0)) * 5 * nsp;
   double big = 0;
@@ -155,50 +156,49 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
   bool pivoted = !row;
   double sc = 1.0;
   int qi = 0;
-  for (int j = 0; j < n && !singular; j++) {
-    const int jb = j / VW, jo = j % VW;
-    double aj = 0;
+  bool done = singular;
 #pragma unroll
-    for (int b = 0; b < NB; b++)
-      if (b == jb) aj = a[b][jo];
-    const double cand = pivoted ? -1.0 : vv * fabs(aj);
-    const int p = wave_argmax_last(cand, lane);
-    // no candidate (NaN entries) or a zero pivot: singular
-    if (p < 0 || !(wave_bcast(cand, p) >= 0.0)) {
-      singular = true;
-      break;
-    }
-    const double pv = wave_bcast(aj, p);
-    if (pv == 0.0) {
-      singular = true;
-      break;
-    }
-    if (lane == 0) {
-      pl[j] = p;
-      pvs[j] = pv;
-    }
-    double* prow = prow2[j & 1];
-    const bool me = lane == p;
-    if (me) {
+  for (int jb = 0; jb < NB; jb++) {
+    for (int jo = 0; jo < VW && !done; jo++) {
+      const int j = jb * VW + jo;
+      if (j >= n) {
+        done = true;
+        break;
+      }
+      const double aj = a[jb][jo];
+      const double cand = pivoted ? -1.0 : vv * fabs(aj);
+      const int p = wave_argmax_last(cand, lane);
+      // no candidate (NaN entries) or a zero pivot: singular
+      const double pv = p < 0 ? 0.0 : wave_bcast(aj, p);
+      if (p < 0 || !(wave_bcast(cand, p) >= 0.0) || pv == 0.0) {
+        singular = done = true;
+        break;
+      }
+      if (lane == 0) {
+        pl[j] = p;
+        pvs[j] = pv;
+      }
+      double* prow = prow2[j & 1];
+      const bool me = lane == p;
+      if (me) {
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+          for (int k = 0; k < VW; k++) prow[b * VW + k] = a[b][k];
+      }
+      __syncthreads();
+      const double rpv = 1.0 / pv;
+      const double g = me ? 0.0 : aj * rpv;
 #pragma unroll
       for (int b = 0; b < NB; b++)
 #pragma unroll
-        for (int k = 0; k < VW; k++) prow[b * VW + k] = a[b][k];
-    }
-    __syncthreads();
-    const double g = me ? 0.0 : aj * (1.0 / pv);
-#pragma unroll
-    for (int b = 0; b < NB; b++)
-#pragma unroll
-      for (int k = 0; k < VW; k++) a[b][k] = fma(-g, prow[b * VW + k], a[b][k]);
-    const double nv = me ? 1.0 : -g;
-#pragma unroll
-    for (int b = 0; b < NB; b++)
-      if (b == jb) a[b][jo] = nv;
-    if (me) {
-      sc = 1.0 / pv;
-      qi = j;
-      pivoted = true;
+        for (int k = 0; k < VW; k++) a[b][k] = fma(-g, prow[b * VW + k], a[b][k]);
+      a[jb][jo] = me ? 1.0 : -g;
+      if (me) {
+        sc = rpv;
+        qi = j;
+        pivoted = true;
+      }
     }
   }
   if (singular) {
@@ -236,6 +236,170 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
   }
 }
 
+// ---------------------------------------------------------------- k_inv_blk
+// The same inversion for 33..64 electrons of a spin with the matrix spread over the wave as
+// 8x8 blocks: lane (bi, bj) = (lane / 8, lane % 8) holds rows 8bi..8bi+7 and columns
+// 8bj..8bj+7, column c of its block as one register vector a[c][r].  A step exchanges only
+// the pivot row (8 values per lane) and the multipliers (8 per lane) through LDS, a quarter of
+// what the row-per-lane layout broadcasts, and every register index except the pivot row's is
+// a compile-time constant.  Each element sees the operations of k_inv in the same order (same
+// pivots, multipliers a[i][j] * (1/pivot), one FMA per step, column j replaced by -multiplier
+// or 1), so both kernels give bitwise-equal results.  Rows and columns >= n hold the identity
+// and are never pivots before step n.
+template <typename TIN, typename TI>
+__global__ __launch_bounds__(64) void k_inv_blk(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                                double* sgn_out, int nw, int only_spin) {
+  typedef double dv8 __attribute__((ext_vector_type(8)));
+  __shared__ double prowL[2][64];
+  __shared__ double gL[2][64];
+  __shared__ double pvs[64];
+  __shared__ int pl[64];
+  __shared__ int qinv[64];
+  const int lane = threadIdx.x, bi = lane >> 3, bj = lane & 7;
+  const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+  const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
+  if (w >= nw) return;
+  const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
+  TI* out = inv_out + ((size_t)w * 2 + s) * nsp * nsp;
+  if (n == 0) {
+    if (lane == 0) {
+      logdet_out[2 * w + s] = 0;
+      sgn_out[2 * w + s] = 1;
+    }
+    return;
+  }
+  dv8 a[8];
+  double vv[8];
+  bool zero_row = false;
+  int pivm = 0;  // bit r: row 8bi + r already a pivot (rows >= n count as such)
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const int row = 8 * bi + r;
+    const TIN* rp = mo + ((size_t)w * S.ne + off + (row < n ? row : 0)) * 5 * nsp;
+    double mx = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int col = 8 * bj + c;
+      const double v = (row < n && col < n) ? (double)rp[col] : (row == col ? 1.0 : 0.0);
+      a[c][r] = v;
+      mx = fmax(mx, fabs(v));
+    }
+    // row maximum over the 8 lanes of the row block (lanes 8bi .. 8bi+7)
+    mx = fmax(mx, dpp_d<kDppXor1>(mx));
+    mx = fmax(mx, dpp_d<kDppXor2>(mx));
+    mx = fmax(mx, dpp_d<kDppHalfMirror>(mx));
+    vv[r] = mx == 0.0 ? 0.0 : 1.0 / mx;
+    zero_row |= row < n && mx == 0.0;
+    if (row >= n) pivm |= 1 << r;
+  }
+  bool singular = wave_any(zero_row);
+  bool done = singular;
+  for (int J = 0; J < 8 && !done; J++) {
+    const bool mine = bj == J;  // this lane holds column 8J + c of its rows
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int j = 8 * J + c;
+      if (done || j >= n) {
+        done = true;
+        break;
+      }
+      double best = -1.0, braw = 0.0;
+      int br = 0;
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const double x = a[c][r];
+        const double cd = (mine && !((pivm >> r) & 1)) ? vv[r] * fabs(x) : -1.0;
+        if (cd >= best) {
+          best = cd;
+          br = r;
+          braw = x;
+        }
+      }
+      const int L = wave_argmax_last(best, lane);
+      // no candidate (NaN entries) or a zero pivot: singular
+      const double pv = L < 0 ? 0.0 : wave_bcast(braw, L);
+      if (L < 0 || !(wave_bcast(best, L) >= 0.0) || pv == 0.0) {
+        singular = done = true;
+        break;
+      }
+      const int p = 8 * (L >> 3) + __builtin_amdgcn_readlane(br, L);
+      const int pb = p >> 3, pr = p & 7;
+      if (lane == 0) {
+        pl[j] = p;
+        pvs[j] = pv;
+      }
+      double* prow = prowL[j & 1];
+      double* gw = gL[j & 1];
+      if (bi == pb) {
+#pragma unroll
+        for (int c2 = 0; c2 < 8; c2++) prow[8 * bj + c2] = a[c2][pr];
+      }
+      const double rpv = 1.0 / pv;
+      if (mine) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) gw[8 * bi + r] = (8 * bi + r == p) ? 0.0 : a[c][r] * rpv;
+      }
+      __syncthreads();
+      double pr8[8], g8[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        pr8[k] = prow[8 * bj + k];
+        g8[k] = gw[8 * bi + k];
+      }
+#pragma unroll
+      for (int c2 = 0; c2 < 8; c2++)
+#pragma unroll
+        for (int r = 0; r < 8; r++) a[c2][r] = fma(-g8[r], pr8[c2], a[c2][r]);
+      if (mine) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) a[c][r] = (8 * bi + r == p) ? 1.0 : -g8[r];
+      }
+      if (bi == pb) pivm |= 1 << pr;
+    }
+  }
+  if (singular) {
+    if (lane == 0) {
+      logdet_out[2 * w + s] = -1e201;
+      sgn_out[2 * w + s] = 1;
+    }
+    for (int k = lane; k < n * nsp; k += kWave) out[k] = (TI)0;
+    return;
+  }
+  __syncthreads();
+  for (int j = lane; j < n; j += kWave) qinv[pl[j]] = j;
+  // log|det|, negative pivots and the inversion count of the pivot sequence
+  double lg = 0;
+  int neg = 0, inv_cnt = 0;
+  for (int j = lane; j < n; j += kWave) {
+    lg += log(fabs(pvs[j]));
+    neg += pvs[j] < 0;
+    const int pj = pl[j];
+    for (int m = j + 1; m < n; m++) inv_cnt += pl[m] < pj;
+  }
+  lg = wsum(lg);
+  const int par = wave_isum(neg + inv_cnt);
+  if (lane == 0) {
+    logdet_out[2 * w + s] = lg;
+    sgn_out[2 * w + s] = (par & 1) ? -1.0 : 1.0;
+  }
+  __syncthreads();
+  // row 8bi+r was the pivot of step q: it holds row q of the inverse of the row-permuted
+  // matrix, scaled by 1/pivot_q; its column m goes to out[p_m][q]
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const int row = 8 * bi + r;
+    if (row < n) {
+      const int q = qinv[row];
+      const double sc = 1.0 / pvs[q];
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int m = 8 * bj + c;
+        if (m < n) out[(size_t)pl[m] * nsp + q] = (TI)(a[c][r] * sc);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- k_sweep
 struct SweepArgs {
   int s, k0, k1;
@@ -247,6 +411,30 @@ struct SweepArgs {
   const unsigned* stepp;  // device step counter (replaces `step` when set; graph replays)
 };
 __device__ inline unsigned step_of(unsigned step, const unsigned* stepp) { return stepp ? *stepp : step; }
+
+// Random numbers of the moves of one sweep launch, drawn one move per lane: lane l takes the
+// electron off+k0+l and draws its proposal Gaussians and acceptance uniform for each stage from
+// the same counter-based streams a per-move draw uses, so the wave evaluates the draws of up
+// to 64 moves at once instead of repeating each on every lane.  rnd: LDS [move][DEPTH][4]
+// (Gaussians x, y, z, uniform).
+template <int DEPTH>
+__device__ inline void sweep_randoms(const SweepArgs& A, unsigned w, int off, double* rnd, int lane) {
+  const unsigned st = step_of(A.step, A.stepp);
+  for (int k = A.k0 + lane; k < A.k1; k += kWave) {
+    const unsigned e = (unsigned)(off + k);
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++) {
+      double g[3];
+      gauss3(A.seed, w, st, e, (d == 0 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      double* r = rnd + ((size_t)(k - A.k0) * DEPTH + d) * 4;
+      r[0] = g[0];
+      r[1] = g[1];
+      r[2] = g[2];
+      r[3] = uniform(A.seed, w, st, e, (d == 0 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
+    }
+  }
+}
+__host__ __device__ inline size_t sweep_rand_bytes(int ns, int depth) { return (size_t)ns * depth * 4 * sizeof(double); }
 
 // Advances the device step counter (one thread).
 __global__ void k_step_advance(unsigned* stepp) {
@@ -281,9 +469,10 @@ __device__ inline void write_det_parts(const DevSys& S, const DevWalkers& W, int
 // Basis functions per LDS tile batch in the FP64 sweep.
 constexpr int kCapF64 = 128;
 
-__host__ __device__ inline size_t sweep_lds_bytes(int ne, int na, int nsh, int ntab, int ns, int cap, int tsize) {
+__host__ __device__ inline size_t sweep_lds_bytes(int ne, int na, int nsh, int ntab, int ns, int cap, int tsize,
+                                                  int depth) {
   return (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15) + ((sys_lds_bytes(na, nsh, ntab) + 15) & ~(size_t)15) +
-         (size_t)5 * cap * tsize + (size_t)cap * sizeof(int) + (size_t)2 * ns * tsize + 16;
+         sweep_rand_bytes(ns, depth) + (size_t)5 * cap * tsize + (size_t)cap * sizeof(int) + (size_t)2 * ns * tsize + 16;
 }
 
 // U: coefficient loads in flight in the LCAO (chosen by the engine's online tuner; every U
@@ -297,18 +486,21 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
   double* pos = lds;
   SysLds V;
   char* q = sys_lds_carve(S, (char*)lds + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
-  double* chi = (double*)q;
+  double* rnd = (double*)q;
+  double* chi = rnd + sweep_rand_bytes(NS, DEPTH) / sizeof(double);
   int* roff = (int*)(chi + 5 * kCapF64);
   double* rowb = (double*)(roff + kCapF64);
   double* phib = rowb + NS;
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   stage_sys(S, V, lane);
+  sweep_randoms<DEPTH>(A, (unsigned)w, off, rnd, lane);
   double inv[NS];
   const double* ig = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
 #pragma unroll
   for (int j = 0; j < NS; j++) inv[j] = (lane < n && j < n) ? ig[(size_t)lane * nsp + j] : 0.0;
   double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
   int nacc = 0;
+  double dprod = 1.0;
   double* mog = W.mo + (size_t)w * ne * 5 * nsp;
   const double tau = A.tau, stau = sqrt(tau);
   // MO gradient row of the electron about to move, loaded one move ahead
@@ -342,10 +534,10 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
     double acc_first = 0;
     for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
       const double t = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
-      double g[3], dr[3], rn[3], dn[3];
-      gauss3(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      const double* rq = rnd + ((size_t)(k - A.k0) * DEPTH + (dep - 1)) * 4;
+      double dr[3], rn[3], dn[3];
       lim_drift(d0, t, dr);
-      for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
+      for (int d = 0; d < 3; d++) rn[d] = r0[d] + (rq[d] * st + dr[d]);
       mo_lap_point<double, double, U>(S, V, S.spline, S.C[s], n, rn, chi, roff, kCapF64, phi, lane);
       double t0 = wsum(rk * phi[0]);
       double t1 = wsum(rk * phi[1]), t2 = wsum(rk * phi[2]), t3 = wsum(rk * phi[3]);
@@ -354,10 +546,15 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
       dn[0] = t1 / t0 + gj[0];
       dn[1] = t2 / t0 + gj[1];
       dn[2] = t3 / t0 + gj[2];
-      double lnn = log(fabs(t0)) + (un - u0);
-      bool neg = t0 < 0;
-      double acc;
-      if (dep == 1) {
+      const bool neg = t0 < 0;
+      double acc, lnn = 0;
+      if constexpr (DEPTH == 1) {
+        // |ratio|^2 exp(dtrans) as t0^2 exp(2 du + dtrans): one exponential, no logarithm
+        acc = (A.restrict_nodes && neg)
+                  ? 0.0
+                  : fmin(1.0, t0 * t0 * exp(2.0 * (un - u0) + (trans(rn, dn, r0, t) - trans(r0, d0, rn, t))));
+      } else if (dep == 1) {
+        lnn = log(fabs(t0)) + (un - u0);
         double rr = exp(lnn);
         acc = (A.restrict_nodes && neg) ? 0.0
                                         : fmin(1.0, rr * rr * exp(trans(rn, dn, r0, t) - trans(r0, d0, rn, t)));
@@ -367,6 +564,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
       } else {
         // delayed rejection from the origin to the second point, through the rejected first
         // point (src/method/Split_sample.cpp:302-373)
+        lnn = log(fabs(t0)) + (un - u0);
         double rr = exp(lnn);
         double pt = (trans(rn, dn, r1, tau) - trans(r0, d0, r1, tau)) + (trans(rn, dn, r0, t) - trans(r0, d0, rn, t));
         double r21 = exp(ln1 - lnn);
@@ -376,8 +574,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
         else if (fabs(den) < 1e-10) acc = fabs(num) < 1e-10 ? 0.0 : 1.0;
         else acc = fmin(1.0, rr * rr * exp(pt) * num / den);
       }
-      double u = uniform(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
-      if (acc + u > 1.0) {
+      if (acc + rq[3] > 1.0) {
         accepted = true;
         R = t0;
         for (int d = 0; d < 3; d++) r1[d] = rn[d];
@@ -404,7 +601,12 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
         }
         for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + lane] = phi[c];
       }
-      logdet += log(fabs(R));
+      // accepted ratios multiply into dprod, folded into log|det| before it could overflow
+      dprod *= R;
+      if (!(fabs(dprod) < 0x1p+512 && fabs(dprod) > 0x1p-512)) {
+        logdet += log(fabs(dprod));
+        dprod = 1.0;
+      }
       if (R < 0) sgn = -sgn;
       nacc++;
       if (lane < 3) pos[3 * e + lane] = r1[lane];
@@ -425,6 +627,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
   }
   if (A.dets) write_det_parts<NS>(S, W, w, s, inv, (const double*)mog, lane);
   for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
+  logdet += log(fabs(dprod));
   if (lane == 0) {
     W.logdet[2 * w + s] = logdet;
     W.sgn[2 * w + s] = sgn;

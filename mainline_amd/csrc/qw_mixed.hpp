@@ -231,41 +231,90 @@ __global__ __launch_bounds__(64) void k_mo_rows_mx(DevSys S, MixedTables M, DevW
 }
 
 // ---------------------------------------------------------------- k_sweep_mx
-// Basis functions per LDS tile batch in the FP32 sweep.
+// Basis functions per LDS tile batch in the FP32 sweep (one walker per block), and with the
+// coefficients staged in LDS (several walkers per block, where LDS is shared by all of them).
 constexpr int kCapF32 = 256;
+constexpr int kCapCL = 64;
+// Walkers per block of the LDS-staged variant: two waves per SIMD of one compute unit.
+constexpr int kWpbCL = 8;
+constexpr size_t kLdsMax = 163840;  // LDS per workgroup on gfx950
 
-// LDS of k_sweep_mx: positions | system tables | basis tile (FP32 sparse: 5 x cap floats +
-// cap row offsets; MFMA: 5 x ldchi floats + 5 x 64 accumulator exchange + ldchi ints) |
-// inverse row and value row broadcast (2 x NS floats).
-__host__ __device__ inline size_t sweep_mx_lds_bytes(int ne, int na, int nsh, int ntab, int ns, int mode, int ldchi) {
-  const size_t tile = mode == LCAO_FP32 ? (size_t)5 * kCapF32 * 4 + (size_t)kCapF32 * 4
+__host__ __device__ inline size_t lds_round16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+// Per-walker LDS of k_sweep_mx: positions | (several walkers per block: active-shell list) |
+// basis tile (FP32 sparse: 5 x cap floats + cap row offsets; MFMA: 5 x ldchi floats + 5 x 64
+// accumulator exchange + ldchi ints) | inverse row and value row broadcast (2 x NS floats).
+__host__ __device__ inline size_t sweep_mx_walker_lds(int ne, int nsh, int ns, int mode, int ldchi, int wpb,
+                                                      int depth) {
+  const int cap = wpb > 1 ? kCapCL : kCapF32;
+  const size_t tile = mode == LCAO_FP32 ? (size_t)5 * cap * 4 + (size_t)cap * 4
                                         : (size_t)(5 * ldchi + 5 * 64) * 4 + (size_t)ldchi * 4;
-  return (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15) + ((sys_lds_bytes(na, nsh, ntab) + 15) & ~(size_t)15) +
-         tile + (size_t)2 * ns * 4 + 16;
+  return lds_round16((size_t)3 * ne * sizeof(double)) + (wpb > 1 ? lds_round16((size_t)nsh * 4) : 0) +
+         sweep_rand_bytes(ns, depth) + tile + (size_t)2 * ns * 4 + 16;
+}
+// LDS of k_sweep_mx: (several walkers per block: the spin's coefficients, nb x nsp floats) |
+// system tables | per-walker regions.
+__host__ __device__ inline size_t sweep_mx_lds_bytes(int ne, int na, int nsh, int ntab, int ns, int mode, int ldchi,
+                                                     int depth, int nb = 0, int nsp = 0, int wpb = 1) {
+  return (wpb > 1 ? lds_round16((size_t)nb * nsp * 4) : 0) + lds_round16(sys_lds_bytes(na, nsh, ntab)) +
+         (size_t)wpb * sweep_mx_walker_lds(ne, nsh, ns, mode, ldchi, wpb, depth);
 }
 
-template <int NS, int DEPTH, int MODE, int U>
-__global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W, SweepArgs A) {
+// Moves of electrons [k0, k1) of spin s for WPB walkers per block (one wavefront each); all
+// FP32 except positions, drift/proposal, acceptance and log|det| (FP64).  With WPB > 1 the
+// block first stages the spin's MO coefficients in LDS, so the LCAO of every move reads LDS
+// instead of L2; the walkers then run independently (wavefront-level barriers only).  The
+// arithmetic is the same for every WPB and U.
+template <int NS, int DEPTH, int MODE, int U, int WPB>
+__global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W,
+                                                                          SweepArgs A) {
+  static_assert(WPB == 1 || MODE == LCAO_FP32, "LDS-staged coefficients are FP32");
   extern __shared__ double ldsd[];
-  const int lane = threadIdx.x, w = blockIdx.x;
-  if (w >= W.nw) return;
+  const int lane = (int)threadIdx.x & (kWave - 1), wid = WPB == 1 ? 0 : (int)(threadIdx.x / kWave);
+  const int w = (int)blockIdx.x * WPB + wid;
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
-  double* pos = ldsd;
+  char* base = (char*)ldsd;
+  float* Cl = (float*)base;
+  if constexpr (WPB > 1) base += lds_round16((size_t)S.nb * nsp * 4);
   SysLds V;
-  char* q = sys_lds_carve(S, (char*)ldsd + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
+  base = sys_lds_carve(S, base, V);
+  if constexpr (WPB > 1) {
+    // stage the coefficients (16-byte chunks while aligned) and the system tables
+    const int tid = (int)threadIdx.x, nt = 64 * WPB, nc = S.nb * n;
+    const float4* src4 = reinterpret_cast<const float4*>(M.Cf[s]);
+    float4* dst4 = reinterpret_cast<float4*>(Cl);
+    for (int i = tid; i < nc / 4; i += nt) dst4[i] = src4[i];
+    for (int i = (nc / 4) * 4 + tid; i < nc; i += nt) Cl[i] = M.Cf[s][i];
+    stage_sys(S, V, tid, nt);
+    base += (size_t)wid * sweep_mx_walker_lds(ne, S.nsh, NS, MODE, M.ldchi, WPB, DEPTH);
+    __syncthreads();
+  } else {
+    stage_sys(S, V, lane);
+  }
+  if (w >= W.nw) return;
+  double* pos = (double*)base;
+  char* q = base + lds_round16((size_t)3 * ne * sizeof(double));
+  if constexpr (WPB > 1) {
+    V.actl = (int*)q;
+    q += lds_round16((size_t)S.nsh * 4);
+  }
+  constexpr int cap = WPB > 1 ? kCapCL : kCapF32;
+  double* rnd = (double*)q;
+  q += sweep_rand_bytes(NS, DEPTH);
   float* chi = (float*)q;
-  float* phis = chi + 5 * (MODE == LCAO_FP32 ? kCapF32 : M.ldchi);  // MFMA modes only
+  float* phis = chi + 5 * (MODE == LCAO_FP32 ? cap : M.ldchi);  // MFMA modes only
   int* flist = (int*)(MODE == LCAO_FP32 ? phis : phis + 5 * 64);
-  float* rowb = (float*)(flist + (MODE == LCAO_FP32 ? kCapF32 : M.ldchi));
+  float* rowb = (float*)(flist + (MODE == LCAO_FP32 ? cap : M.ldchi));
   float* phib = rowb + NS;
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
-  stage_sys(S, V, lane);
+  sweep_randoms<DEPTH>(A, (unsigned)w, off, rnd, lane);
   float inv[NS];
   const float* ig = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
 #pragma unroll
   for (int j = 0; j < NS; j++) inv[j] = (lane < n && j < n) ? ig[(size_t)lane * nsp + j] : 0.f;
   double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
   int nacc = 0;
+  double dprod = 1.0;
   float* mog = W.mof + (size_t)w * ne * 5 * nsp;
   const double tau = A.tau, stau = sqrt(tau);
   float m1 = 0, m2 = 0, m3 = 0;
@@ -275,7 +324,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
     m2 = mog[(e0 * 5 + 2) * nsp + lane];
     m3 = mog[(e0 * 5 + 3) * nsp + lane];
   }
-  __syncthreads();
+  walker_sync<WPB>();
   for (int k = A.k0; k < A.k1; k++) {
     const int e = off + k;
     double r0[3] = {pos[3 * e], pos[3 * e + 1], pos[3 * e + 2]};
@@ -283,7 +332,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
 #pragma unroll
       for (int j = 0; j < NS; j++) rowb[j] = inv[j];
     }
-    __syncthreads();
+    walker_sync<WPB>();
     const float rk = lane < n ? rowb[lane] : 0.f;
     float gj0[3];
     const float u0 = jastrow_e_f(S, e, r0, pos, gj0, lane, V.epos, 4);
@@ -295,12 +344,14 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
     double acc_first = 0;
     for (int dep = 1; dep <= DEPTH && !accepted; dep++) {
       const double t = dep == 1 ? tau : tau * 0.5, st = dep == 1 ? stau : sqrt(tau * 0.5);
-      double g[3], dr[3], rn[3], dn[3];
-      gauss3(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
+      const double* rq = rnd + ((size_t)(k - A.k0) * DEPTH + (dep - 1)) * 4;
+      double dr[3], rn[3], dn[3];
       lim_drift(d0, t, dr);
-      for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
-      if constexpr (MODE == LCAO_FP32)
-        mo_lap_point<float, float, U>(S, V, M.spline, M.Cf[s], n, rn, chi, flist, kCapF32, phi, lane);
+      for (int d = 0; d < 3; d++) rn[d] = r0[d] + (rq[d] * st + dr[d]);
+      if constexpr (MODE == LCAO_FP32 && WPB > 1)
+        mo_lap_point<float, float, U, WPB>(S, V, M.spline, (const float*)Cl, n, rn, chi, flist, cap, phi, lane);
+      else if constexpr (MODE == LCAO_FP32)
+        mo_lap_point<float, float, U>(S, V, M.spline, M.Cf[s], n, rn, chi, flist, cap, phi, lane);
       else
         lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, flist, phi, lane);
       float t0 = wsumf(rk * phi[0]);
@@ -311,16 +362,23 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
       dn[0] = (double)(t1 * it0 + gj[0]);
       dn[1] = (double)(t2 * it0 + gj[1]);
       dn[2] = (double)(t3 * it0 + gj[2]);
-      double lnn = log(fabs((double)t0)) + (double)(un - u0);
-      bool neg = t0 < 0;
-      double acc;
-      if (dep == 1) {
+      const bool neg = t0 < 0;
+      double acc, lnn = 0;
+      if constexpr (DEPTH == 1) {
+        // |ratio|^2 exp(dtrans) as t0^2 exp(2 du + dtrans): one exponential, no logarithm
+        const double td = (double)t0;
+        acc = (A.restrict_nodes && neg)
+                  ? 0.0
+                  : fmin(1.0, td * td * exp(2.0 * (double)(un - u0) + (trans(rn, dn, r0, t) - trans(r0, d0, rn, t))));
+      } else if (dep == 1) {
+        lnn = log(fabs((double)t0)) + (double)(un - u0);
         double rr = exp(lnn);
         acc = (A.restrict_nodes && neg) ? 0.0 : fmin(1.0, rr * rr * exp(trans(rn, dn, r0, t) - trans(r0, d0, rn, t)));
         acc_first = acc;
         for (int d = 0; d < 3; d++) r1[d] = rn[d], d1[d] = dn[d];
         ln1 = lnn;
       } else {
+        lnn = log(fabs((double)t0)) + (double)(un - u0);
         double rr = exp(lnn);
         double pt = (trans(rn, dn, r1, tau) - trans(r0, d0, r1, tau)) + (trans(rn, dn, r0, t) - trans(r0, d0, rn, t));
         double r21 = exp(ln1 - lnn);
@@ -330,8 +388,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
         else if (fabs(den) < 1e-10) acc = fabs(num) < 1e-10 ? 0.0 : 1.0;
         else acc = fmin(1.0, rr * rr * exp(pt) * num / den);
       }
-      double u = uniform(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_ACC1 : TAG_ACC2) | (A.sweep << 8));
-      if (acc + u > 1.0) {
+      if (acc + rq[3] > 1.0) {
         accepted = true;
         R = t0;
         for (int d = 0; d < 3; d++) r1[d] = rn[d];
@@ -339,7 +396,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
     }
     if (accepted) {
       if (lane < NS) phib[lane] = lane < n ? phi[0] : 0.f;
-      __syncthreads();
+      walker_sync<WPB>();
       if (lane < n) {
         // four independent partial sums: the dot product is not one serial FMA chain
         float tq[4] = {0, 0, 0, 0};
@@ -357,7 +414,12 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
         }
         for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + lane] = phi[c];
       }
-      logdet += log(fabs((double)R));
+      // accepted ratios multiply into dprod, folded into log|det| before it could overflow
+      dprod *= (double)R;
+      if (!(fabs(dprod) < 0x1p+512 && fabs(dprod) > 0x1p-512)) {
+        logdet += log(fabs(dprod));
+        dprod = 1.0;
+      }
       if (R < 0) sgn = -sgn;
       nacc++;
       if (lane < 3) pos[3 * e + lane] = r1[lane];
@@ -368,7 +430,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
       m2 = mog[(e1 * 5 + 2) * nsp + lane];
       m3 = mog[(e1 * 5 + 3) * nsp + lane];
     }
-    __syncthreads();
+    walker_sync<WPB>();
   }
   float* igw = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
   if (lane < n) {
@@ -378,6 +440,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep_mx(DevSys S, MixedTables M, Dev
   }
   if (A.dets) write_det_parts<NS>(S, W, w, s, inv, (const float*)mog, lane);
   for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
+  logdet += log(fabs(dprod));
   if (lane == 0) {
     W.logdet[2 * w + s] = logdet;
     W.sgn[2 * w + s] = sgn;

@@ -63,6 +63,12 @@ void mixed_setup(qw_engine& e) {
     }
   }
   M.ready = 1;
+  // the LDS-staged sweep variant is a tuning candidate where its LDS fits one workgroup
+  int maxlds = 0;
+  if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, e.device) != hipSuccess) maxlds = 0;
+  const size_t need = sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, LCAO_FP32, M.ldchi, 2, e.S.nb,
+                                         e.S.nsp, kWpbCL);
+  e.cl_ok = e.precision == QW_PREC_MIXED_FP32 && !e.big && need <= std::min<size_t>(kLdsMax, (size_t)maxlds);
 }
 
 namespace {
@@ -70,8 +76,8 @@ namespace {
 size_t lds_mx_rows(const qw_engine& e) {
   return (5 * (size_t)e.MT.ldchi + 5 * 64) * sizeof(float) + (size_t)e.MT.ldchi * sizeof(int);
 }
-size_t lds_mx_sweep(const qw_engine& e, int mode) {
-  return sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, mode, e.MT.ldchi);
+size_t lds_mx_sweep(const qw_engine& e, int mode, int depth) {
+  return sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, mode, e.MT.ldchi, depth);
 }
 
 // FP64 recompute of the FP32 inverses of spin `only_spin` (or both when -1) from the current
@@ -86,31 +92,41 @@ void mixed_rows_t(qw_engine& e) {
   });
 }
 
-template <int DEPTH, int MODE, int U>
+// LDS of the variant with the coefficients staged in LDS (kWpbCL walkers per block).
+size_t lds_mx_cl(const qw_engine& e, int depth) {
+  return sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, LCAO_FP32, e.MT.ldchi, depth, e.S.nb, e.S.nsp,
+                            kWpbCL);
+}
+
+template <int DEPTH, int MODE, int U, int WPB>
 void mixed_sweep_u(qw_engine& e, const SweepArgs& A) {
+  const size_t lds = WPB > 1 ? lds_mx_cl(e, DEPTH) : lds_mx_sweep(e, MODE, DEPTH);
+  const unsigned grid = (unsigned)((e.nw + WPB - 1) / WPB);
   auto go = [&](auto kern) {
-    e.tuned_launch([&] {
-      hipLaunchKernelGGL(kern, dim3(e.nw), dim3(64), lds_mx_sweep(e, MODE), e.stream, e.S, e.MT, e.W, A);
-    });
+    e.tuned_launch([&] { hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * WPB), lds, e.stream, e.S, e.MT, e.W, A); });
   };
   switch (e.ns) {
-    case 4: go(k_sweep_mx<4, DEPTH, MODE, U>); break;
-    case 8: go(k_sweep_mx<8, DEPTH, MODE, U>); break;
-    case 16: go(k_sweep_mx<16, DEPTH, MODE, U>); break;
-    case 32: go(k_sweep_mx<32, DEPTH, MODE, U>); break;
-    default: go(k_sweep_mx<64, DEPTH, MODE, U>); break;
+    case 4: go(k_sweep_mx<4, DEPTH, MODE, U, WPB>); break;
+    case 8: go(k_sweep_mx<8, DEPTH, MODE, U, WPB>); break;
+    case 16: go(k_sweep_mx<16, DEPTH, MODE, U, WPB>); break;
+    case 32: go(k_sweep_mx<32, DEPTH, MODE, U, WPB>); break;
+    default: go(k_sweep_mx<64, DEPTH, MODE, U, WPB>); break;
   }
 }
 template <int DEPTH, int MODE>
 void mixed_sweep_t(qw_engine& e, const SweepArgs& A) {
-  if (MODE != LCAO_FP32) {  // the MFMA modes read the dense coefficient fragments (no U)
-    mixed_sweep_u<DEPTH, MODE, 8>(e, A);
-    return;
-  }
-  switch (e.pick_u()) {
-    case 4: mixed_sweep_u<DEPTH, MODE, 4>(e, A); break;
-    case 16: mixed_sweep_u<DEPTH, MODE, 16>(e, A); break;
-    default: mixed_sweep_u<DEPTH, MODE, 8>(e, A); break;
+  if constexpr (MODE != LCAO_FP32) {  // the MFMA modes read the dense coefficient fragments (no U)
+    mixed_sweep_u<DEPTH, MODE, 8, 1>(e, A);
+  } else {
+    // launch choice: 100 * (coefficients staged in LDS) + U
+    switch (e.pick_u()) {
+      case 4: mixed_sweep_u<DEPTH, MODE, 4, 1>(e, A); break;
+      case 16: mixed_sweep_u<DEPTH, MODE, 16, 1>(e, A); break;
+      case 104: mixed_sweep_u<DEPTH, MODE, 4, kWpbCL>(e, A); break;
+      case 108: mixed_sweep_u<DEPTH, MODE, 8, kWpbCL>(e, A); break;
+      case 116: mixed_sweep_u<DEPTH, MODE, 16, kWpbCL>(e, A); break;
+      default: mixed_sweep_u<DEPTH, MODE, 8, 1>(e, A); break;
+    }
   }
 }
 }  // namespace

@@ -40,8 +40,8 @@ __device__ inline char* sys_lds_carve(const DevSys& S, char* base, SysLds& V) {
   return base + ((sys_lds_bytes(S.na, S.nsh, S.ntab) + 15) & ~(size_t)15);
 }
 
-__device__ inline void stage_sys(const DevSys& S, const SysLds& V, int lane) {
-  for (int a = lane; a < S.na; a += kWave) {
+__device__ inline void stage_sys(const DevSys& S, const SysLds& V, int lane, int stride = kWave) {
+  for (int a = lane; a < S.na; a += stride) {
     for (int d = 0; d < 3; d++) {
       V.apos[4 * a + d] = S.atom_pos[3 * a + d];
       V.epos[4 * a + d] = S.eit_pos[3 * a + d];
@@ -49,8 +49,8 @@ __device__ inline void stage_sys(const DevSys& S, const SysLds& V, int lane) {
     V.apos[4 * a + 3] = 0.0;
     V.epos[4 * a + 3] = 0.0;
   }
-  for (int i = lane; i < S.nsh; i += kWave) V.sh[i] = S.shl[i];
-  for (int i = lane; i < 2 * S.ntab; i += kWave) V.tabs[i] = S.tabs[i];
+  for (int i = lane; i < S.nsh; i += stride) V.sh[i] = S.shl[i];
+  for (int i = lane; i < 2 * S.ntab; i += stride) V.tabs[i] = S.tabs[i];
 }
 
 // Pass 1: active shells of point p, compacted into V.actl; returns their count.
@@ -152,19 +152,20 @@ __device__ inline void lcao_batch(const T* __restrict__ C, const int* roff, cons
 
 // Value, gradient and laplacian of the n MOs of spin s at p (lane j < n holds MO j):
 // pass 1, then batches of pass 2 + contraction.  Barriers separate the LDS producers and
-// consumers (one wave per block).
-template <typename T, typename TS, int U>
+// consumers of the walker (WPB walkers per block, one per wave).  C: coefficients in global
+// memory or staged in LDS.
+template <typename T, typename TS, int U, int WPB = 1>
 __device__ inline void mo_lap_point(const DevSys& S, const SysLds& V, const TS* spline, const T* C, int n,
                                     const double p[3], T* chi, int* roff, int cap, T phi[5], int lane) {
   for (int c = 0; c < 5; c++) phi[c] = (T)0;
   const int nact = basis_active(S, V, p, lane);
-  __syncthreads();
+  walker_sync<WPB>();
   for (int i0 = 0; i0 < nact;) {
     int nfun;
     const int take = basis_batch_lap<T, TS>(S, V, spline, p, i0, nact, n, chi, cap, roff, cap, &nfun, lane);
-    __syncthreads();
+    walker_sync<WPB>();
     if (lane < n) lcao_batch<T, U>(C, roff, chi, cap, nfun, phi, lane);
-    __syncthreads();
+    walker_sync<WPB>();
     i0 += take;
   }
 }

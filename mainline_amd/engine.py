@@ -72,6 +72,7 @@ def lib():
         L.qw_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp,
                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.qw_engine_walker_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]
+        L.qw_engine_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.qw_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(ctypes.c_void_p)]
         L.qw_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(JobDesc),
                                        ctypes.POINTER(ctypes.c_longlong)]
@@ -89,7 +90,7 @@ def exported_symbols():
     names = []
     for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
               "qw_method_read", "qw_walkers_set", "qw_walkers_get", "qw_walkers_random", "qw_run",
-              "qw_kernel_times", "qw_engine_walker_bytes", "qw_server_create", "qw_server_submit",
+              "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_server_create", "qw_server_submit",
               "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
         if hasattr(L, n):
             names.append(n)
@@ -176,6 +177,13 @@ class Engine:
         self._check(lib().qw_run(self.h, ctypes.byref(m), _p(eb), _p(stats)))
         return dict(eblock=eb[:m.nblock], acceptance=stats[0], moves=stats[1], seconds=stats[2],
                     moves_per_s=stats[3])
+
+    def tuning(self):
+        """Launch choices: sweep (U, plus 100 when the coefficients are staged in LDS; 0 while
+        the tuner measures), HIP graphs (0/1), block-per-walker path (0/1), inverse tile width."""
+        v = (ctypes.c_int * 4)()
+        self._check(lib().qw_engine_tuning(self.h, v))
+        return dict(sweep=v[0], graphs=v[1], big=v[2], ns=v[3])
 
     def kernel_times(self):
         buf = ctypes.create_string_buffer(4096)

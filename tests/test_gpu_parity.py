@@ -132,3 +132,54 @@ def test_graph_replayed_dmc_equals_direct_launches(prec, monkeypatch):
     _say("graph vs direct", out["0"][0][:, 0], out["1"][0][:, 0])
     for a, b in zip(out["0"], out["1"]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec,name,choices", [
+    ("mixed_fp32", "ae32", ["4", "8", "16"]),
+    ("mixed_fp32", "pp16", ["8", "104", "108", "116"]),  # the LDS layout fits this system
+    ("fp64", "ae32", ["2", "4", "8"]),
+])
+def test_sweep_launch_choices_give_identical_results(prec, name, choices, monkeypatch):
+    """Every sweep launch the tuner may pick (coefficient loads in flight; coefficients read
+    from L2 or staged in LDS for eight walkers per block) gives bit-identical VMC and DMC."""
+    deck, mol = synth_deck(name)
+    nw = 40  # not a multiple of the walkers per block
+    pos = synth.random_configs(mol, nw, seed=29)
+    out = {}
+    for c in choices:
+        monkeypatch.setenv("QW_SWEEP", c)
+        eng = E.Engine(deck, prec, seed=777)
+        assert eng.tuning()["sweep"] == int(c)
+        eng.set_walkers(pos)
+        rv = eng.run(E.method("VMC", nw, nblock=1, nstep=2, timestep=0.3))
+        rd = eng.run(E.method("DMC", nw, nblock=2, nstep=4, timestep=0.01))
+        p, w = eng.get_walkers(nw)
+        out[c] = (rv["eblock"], rd["eblock"], p, w)
+        eng.close()
+        _say("sweep", prec, name, c, out[c][1][:, 0])
+    ref = out[choices[0]]
+    for c in choices[1:]:
+        for a, b in zip(ref, out[c]):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_fp32"])
+def test_block_and_row_inversions_are_bitwise_equal(prec, monkeypatch):
+    """The 8x8-block inversion (default for 33..64 electrons per spin) and the row-per-lane
+    inversion give the same log|psi|, inverse-derived gradients and DMC trajectory bit for bit."""
+    deck, mol = synth_deck("ae32")
+    nw = 24
+    pos = synth.random_configs(mol, nw, seed=31)
+    out = {}
+    for mode in ("rows", "blocks"):
+        monkeypatch.setenv("QW_INV", mode)
+        eng = E.Engine(deck, prec, seed=4242)
+        r = eng.eval(pos)
+        eng.set_walkers(pos)
+        rd = eng.run(E.method("DMC", nw, nblock=1, nstep=3, timestep=0.01))
+        p, w = eng.get_walkers(nw)
+        out[mode] = (r["logpsi"], r["sign"], r["grad"], r["lap"], r["eparts"], rd["eblock"], p, w)
+        eng.close()
+    _say("inversions", prec, out["rows"][0][:3], out["blocks"][0][:3])
+    for a, b in zip(out["rows"], out["blocks"]):
+        assert np.array_equal(a, b)
