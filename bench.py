@@ -53,13 +53,20 @@ def run_gpu(deck, mol, precision, nw, steps, warmup, tau):
     log("[%s] warmup %d steps" % (precision, warmup))
     eng.run(E.method("DMC", nw, nblock=1, nstep=max(1, warmup), timestep=tau))
     log("[%s] timed %d steps" % (precision, steps))
+    kt0 = eng.kernel_times()
     t0 = time.perf_counter()
     res = eng.run(E.method("DMC", nw, nblock=1, nstep=steps, timestep=tau))
     dt = time.perf_counter() - t0
-    kt = eng.kernel_times()
+    kt1 = eng.kernel_times()
+    # per-kernel average over the timed steps only (event times on the engine's stream)
+    kt = {}
+    for k, (a1, c1) in kt1.items():
+        a0, c0 = kt0.get(k, (0.0, 0))
+        if c1 > c0:
+            kt[k] = [(a1 * c1 - a0 * c0) / (c1 - c0), int(c1 - c0)]
     moves = nw * mol.nelec * steps
     return dict(moves_per_s=moves / dt, seconds=dt, ms_per_step=1e3 * dt / steps, energy=float(res["eblock"][0, 0]),
-                acceptance=float(res["acceptance"]), kernels={k: [v[0], int(v[1])] for k, v in kt.items()})
+                acceptance=float(res["acceptance"]), kernels=kt, tuning=eng.tuning())
 
 
 def accuracy_leg(deck, mol, precs, nw, nblock, nstep, tau, nequil):
@@ -137,6 +144,20 @@ def serving_leg(precision, njobs, rate, seed, tmp):
     return res
 
 
+def profile_traffic(precision, walkers):
+    """HBM bytes per launch of the sweep kernel (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel's reading) from the counter runs summarised in profiles/traffic.tsv, when a run of
+    this precision and walker count is recorded there; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic.tsv")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.rstrip("\n").split("\t")
+        if len(f) >= 5 and not line.startswith("#") and f[1] == precision and int(f[2]) == walkers:
+            return float(f[3]) + float(f[4])
+    return None
+
+
 def copy_probe(nbytes):
     """Three device-to-device copies of nbytes through the HIP runtime (its copy kernel):
     the reference reading for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -157,8 +178,48 @@ def copy_probe(nbytes):
     log("copied %d bytes x3" % nbytes)
 
 
+REF_QWALK = os.path.join(ROOT, "oracle", "_ref", "qwalk")
+
+
+def cpu_baseline_reference(deck, mol, seconds_target):
+    """The reference program itself (built from its sources by oracle/Makefile.ref, -O3
+    -ffast-math as its own Linux.mk), one serial process per host core as the stand-in for MPI
+    ranks, each running DMC of the same workload with 8 walkers and its own seed.  Two rounds
+    that differ only in the step count cancel the set-up time (deck reading, spline fits,
+    initial walkers): rate = extra moves / extra wall time."""
+    import subprocess
+    procs = min(16, os.cpu_count() or 1)
+    work = os.path.dirname(deck)
+    text = open(deck).read().split("\n", 1)  # INCLUDE <sys>, then the trial function
+    def round_(nstep, tag):
+        paths = []
+        for i in range(procs):
+            p = os.path.join(work, "ref_%s_%d.in" % (tag, i))
+            with open(p, "w") as f:
+                f.write("METHOD {\n  DMC\n  NCONFIG 8\n  NBLOCK 1\n  NSTEP %d\n  TIMESTEP 0.01\n}\n" % nstep)
+                f.write("RANDOMSEED { %d %d }\n" % (1000 + i, 77 + i))
+                f.write("\n".join(text))
+            paths.append(p)
+        t0 = time.perf_counter()
+        ps = [subprocess.Popen([REF_QWALK, os.path.basename(p)], cwd=work, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL) for p in paths]
+        codes = [q.wait() for q in ps]
+        if any(codes):
+            raise RuntimeError("reference qwalk exited with %s" % codes)
+        return time.perf_counter() - t0
+    t_short = round_(4, "a")
+    per_step = max(1e-3, (round_(24, "b") - t_short) / 20)
+    n_long = 4 + max(20, int(seconds_target / per_step))
+    t_long = round_(n_long, "c")
+    moves = procs * 8 * mol.nelec * (n_long - 4)
+    return dict(value=moves / (t_long - t_short), unit="electron-moves/s", cores=procs, kind="reference",
+                sample="reference qwalk DMC, %d processes x 8 walkers x %d steps of the 128-electron workload "
+                       "minus a %d-step run (%.1f s - %.1f s)" % (procs, n_long, 4, t_long, t_short))
+
+
 def cpu_baseline(deck, mol, seconds_target):
-    """The oracle's FP64 CPU engine (same algorithm, threads over walkers) on a bounded sample."""
+    """The oracle's FP64 CPU engine (same algorithm, threads over walkers) on a bounded sample;
+    used where the reference program is not built."""
     from oracle import oracle as O
     from mainline_amd import synth
     threads = min(16, os.cpu_count() or 1)
@@ -230,14 +291,19 @@ def main():
     f_em = flops_per_move(nb, ns, ne, na, 5 * na, 5, 4, p_acc)
     dt = "f64" if head == "fp64" else "f32"
     achieved = f_em * moves_per_launch / (avg_ms * 1e-3)
-    roof = {"bound": "fp64" if head == "fp64" else "fp32", "achieved": achieved / 1e12,
-            "peak": PEAK[dt] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[dt], "traffic": None,
-            "kernel": kname, "avg_ms": avg_ms, "flops_per_move": f_em, "moves_per_launch": moves_per_launch}
+    # The dense-LCAO algorithmic flops are priced against the matrix-core peak of the arithmetic
+    # type (the sweep itself runs them on the vector ALUs, sparse over the active functions).
+    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[dt] / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK[dt], "traffic": profile_traffic(head, a.walkers),
+            "kernel": kname, "avg_ms": avg_ms, "flops_per_move": f_em, "moves_per_launch": moves_per_launch,
+            "launch_choice": R.get("tuning", {}).get("sweep")}
     serving = {}
     if a.serving_jobs > 0:
         for p in dict.fromkeys(["fp64", head]):
             serving[p] = serving_leg(p, a.serving_jobs, 20.0, 4242, tmp)
-    cpu = None if a.no_cpu else cpu_baseline(deck, mol, a.cpu_seconds)
+    cpu = None
+    if not a.no_cpu:
+        cpu = (cpu_baseline_reference if os.access(REF_QWALK, os.X_OK) else cpu_baseline)(deck, mol, a.cpu_seconds)
     line = {
         "metric": "electron-moves/s, 128e DMC, mixed vs FP64 at equal E_L; p50/p99 job latency",
         "value": R["moves_per_s"], "unit": "electron-moves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

@@ -1140,6 +1140,13 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
 extern "C" qw_status qw_kernel_times(const qw_engine* e, char* buf, int buflen, double* avg_ms, int* count, int n,
                                      int* nk) {
   if (!e) return QW_ERR_ARG;
+  // fold in the launches still pending on the stream (reading the totals does not change
+  // what the engine computes)
+  try {
+    const_cast<qw_engine*>(e)->drain_events();
+  } catch (std::exception&) {
+    return QW_ERR_DEVICE;
+  }
   std::string names;
   int k = (int)e->knames.size();
   for (int i = 0; i < k; i++) {

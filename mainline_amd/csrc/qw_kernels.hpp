@@ -739,23 +739,31 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
       lD[e] = d[3];
     }
   }
+  // otherwise from the inverse and the MO cache: electron by electron, lanes over the MOs, so
+  // every row is read coalesced
   for (int s = 0; s < 2 && !A.use_dgl; s++) {
     const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0;
     const TM* ig = inv_store<TM>(W) + ((size_t)w * 2 + s) * nsp * nsp;
-    for (int k = lane; k < n; k += kWave) {
+    for (int k = 0; k < n; k++) {
       const int e = off + k;
       double t1 = 0, t2 = 0, t3 = 0, t4 = 0;
-      for (int j = 0; j < n; j++) {
-        double iv = ig[(size_t)k * nsp + j];
+      for (int j = lane; j < n; j += kWave) {
+        const double iv = ig[(size_t)k * nsp + j];
         t1 += mog[((size_t)e * 5 + 1) * nsp + j] * iv;
         t2 += mog[((size_t)e * 5 + 2) * nsp + j] * iv;
         t3 += mog[((size_t)e * 5 + 3) * nsp + j] * iv;
         t4 += mog[((size_t)e * 5 + 4) * nsp + j] * iv;
       }
-      gD[3 * e] = t1;
-      gD[3 * e + 1] = t2;
-      gD[3 * e + 2] = t3;
-      lD[e] = t4;
+      t1 = wsum(t1);
+      t2 = wsum(t2);
+      t3 = wsum(t3);
+      t4 = wsum(t4);
+      if (lane == 0) {
+        gD[3 * e] = t1;
+        gD[3 * e + 1] = t2;
+        gD[3 * e + 2] = t3;
+        lD[e] = t4;
+      }
     }
   }
   __syncthreads();

@@ -605,6 +605,7 @@ SysTables read_deck(const std::string& path, bool* unsup) {
 
     // ---- Jastrow
     T.atom_ei0.assign(T.na + 1, 0);
+    T.atom_label = labels;
     std::vector<std::vector<JObj>> ei_atom(T.na);
     if (!wf2.empty()) {
       if (!keyword(wf2, "JASTROW2")) unsupported("WF2 must be JASTROW2");
@@ -623,6 +624,7 @@ SysTables read_deck(const std::string& path, bool* unsup) {
         if (section0(g, ob, "ONEBODY")) {
           double scale = 1;
           dvalue(ob, scale, "SCALE");
+          const bool frozen = keyword(ob, "FREEZE");
           std::map<std::string, std::vector<double>> cf;
           size_t p3 = 0;
           Words cs;
@@ -631,6 +633,21 @@ SysTables read_deck(const std::string& path, bool* unsup) {
             std::vector<double> c;
             for (size_t i = 1; i < cs.size(); i++) c.push_back(std::atof(cs[i].c_str()) * scale);
             cf[cs[0]] = c;
+            // parameters of this section: coefficient k multiplies the k-th function of the
+            // atom type's electron-ion basis objects, concatenated in input order
+            for (size_t k = 0; k < c.size() && !frozen; k++) {
+              size_t kk = k;
+              for (size_t q = 0; q < eib.size(); q++) {
+                if (eib_label[q] != cs[0]) continue;
+                if (kk < (size_t)eib[q].nf) {
+                  JParm jp{1, 0, cs[0], c[k], eib[q]};
+                  for (int i = 0; i < kMaxJF; i++) jp.unit.cl[i] = jp.unit.cu[i] = i == (int)kk ? 1.0 : 0.0;
+                  T.jparms.push_back(jp);
+                  break;
+                }
+                kk -= eib[q].nf;
+              }
+            }
           }
           for (int a = 0; a < T.na; a++) {
             auto it = cf.find(labels[a]);
@@ -668,6 +685,26 @@ SysTables read_deck(const std::string& path, bool* unsup) {
           int nfe = 0;
           for (auto& o : eeb) nfe += o.nf;
           if ((int)cl.size() > nfe) bad("not enough EE basis functions for the two-body coefficients");
+          if (!keyword(tb, "FREEZE")) {
+            // TWOBODY: one parameter per coefficient; TWOBODY_SPIN: the like ones, then unlike
+            for (int sp = 0; sp < (twos ? 2 : 1); sp++)
+              for (size_t c = 0; c < cl.size(); c++) {
+                size_t kk = c;
+                for (auto& o : eeb) {
+                  if (kk < (size_t)o.nf) {
+                    JParm jp{2, twos ? 1 + sp : 0, std::string(), sp ? cu[c] : cl[c], o};
+                    for (int i = 0; i < kMaxJF; i++) {
+                      const double one = i == (int)kk ? 1.0 : 0.0;
+                      jp.unit.cl[i] = (!twos || sp == 0) ? one : 0.0;
+                      jp.unit.cu[i] = (!twos || sp == 1) ? one : 0.0;
+                    }
+                    T.jparms.push_back(jp);
+                    break;
+                  }
+                  kk -= o.nf;
+                }
+              }
+          }
           for (auto o : eeb) {
             for (int i = 0; i < o.nf; i++, k++) {
               o.cl[i] = k < (int)cl.size() ? cl[k] : 0.0;

@@ -50,6 +50,19 @@ struct PPType {
   double cutoff;
 };
 
+// A linear Jastrow parameter, in the reference's order: group by group, the one-body
+// COEFFICIENTS sections in input order (each coefficient of its atom type), then the two-body
+// coefficients (TWOBODY in order; TWOBODY_SPIN like ones then unlike ones).  Pieces marked
+// FREEZE have none (src/wavefunction/Jastrow2_wf.cpp:477-520, Jastrow2_one.cpp:25-100,
+// Jastrow2_two.h:96,148).
+struct JParm {
+  int body;           // 1: electron-ion (one-body), 2: electron-electron
+  int spin;           // two-body pairs it applies to: 0 all, 1 like spins, 2 unlike spins
+  std::string label;  // one-body: the atom label
+  double value;       // the coefficient (after SCALE)
+  JObj unit;          // the basis object with only this coefficient, set to 1
+};
+
 struct SysTables {
   int nup = 0, ndn = 0, na = 0, nb = 0;
   std::vector<double> atom_pos;   // [na][3]
@@ -63,6 +76,8 @@ struct SysTables {
   std::vector<JObj> ee;           // electron-electron objects (merged over groups)
   std::vector<JObj> ei;           // electron-ion objects, atom ranges in atom_ei0
   std::vector<int> atom_ei0;      // [na+1]
+  std::vector<JParm> jparms;      // linear Jastrow parameters (getParmDeriv order)
+  std::vector<std::string> atom_label;  // [na]
   std::vector<PPType> pptypes;
   std::vector<int> atom_pp;       // [na], -1 = no pseudopotential
   double vii = 0;                 // ion-ion Coulomb energy

@@ -166,14 +166,28 @@ def parse_golden(text):
         if not f:
             continue
         if f[0] == "config":
-            cur = dict(elec=[])
+            cur = dict(elec=[], parm=[], pdgrad=[], pdgd=[], pdvg=[])
             out.append(cur)
         elif f[0] == "elec":
             cur["elec"].append([float(x) for x in f[2:6]])
+        elif f[0] in ("parm", "pdgrad"):
+            cur[f[0]].append(float(f[2]))
+        elif f[0] == "pdgd":
+            cur["pdgd"].append([float(x) for x in f[3:7]])
+        elif f[0] == "pdvg":
+            cur["pdvg"].append([float(x) for x in f[2:5]])
+        elif f[0] in ("pdok", "nparm"):
+            cur[f[0]] = int(f[1])
         elif f[0] == "sign":
             cur["sign"] = int(f[1])
         else:
             cur[f[0]] = float(f[1])
     for c in out:
         c["elec"] = np.array(c["elec"])
+        npar = c.get("nparm", 0)
+        ne = len(c["elec"])
+        c["parm"] = np.array(c["parm"])
+        c["pdgrad"] = np.array(c["pdgrad"])
+        c["pdgd"] = np.array(c["pdgd"]).reshape(npar, ne, 4)
+        c["pdvg"] = np.array(c["pdvg"]).reshape(-1, 3)
     return out

@@ -5,7 +5,9 @@
 // configuration it builds a fresh Wavefunction and Sample_point (so no incremental state
 // carries over) and prints, with %.17g,
 //   ln|Psi|, sign, per electron grad(Psi)/Psi (3) and lap(Psi)/Psi,
-//   kinetic energy, local (Coulomb) potential, nonlocal pseudopotential energy.
+//   kinetic energy, local (Coulomb) potential, nonlocal pseudopotential energy,
+//   and the linear-parameter derivatives of Wavefunction::getParmDeriv (gradient, gradderiv,
+//   val_gradient) with the parameter values of Wavefunction_data::getVarParms.
 // The pseudopotential is made deterministic with the identity quadrature rotation, as the
 // reference allows (src/system/Pseudopotential.h:80 setDeterministic,
 // src/system/Pseudopotential.cpp:828 rotateQuadrature).
@@ -120,6 +122,23 @@ int main(int argc, char** argv) {
     vnl = 0.0;
     psp->calcNonlocWithTest(wfdata, sys, sample, wf, test, vnl);
     print_config(k, ne, val, lap, kin(0), vloc, vnl(0));
+    // linear-parameter derivatives (Wavefunction::getParmDeriv) and the parameter values
+    Parm_deriv_return pd;
+    pd.need_hessian = 0;
+    const int pdok = wf->getParmDeriv(wfdata, sample, pd);
+    Array1<doublevar> parms;
+    wfdata->getVarParms(parms);
+    const int np = pd.gradient.GetDim(0);
+    printf("pdok %d\n", pdok);
+    printf("nparm %d\n", np);
+    for (int q = 0; q < parms.GetDim(0); q++) printf("parm %d %.17g\n", q, parms(q));
+    for (int q = 0; q < np; q++) printf("pdgrad %d %.17g\n", q, pd.gradient(q));
+    for (int q = 0; q < np; q++)
+      for (int e = 0; e < ne; e++)
+        printf("pdgd %d %d %.17g %.17g %.17g %.17g\n", q, e, pd.gradderiv(q, e, 0), pd.gradderiv(q, e, 1),
+               pd.gradderiv(q, e, 2), pd.gradderiv(q, e, 3));
+    for (int e = 0; e < ne && pd.val_gradient.GetDim(0) == ne; e++)
+      printf("pdvg %d %.17g %.17g %.17g\n", e, pd.val_gradient(e, 0), pd.val_gradient(e, 1), pd.val_gradient(e, 2));
     delete wf;
     delete sample;
   }

@@ -7,7 +7,9 @@ configurations through the reference's own Wavefunction / System / Pseudopotenti
 as an .npz fixture:
 
   pos[K][Ne][3], logpsi[K], sign[K], grad[K][Ne][3], lap[K][Ne], kinetic[K], vloc[K],
-  vnonloc[K]  (+ orb_sha256 for generated decks, to detect generator drift)
+  vnonloc[K], and from Wavefunction::getParmDeriv: parms[np] (getVarParms),
+  pd_gradient[K][np], pd_gradderiv[K][np][Ne][4], pd_valgrad[K][Ne][3]
+  (+ orb_sha256 for generated decks, to detect generator drift)
 
 Cases:
   h_atom    the reference's tests/h deck (H atom, AOSPLINE S shells, CUTOFF_MO)
@@ -83,6 +85,10 @@ def run_ref(deck_dir, deck, pos):
         kinetic=np.array([c["kinetic"] for c in g]),
         vloc=np.array([c["vloc"] for c in g]),
         vnonloc=np.array([c["vnonloc"] for c in g]),
+        parms=g[0]["parm"],
+        pd_gradient=np.array([c["pdgrad"] for c in g]),
+        pd_gradderiv=np.array([c["pdgd"] for c in g]),
+        pd_valgrad=np.array([c["pdvg"] for c in g]),
     ) if K else None
 
 
