@@ -20,6 +20,7 @@
 #include "qw_big.hpp"
 #include "qw_kernels.hpp"
 #include "qw_mixed.hpp"
+#include "qw_parm.hpp"
 #include "qw_setup.hpp"
 
 using namespace qw;
@@ -264,6 +265,26 @@ struct qw_engine {
       std::vector<float> cf(T.C[s].begin(), T.C[s].end());
       S.Cf[s] = upload(cf);
     }
+    // linear Jastrow parameters: one unit-coefficient object each, atom lists for one-body ones
+    {
+      std::vector<JParmDev> pd;
+      std::vector<JObj> units;
+      std::vector<int> patoms;
+      for (const JParm& jp : T.jparms) {
+        JParmDev d{jp.body, jp.spin, (int)patoms.size(), 0, jp.value};
+        if (jp.body == 1)
+          for (int a = 0; a < T.na; a++)
+            if (T.atom_label[a] == jp.label) patoms.push_back(a);
+        d.a1 = (int)patoms.size();
+        pd.push_back(d);
+        units.push_back(jp.unit);
+        parm_values.push_back(jp.value);
+      }
+      npar = (int)pd.size();
+      d_parm = upload(pd);
+      d_punit = upload(jrad_table<double>(units));
+      d_patoms = upload(patoms);
+    }
     S.ee = upload(jrad_table<double>(T.ee));
     S.eef = upload(jrad_table<float>(T.ee));
     {
@@ -431,6 +452,12 @@ struct qw_engine {
     }
   }
   bool inv_rows = false;  // 33..64 per spin: row-per-lane inversion instead of 8x8 blocks (QW_INV=rows)
+  // linear Jastrow parameters (getParmDeriv)
+  int npar = 0;
+  std::vector<double> parm_values;
+  JParmDev* d_parm = nullptr;
+  JRad<double>* d_punit = nullptr;
+  int* d_patoms = nullptr;
   void refresh_all() {
     if (big) {
       big_refresh_all();
@@ -903,6 +930,72 @@ extern "C" qw_status qw_walkers_get(qw_engine* e, int n, double* pos, double* we
     if (pos) HIPCHK(hipMemcpyAsync(pos, e->W.pos, (size_t)n * ne * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     if (weights) HIPCHK(hipMemcpyAsync(weights, e->W.weight, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    return QW_OK;
+  });
+}
+
+extern "C" qw_status qw_parm_count(const qw_engine* e, int* np, double* values) {
+  if (!e || !np) return QW_ERR_ARG;
+  *np = e->npar;
+  if (values)
+    for (int p = 0; p < e->npar; p++) values[p] = e->parm_values[p];
+  return QW_OK;
+}
+
+extern "C" qw_status qw_parm_deriv(qw_engine* e, int K, const double* pos, double* gradient, double* hessian,
+                                   double* gradderiv, double* val_gradient) {
+  if (!e || K <= 0 || !pos || !gradient || !gradderiv || !val_gradient) return QW_ERR_ARG;
+  return guarded(e, [&] {
+    e->ensure_walkers(K);
+    e->dmc_ready = false;
+    const size_t ne = e->S.ne, np = (size_t)e->npar;
+    HIPCHK(hipMemcpyAsync(e->W.pos, pos, (size_t)K * ne * 3 * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    e->refresh_all();
+    std::vector<void*> tmp;
+    auto dalloc = [&](size_t bytes) {
+      void* p = nullptr;
+      {
+        std::lock_guard<std::mutex> g(g_devmem_mu);
+        HIPCHK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+      }
+      tmp.push_back(p);
+      return (double*)p;
+    };
+    struct Free {
+      std::vector<void*>& v;
+      hipStream_t st;
+      ~Free() {
+        (void)hipStreamSynchronize(st);
+        std::lock_guard<std::mutex> g(g_devmem_mu);
+        for (void* p : v) (void)hipFree(p);
+      }
+    } fr{tmp, e->stream};
+    double* d_g = dalloc((size_t)K * np * sizeof(double));
+    double* d_gd = dalloc((size_t)K * np * ne * 4 * sizeof(double));
+    double* d_vg = dalloc((size_t)K * ne * 3 * sizeof(double));
+    const size_t lds = (size_t)9 * ne * sizeof(double);
+    e->launch("k_parm_deriv", [&] {
+      if (e->precision == QW_PREC_FP64)
+        hipLaunchKernelGGL(k_parm_deriv<double>, dim3(K), dim3(64), lds, e->stream, e->S, e->W,
+                           (const JParmDev*)e->d_parm, (const JRad<double>*)e->d_punit, (const int*)e->d_patoms,
+                           (int)np, K, d_g, d_gd, d_vg);
+      else
+        hipLaunchKernelGGL(k_parm_deriv<float>, dim3(K), dim3(64), lds, e->stream, e->S, e->W,
+                           (const JParmDev*)e->d_parm, (const JRad<double>*)e->d_punit, (const int*)e->d_patoms,
+                           (int)np, K, d_g, d_gd, d_vg);
+    });
+    if (np) {
+      HIPCHK(hipMemcpyAsync(gradient, d_g, (size_t)K * np * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipMemcpyAsync(gradderiv, d_gd, (size_t)K * np * ne * 4 * sizeof(double), hipMemcpyDeviceToHost,
+                            e->stream));
+    }
+    HIPCHK(hipMemcpyAsync(val_gradient, d_vg, (size_t)K * ne * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (hessian)
+      for (int k = 0; k < K; k++)
+        for (size_t i = 0; i < np; i++)
+          for (size_t j = 0; j < np; j++)
+            hessian[((size_t)k * np + i) * np + j] = gradient[(size_t)k * np + i] * gradient[(size_t)k * np + j];
     return QW_OK;
   });
 }

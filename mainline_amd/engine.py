@@ -73,6 +73,8 @@ def lib():
                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.qw_engine_walker_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_ulonglong)]
         L.qw_engine_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.qw_parm_count.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _dp]
+        L.qw_parm_deriv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.qw_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(ctypes.c_void_p)]
         L.qw_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(JobDesc),
                                        ctypes.POINTER(ctypes.c_longlong)]
@@ -90,7 +92,8 @@ def exported_symbols():
     names = []
     for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
               "qw_method_read", "qw_walkers_set", "qw_walkers_get", "qw_walkers_random", "qw_run",
-              "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_server_create", "qw_server_submit",
+              "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_parm_count",
+              "qw_parm_deriv", "qw_server_create", "qw_server_submit",
               "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
         if hasattr(L, n):
             names.append(n)
@@ -151,6 +154,30 @@ class Engine:
         pr = None if pp_rand is None else np.ascontiguousarray(pp_rand, dtype=np.float64)
         self._check(lib().qw_eval_fixed(self.h, K, _p(pos), _p(r), pp_mode, _p(pr), _p(out["logpsi"]),
                                         _p(out["sign"]), _p(out["grad"]), _p(out["lap"]), _p(out["eparts"])))
+        return out
+
+    def parms(self):
+        """Values of the linear Jastrow parameters (getParmDeriv order)."""
+        n = ctypes.c_int()
+        self._check(lib().qw_parm_count(self.h, ctypes.byref(n), None))
+        v = np.zeros(max(n.value, 1))
+        self._check(lib().qw_parm_count(self.h, ctypes.byref(n), _p(v)))
+        return v[:n.value]
+
+    def parm_deriv(self, pos):
+        """gradient[K][np], hessian[K][np][np], gradderiv[K][np][Ne][4], val_gradient[K][Ne][3]
+        of Wavefunction::getParmDeriv at fixed configurations."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        K = pos.shape[0]
+        n = len(self.parms())
+        m = max(n, 1)
+        out = dict(gradient=np.zeros((K, m)), hessian=np.zeros((K, m, m)), gradderiv=np.zeros((K, m, self.ne, 4)),
+                   val_gradient=np.zeros((K, self.ne, 3)))
+        self._check(lib().qw_parm_deriv(self.h, K, _p(pos), _p(out["gradient"]), _p(out["hessian"]),
+                                        _p(out["gradderiv"]), _p(out["val_gradient"])))
+        out["gradient"] = out["gradient"][:, :n]
+        out["hessian"] = out["hessian"][:, :n, :n]
+        out["gradderiv"] = out["gradderiv"][:, :n]
         return out
 
     def set_walkers(self, pos, weights=None):

@@ -35,6 +35,9 @@ def lib():
         L.orc_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
                                _dp, _dp, _dp, _dp, _dp]
         L.orc_mo_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.orc_nparm.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_parm_deriv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_set_parms.argtypes = [ctypes.c_void_p, _dp]
         L.orc_basis_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.orc_spline.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.orc_mo_coeff.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
@@ -87,6 +90,28 @@ class Oracle:
         lib().orc_eval(self.h, K, _p(pos), _p(r), pp_mode, _p(pr), _p(out["logpsi"]), _p(out["sign"]),
                        _p(out["grad"]), _p(out["lap"]), _p(out["eparts"]))
         return out
+
+    def parms(self):
+        """Values of the linear Jastrow parameters (getParmDeriv order)."""
+        n = lib().orc_nparm(self.h, None)
+        v = np.zeros(max(n, 1))
+        lib().orc_nparm(self.h, _p(v))
+        return v[:n]
+
+    def set_parms(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        lib().orc_set_parms(self.h, _p(v))
+
+    def parm_deriv(self, pos):
+        """gradient[K][np], gradderiv[K][np][Ne][4], val_gradient[K][Ne][3] of getParmDeriv."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        K = pos.shape[0]
+        n = lib().orc_nparm(self.h, None)
+        g = np.zeros((K, max(n, 1)))
+        gd = np.zeros((K, max(n, 1), self.ne, 4))
+        vg = np.zeros((K, self.ne, 3))
+        lib().orc_parm_deriv(self.h, K, _p(pos), _p(g), _p(gd), _p(vg))
+        return dict(gradient=g[:, :n], gradderiv=gd[:, :n], val_gradient=vg)
 
     def mo(self, spin, pts):
         pts = np.ascontiguousarray(pts, dtype=np.float64)

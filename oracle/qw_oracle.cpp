@@ -568,6 +568,11 @@ struct JGroup {
   std::vector<double> twocoef;                // TWOBODY
   std::vector<double> like, unlike;           // TWOBODY_SPIN
   double eecut = 0;                           // per EE basis cutoff handled per object
+  // parameter bookkeeping (src/wavefunction/Jastrow2_one.cpp:25-100,250-258,
+  // Jastrow2_two.h:96,148): FREEZE flags and the one-body COEFFICIENTS sections in input order
+  bool one_frozen = false, two_frozen = false;
+  std::vector<std::string> sec_label;
+  std::vector<std::vector<double>> sec_coef;
 };
 
 struct Atom {
@@ -907,9 +912,12 @@ void read_jastrow(System& S, const Tok& wf2) {
       size_t p3 = 0;
       Tok cs;
       std::vector<bool> got(nat, false);
+      G.one_frozen = haskeyword(ob, "FREEZE");
       while (readsection(ob, p3, cs, "COEFFICIENTS")) {
         std::vector<double> c;
         for (size_t i = 1; i < cs.size(); i++) c.push_back(atof(cs[i].c_str()) * (has_scale ? scale : 1.0));
+        G.sec_label.push_back(cs[0]);
+        G.sec_coef.push_back(c);
         for (int a = 0; a < nat; a++)
           if (S.atoms[a].label == cs[0]) G.onecoef[a] = c, got[a] = true;
       }
@@ -926,6 +934,7 @@ void read_jastrow(System& S, const Tok& wf2) {
       Tok cs;
       if (!readsection0(tb, cs, "COEFFICIENTS")) fail("TWOBODY needs COEFFICIENTS");
       for (auto& x : cs) G.twocoef.push_back(atof(x.c_str()));
+      G.two_frozen = haskeyword(tb, "FREEZE");
     }
     if (readsection0(g, tb, "TWOBODY_SPIN")) {
       if (G.has_two) fail("only one of TWOBODY or TWOBODY_SPIN");
@@ -937,6 +946,7 @@ void read_jastrow(System& S, const Tok& wf2) {
       if (l1.size() != l2.size()) fail("like/unlike coefficient counts differ");
       for (auto& x : l1) G.like.push_back(atof(x.c_str()));
       for (auto& x : l2) G.unlike.push_back(atof(x.c_str()));
+      G.two_frozen = haskeyword(tb, "FREEZE");
     }
     if (G.has_two) {
       int nb = 0;
@@ -1694,6 +1704,141 @@ extern "C" void orc_info(const orc_system* s, int* out) {
   out[4] = s->S.pp.empty() ? 0 : 1;
 }
 
+// ---------------------------------------------------------------- parameter derivatives
+// Linear Jastrow parameters and d ln Psi / dp at one configuration: Wavefunction::getParmDeriv
+// of SLATER-JASTROW with one determinant (no determinant or orbital parameters) and JASTROW2
+// (src/wavefunction/Slat_Jastrow.cpp:171-193, Slat_wf.h:555-575,680-686,
+// Jastrow2_wf.cpp:1486-1553 and 1900-1945 (create_parm_deriv), Jastrow2_one.cpp:214-248,
+// Jastrow2_two.cpp:153-180,407-440, Wavefunction.cpp:27-113 (extend_parm_deriv)).
+// A piece is a group's one-body or two-body term with parameters (FREEZE: none, and the piece
+// contributes nothing, not even to val_gradient).  For a piece with basis sums func(p,e,0..4)
+// (value, gradient, laplacian; two-body values counted once per pair, gradients
+// antisymmetric, laplacians on both electrons):
+//   gradient(p) = sum_e func(p,e,0);  gd(p,e,0..2) = func(p,e,1..3);
+//   vg_piece(e) = sum_p c_p func(p,e,1..3);  gd(p,e,3) = func(p,e,4) + 2 gd(p,e,0..2).vg_piece(e).
+// Appending a piece adds to each earlier parameter 2 gd.vg_new and to each new one
+// 2 gd.vg_before, and vg accumulates; the Slater determinant (no parameters) finally adds
+// 2 gd(p,e,0..2).grad ln D(e) and its gradient to vg.
+struct PDeriv {
+  std::vector<double> g;   // [np]
+  std::vector<double> gd;  // [np][ne][4]
+  std::vector<double> vg;  // [ne][3]
+  std::vector<double> parms;
+};
+void pd_append(PDeriv& acc, const std::vector<double>& func, const std::vector<double>& coef, int ne) {
+  const int np = (int)coef.size(), np0 = (int)acc.g.size();
+  std::vector<double> vgp((size_t)ne * 3, 0.0);
+  for (int p = 0; p < np; p++)
+    for (int e = 0; e < ne; e++)
+      for (int d = 0; d < 3; d++) vgp[3 * e + d] += coef[p] * func[((size_t)p * ne + e) * 5 + 1 + d];
+  // earlier parameters: cross term with the new piece's gradient
+  for (int p = 0; p < np0; p++)
+    for (int e = 0; e < ne; e++) {
+      double dot = 0;
+      for (int d = 0; d < 3; d++) dot += acc.gd[((size_t)p * ne + e) * 4 + d] * vgp[3 * e + d];
+      acc.gd[((size_t)p * ne + e) * 4 + 3] += 2 * dot;
+    }
+  for (int p = 0; p < np; p++) {
+    double g = 0;
+    for (int e = 0; e < ne; e++) g += func[((size_t)p * ne + e) * 5];
+    acc.g.push_back(g);
+    acc.parms.push_back(coef[p]);
+    for (int e = 0; e < ne; e++) {
+      const double* f = &func[((size_t)p * ne + e) * 5];
+      double own = 0, before = 0;
+      for (int d = 0; d < 3; d++) own += vgp[3 * e + d] * f[1 + d], before += f[1 + d] * acc.vg[3 * e + d];
+      double l = f[4] + 2 * own;
+      l += 2 * before;
+      for (int d = 0; d < 3; d++) acc.gd.push_back(f[1 + d]);
+      acc.gd.push_back(l);
+    }
+  }
+  for (int i = 0; i < 3 * ne; i++) acc.vg[i] += vgp[i];
+}
+
+void parm_deriv(const System& S, Walker& w, PDeriv& out, JScratch& js) {
+  const int ne = S.ne(), nat = (int)S.atoms.size();
+  out.g.clear();
+  out.gd.clear();
+  out.parms.clear();
+  out.vg.assign((size_t)ne * 3, 0.0);
+  std::vector<double> func;
+  for (const JGroup& G : S.groups) {
+    if (G.has_one && !G.one_frozen) {
+      // parameters: COEFFICIENTS sections in order, coefficient k of the atom type's
+      // concatenated EI basis
+      std::vector<double> coef;
+      std::vector<std::pair<int, int>> pk;  // (section, k)
+      for (size_t sc = 0; sc < G.sec_label.size(); sc++)
+        for (size_t k = 0; k < G.sec_coef[sc].size(); k++) coef.push_back(G.sec_coef[sc][k]), pk.push_back({(int)sc, (int)k});
+      func.assign(coef.size() * ne * 5, 0.0);
+      for (size_t p = 0; p < pk.size(); p++)
+        for (int e = 0; e < ne; e++)
+          for (int a = 0; a < nat; a++) {
+            if (S.atoms[a].label != G.sec_label[pk[p].first]) continue;
+            double R[5];
+            dist5(&w.pos[3 * e], S.atoms[a].r, R);
+            int k = 0;
+            for (int bi : G.atom2basis[a]) {
+              const JBasis& jb = G.eib[bi];
+              js.b.resize(5 * jb.nfunc());
+              jb.calc_lap(R, js.b.data());
+              for (int i = 0; i < jb.nfunc(); i++, k++)
+                if (k == pk[p].second)
+                  for (int d = 0; d < 5; d++) func[((size_t)p * ne + e) * 5 + d] += js.b[5 * i + d];
+            }
+          }
+      if (!coef.empty()) pd_append(out, func, coef, ne);
+    }
+    if (G.has_two && !G.two_frozen) {
+      const int nb = G.diffspin ? (int)G.like.size() : (int)G.twocoef.size();
+      std::vector<double> coef = G.diffspin ? G.like : G.twocoef;
+      if (G.diffspin) coef.insert(coef.end(), G.unlike.begin(), G.unlike.end());
+      func.assign(coef.size() * ne * 5, 0.0);
+      for (int i = 0; i < ne; i++)
+        for (int j = i + 1; j < ne; j++) {
+          double R[5];
+          dist5(&w.pos[3 * i], &w.pos[3 * j], R);
+          const int shift = G.diffspin && ((j < S.nup) != (i < S.nup)) ? nb : 0;
+          int k = 0;
+          for (const JBasis& jb : G.eeb) {
+            js.b.resize(5 * jb.nfunc());
+            if (R[0] < jb.cutoff()) jb.calc_lap(R, js.b.data());
+            else std::fill(js.b.begin(), js.b.end(), 0.0);
+            for (int q = 0; q < jb.nfunc(); q++, k++) {
+              if (k >= nb) continue;
+              double* fi = &func[((size_t)(k + shift) * ne + i) * 5];
+              double* fj = &func[((size_t)(k + shift) * ne + j) * 5];
+              fi[0] += js.b[5 * q];
+              for (int d = 1; d < 4; d++) fi[d] += js.b[5 * q + d], fj[d] -= js.b[5 * q + d];
+              fi[4] += js.b[5 * q + 4];
+              fj[4] += js.b[5 * q + 4];
+            }
+          }
+        }
+      if (!coef.empty()) pd_append(out, func, coef, ne);
+    }
+  }
+  // the determinant: no parameters, gradient of ln D per electron
+  std::vector<double> gD((size_t)ne * 3);
+  for (int e = 0; e < ne; e++) {
+    double l;
+    det_gradlap(S, w, e, &gD[3 * e], &l);
+  }
+  const int np = (int)out.g.size();
+  if (np == 0) {
+    out.vg = gD;
+    return;
+  }
+  for (int p = 0; p < np; p++)
+    for (int e = 0; e < ne; e++) {
+      double dot = 0;
+      for (int d = 0; d < 3; d++) dot += out.gd[((size_t)p * ne + e) * 4 + d] * gD[3 * e + d];
+      out.gd[((size_t)p * ne + e) * 4 + 3] += 2 * dot;
+    }
+  for (int i = 0; i < 3 * ne; i++) out.vg[i] += gD[i];
+}
+
 extern "C" int orc_eval(orc_system* o, int K, const double* pos, const double* rot, int pp_mode,
                         const double* pp_rand, double* logpsi, double* sign, double* grad, double* lap,
                         double* eparts) {
@@ -1717,6 +1862,70 @@ extern "C" int orc_eval(orc_system* o, int K, const double* pos, const double* r
                                                    tmp, js);
   }
   return 0;
+}
+
+// Number of linear Jastrow parameters, their values (may be NULL).
+extern "C" int orc_nparm(orc_system* o, double* values) {
+  const System& S = o->S;
+  Walker w;
+  walker_init(S, w);
+  w.pos.assign((size_t)S.ne() * 3, 0.0);
+  for (int e = 0; e < S.ne(); e++) w.pos[3 * e] = 100.0 + 7.0 * e;  // far apart: only the count matters
+  std::vector<double> tmp;
+  walker_refresh(S, w, tmp);
+  PDeriv pd;
+  JScratch js;
+  parm_deriv(S, w, pd, js);
+  if (values)
+    for (size_t p = 0; p < pd.parms.size(); p++) values[p] = pd.parms[p];
+  return (int)pd.g.size();
+}
+
+// Sets the linear Jastrow parameters (same order and count as orc_nparm) — for finite
+// differences in the tests.  Returns the number set.
+extern "C" int orc_set_parms(orc_system* o, const double* values) {
+  System& S = o->S;
+  int q = 0;
+  for (JGroup& G : S.groups) {
+    if (G.has_one && !G.one_frozen)
+      for (size_t sc = 0; sc < G.sec_label.size(); sc++)
+        for (size_t k = 0; k < G.sec_coef[sc].size(); k++) {
+          G.sec_coef[sc][k] = values[q++];
+          for (size_t a = 0; a < S.atoms.size(); a++)
+            if (S.atoms[a].label == G.sec_label[sc]) G.onecoef[a][k] = G.sec_coef[sc][k];
+        }
+    if (G.has_two && !G.two_frozen) {
+      if (G.diffspin) {
+        for (auto& c : G.like) c = values[q++];
+        for (auto& c : G.unlike) c = values[q++];
+      } else {
+        for (auto& c : G.twocoef) c = values[q++];
+      }
+    }
+  }
+  return q;
+}
+
+// Parameter derivatives at K configurations: grad[K][np], gd[K][np][ne][4], vg[K][ne][3].
+extern "C" int orc_parm_deriv(orc_system* o, int K, const double* pos, double* grad, double* gd, double* vg) {
+  const System& S = o->S;
+  const int ne = S.ne();
+  std::vector<double> tmp;
+  JScratch js;
+  int np = -1;
+  for (int k = 0; k < K; k++) {
+    Walker w;
+    walker_init(S, w);
+    w.pos.assign(pos + (size_t)k * ne * 3, pos + (size_t)(k + 1) * ne * 3);
+    walker_refresh(S, w, tmp);
+    PDeriv pd;
+    parm_deriv(S, w, pd, js);
+    np = (int)pd.g.size();
+    std::copy(pd.g.begin(), pd.g.end(), grad + (size_t)k * np);
+    std::copy(pd.gd.begin(), pd.gd.end(), gd + (size_t)k * np * ne * 4);
+    std::copy(pd.vg.begin(), pd.vg.end(), vg + (size_t)k * ne * 3);
+  }
+  return np;
 }
 
 extern "C" int orc_mo_eval(orc_system* o, int spin, int n, const double* pts, double* out) {

@@ -34,6 +34,17 @@ int orc_eval(orc_system* s, int K, const double* pos, const double* rot, int pp_
              const double* pp_rand, double* logpsi, double* sign, double* grad, double* lap,
              double* eparts);
 
+/* Linear Jastrow parameters (Wavefunction::getParmDeriv order): returns their number and,
+ * when values != NULL, their values. */
+int orc_nparm(orc_system* s, double* values);
+
+/* Sets the parameters (orc_nparm order); returns the number set.  For finite differences. */
+int orc_set_parms(orc_system* s, const double* values);
+
+/* d ln Psi / dp at K configurations: grad[K][np], gd[K][np][Ne][4] (d/dp of grad ln Psi and
+ * of lap Psi / Psi), vg[K][Ne][3] (the val_gradient of the reference).  Returns np. */
+int orc_parm_deriv(orc_system* s, int K, const double* pos, double* grad, double* gd, double* vg);
+
 /* MO values of one spin's orbital list at n points: out[n][5][nmo_spin]
  * (value, d/dx, d/dy, d/dz, laplacian). */
 int orc_mo_eval(orc_system* s, int spin, int n, const double* pts, double* out);
