@@ -20,16 +20,16 @@ __device__ inline const QW_CONST T* cptr(const T* p) {
 }
 
 // Compact shell record for LDS staging: rc = min(shell cutoff, object cutoff);
-// meta = atom | sym << 10 | nf << 13 | table << 17.
+// meta = atom | sym << 10 | nf << 14 | table << 18.
 struct ShellLds {
   double rc;
   int meta;
   int f0;
 };
 __device__ inline int shl_atom(int m) { return m & 1023; }
-__device__ inline int shl_sym(int m) { return (m >> 10) & 7; }
-__device__ inline int shl_nf(int m) { return (m >> 13) & 15; }
-__device__ inline int shl_table(int m) { return m >> 17; }
+__device__ inline int shl_sym(int m) { return (m >> 10) & 15; }
+__device__ inline int shl_nf(int m) { return (m >> 14) & 15; }
+__device__ inline int shl_table(int m) { return m >> 18; }
 
 // Device form of a Jastrow radial object (CUTOFF_CUSP kind 0, POLYPADE kind 1) with the
 // constants of its evaluation folded in: R = double (FP64 path) or float (mixed path).
@@ -273,6 +273,102 @@ __device__ inline double trans(const double ra[3], const double da[3], const dou
 }
 
 // ---------------------------------------------------------------- basis
+// ---------------------------------------------------------------- F and G shells
+// 7F and 9G are the reference's real solid harmonics (unnormalised, its function order), 15G
+// the Cartesian monomials, all as monomial lists c x^a y^b z^c (the reference's polynomials
+// expanded; src/basis_function/Cubic_spline_calc.cpp:646-720, 720-1070, function orders
+// src/basis_function/Cubic_spline_build.cpp:702-776).
+struct AngMono {
+  signed char c, ex, ey, ez;
+};
+__device__ inline double pw4(double v, int k) {
+  const double v2 = v * v;
+  return k == 0 ? 1.0 : k == 1 ? v : k == 2 ? v2 : k == 3 ? v2 * v : v2 * v2;
+}
+// Monomials of function id (0-6 F0 Fm3 Fp3mod Fp2 Fxyz Fm1 Fp1, 7-15 G0..G8, 16-30 the
+// Cartesian 15G in the order xxxx yyyy zzzz xxxy xxxz yyyx yyyz zzzx zzzy xxyy xxzz yyzz
+// xxyz yyxz zzxy); returns their count, *first the index of the first.
+__device__ inline int ang_monos(int id, const AngMono** first) {
+  static constexpr AngMono m[] = {
+      {2, 0, 0, 3},   {-3, 2, 0, 1},  {-3, 0, 2, 1},                                  // F0
+      {3, 2, 1, 0},   {-1, 0, 3, 0},                                                  // Fm3
+      {1, 3, 0, 0},   {-3, 1, 2, 0},                                                  // Fp3mod
+      {1, 2, 0, 1},   {-1, 0, 2, 1},                                                  // Fp2
+      {1, 1, 1, 1},                                                                   // Fxyz
+      {4, 0, 1, 2},   {-1, 2, 1, 0},  {-1, 0, 3, 0},                                  // Fm1
+      {4, 1, 0, 2},   {-1, 3, 0, 0},  {-1, 1, 2, 0},                                  // Fp1
+      {3, 4, 0, 0},   {3, 0, 4, 0},   {8, 0, 0, 4}, {6, 2, 2, 0}, {-24, 2, 0, 2}, {-24, 0, 2, 2},  // G0
+      {4, 1, 0, 3},   {-3, 3, 0, 1},  {-3, 1, 2, 1},                                  // G1
+      {4, 0, 1, 3},   {-3, 2, 1, 1},  {-3, 0, 3, 1},                                  // G2
+      {-1, 4, 0, 0},  {1, 0, 4, 0},   {6, 2, 0, 2}, {-6, 0, 2, 2},                    // G3
+      {6, 1, 1, 2},   {-1, 3, 1, 0},  {-1, 1, 3, 0},                                  // G4
+      {1, 3, 0, 1},   {-3, 1, 2, 1},                                                  // G5
+      {3, 2, 1, 1},   {-1, 0, 3, 1},                                                  // G6
+      {1, 4, 0, 0},   {-6, 2, 2, 0},  {1, 0, 4, 0},                                   // G7
+      {1, 3, 1, 0},   {-1, 1, 3, 0},                                                  // G8
+      {1, 4, 0, 0},   {1, 0, 4, 0},   {1, 0, 0, 4}, {1, 3, 1, 0}, {1, 3, 0, 1}, {1, 1, 3, 0},
+      {1, 0, 3, 1},   {1, 1, 0, 3},   {1, 0, 1, 3}, {1, 2, 2, 0}, {1, 2, 0, 2}, {1, 0, 2, 2},
+      {1, 2, 1, 1},   {1, 1, 2, 1},   {1, 1, 1, 2}};                                  // 15G
+  static constexpr unsigned char start[32] = {0,  3,  5,  7,  9,  10, 13, 16, 22, 25, 28, 32, 35, 37, 39, 42, 44,
+                                              45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59};
+  *first = m + start[id];
+  return start[id + 1] - start[id];
+}
+// Function id of the i-th function of a 7F, 7F_crystal, 9G, 9G_pyscf or 15G shell.
+__device__ inline int ang_fn(int sym, int i) {
+  static constexpr unsigned char f7c[7] = {0, 6, 5, 3, 4, 2, 1};
+  static constexpr unsigned char g9p[9] = {15, 13, 11, 9, 7, 8, 10, 12, 14};
+  switch (sym) {
+    case SK_7F: return i;
+    case SK_7FC: return f7c[i];
+    case SK_9G: return 7 + i;
+    case SK_9GP: return g9p[i];
+    default: return 16 + i;  // SK_15G
+  }
+}
+// psi = P f(r): grad psi = f grad P + (f'/r) P r_vec; lap psi = f lap P + P (f'' + (2 + 2l) f'/r)
+// for P homogeneous of degree l (r_vec . grad P = l P).
+template <typename T>
+__device__ inline void shell_poly_lap(int sym, double f, double fd, double f2, double x, double y, double z, T* out,
+                                      int ld) {
+  const int nf = (sym == SK_7F || sym == SK_7FC) ? 7 : (sym == SK_15G ? 15 : 9);
+  const double lr = (sym == SK_7F || sym == SK_7FC) ? 8.0 : 10.0;
+  for (int i = 0; i < nf; i++) {
+    const AngMono* m;
+    const int nm = ang_monos(ang_fn(sym, i), &m);
+    double P = 0, Px = 0, Py = 0, Pz = 0, L = 0;
+    for (int k = 0; k < nm; k++) {
+      const double c = m[k].c;
+      const int a = m[k].ex, b = m[k].ey, d = m[k].ez;
+      const double xa = pw4(x, a), yb = pw4(y, b), zd = pw4(z, d);
+      P += c * xa * yb * zd;
+      if (a) Px += c * a * pw4(x, a - 1) * yb * zd;
+      if (b) Py += c * b * xa * pw4(y, b - 1) * zd;
+      if (d) Pz += c * d * xa * yb * pw4(z, d - 1);
+      if (a > 1) L += c * a * (a - 1) * pw4(x, a - 2) * yb * zd;
+      if (b > 1) L += c * b * (b - 1) * xa * pw4(y, b - 2) * zd;
+      if (d > 1) L += c * d * (d - 1) * xa * yb * pw4(z, d - 2);
+    }
+    const double h = P * fd;
+    out[i] = (T)(P * f);
+    out[ld + i] = (T)(f * Px + h * x);
+    out[2 * ld + i] = (T)(f * Py + h * y);
+    out[3 * ld + i] = (T)(f * Pz + h * z);
+    out[4 * ld + i] = (T)(f * L + P * f2 + lr * h);
+  }
+}
+template <typename T>
+__device__ inline void shell_poly_val(int sym, double f, double x, double y, double z, T* out) {
+  const int nf = (sym == SK_7F || sym == SK_7FC) ? 7 : (sym == SK_15G ? 15 : 9);
+  for (int i = 0; i < nf; i++) {
+    const AngMono* m;
+    const int nm = ang_monos(ang_fn(sym, i), &m);
+    double P = 0;
+    for (int k = 0; k < nm; k++) P += m[k].c * pw4(x, m[k].ex) * pw4(y, m[k].ey) * pw4(z, m[k].ez);
+    out[i] = (T)(P * f);
+  }
+}
+
 // Values of one spline shell at distance vector (x,y,z), r: writes nf functions x 5
 // components to out[c*ld + i] (S, P, 5D, 6D, 10F; src/basis_function/Cubic_spline_calc.cpp:353-700).
 template <typename T>
@@ -322,7 +418,7 @@ __device__ inline void shell_lap(int sym, double f, double fd, double f2, double
       v = y * z; h = v * fd;
       put(5, v * f, h * x, h * y + f * z, h * z + f * y, v * f2 + 6. * h);
       break;
-    default:  // SK_10F
+    case SK_10F:
       v2 = x * x; v3 = v2 * x; h = v3 * fd;
       put(0, v3 * f, h * x + f * 3. * v2, h * y, h * z, v3 * f2 + 8. * h + 6. * f * x);
       v2 = y * y; v3 = v2 * y; h = v3 * fd;
@@ -344,6 +440,9 @@ __device__ inline void shell_lap(int sym, double f, double fd, double f2, double
       v3 = x * y * z; v2 = f * z; h = v3 * fd;
       put(9, v3 * f, h * x + v2 * y, h * y + v2 * x, h * z + f * x * y, v3 * f2 + 8. * h);
       break;
+    default:  // 7F, 7F_crystal, 9G, 9G_pyscf, 15G
+      shell_poly_lap<T>(sym, f, fd, f2, x, y, z, out, ld);
+      break;
   }
 }
 
@@ -364,11 +463,14 @@ __device__ inline void shell_val(int sym, double f, double x, double y, double z
       out[0] = (T)((x * x) * f); out[1] = (T)((y * y) * f); out[2] = (T)((z * z) * f);
       out[3] = (T)((x * y) * f); out[4] = (T)((x * z) * f); out[5] = (T)((y * z) * f);
       break;
-    default:
+    case SK_10F:
       out[0] = (T)((x * x * x) * f); out[1] = (T)((y * y * y) * f); out[2] = (T)((z * z * z) * f);
       out[3] = (T)((x * x * y) * f); out[4] = (T)((x * x * z) * f); out[5] = (T)((y * y * x) * f);
       out[6] = (T)((y * y * z) * f); out[7] = (T)((z * z * x) * f); out[8] = (T)((z * z * y) * f);
       out[9] = (T)((x * y * z) * f);
+      break;
+    default:
+      shell_poly_val<T>(sym, f, x, y, z, out);
       break;
   }
 }

@@ -327,12 +327,12 @@ struct qw_engine {
       S.eit_Z = upload(eZ);
     }
     {
-      if (T.na > 1023 || T.ntab >= (1 << 14)) throw std::runtime_error("too many atoms or spline tables");
+      if (T.na > 1023 || T.ntab >= (1 << 13)) throw std::runtime_error("too many atoms or spline tables");
       std::vector<ShellLds> shl(T.shells.size());
       for (size_t i = 0; i < T.shells.size(); i++) {
         const ShellDesc& d = T.shells[i];
         shl[i].rc = std::min(d.rcut, d.objthr);
-        shl[i].meta = d.atom | (d.sym << 10) | (d.nf << 13) | (d.table << 17);
+        shl[i].meta = d.atom | (d.sym << 10) | (d.nf << 14) | (d.table << 18);
         shl[i].f0 = d.f0;
       }
       S.shl = upload(shl);

@@ -115,22 +115,25 @@ __device__ inline int basis_batch_lap(const DevSys& S, const SysLds& V, const TS
       const float f2 = 2.f * c2 + 6.f * h * c3;
       shell_lap<float>(sym, f, fd, f2, xf, yf, zf, (float*)(chi + at), ld);
     }
-    for (int q = 0; q < nf; q++) roff[at + q] = (d.f0 + q) * n;
+    for (int q = 0; q < nf; q++) roff[at + q] = (d.f0 + q) * n * (int)sizeof(T);
   }
   return take;
 }
 
-// acc[c] += sum_b C[roff[b] + lane] * chi[c*ld + b] for b < nfun (lane = MO j < n), with U
-// coefficient loads in flight.
+// acc[c] += sum_b C[f_b][lane] * chi[c*ld + b] for b < nfun (lane = MO j < n), with U
+// coefficient loads in flight.  roff[b] is the byte offset of function f_b's coefficient row,
+// so each load is a uniform base plus a 32-bit per-lane offset (one integer add).
 template <typename T, int U>
 __device__ inline void lcao_batch(const T* __restrict__ C, const int* roff, const T* chi, int ld, int nfun, T acc[5],
                                   int lane) {
-  const T* Cp = C + lane;
+  const char* Cb = reinterpret_cast<const char*>(C);
+  const unsigned lb = (unsigned)lane * (unsigned)sizeof(T);
   int b = 0;
   for (; b + U <= nfun; b += U) {
     T cv[U];
 #pragma unroll
-    for (int q = 0; q < U; q++) cv[q] = Cp[roff[b + q]];
+    for (int q = 0; q < U; q++) cv[q] = *reinterpret_cast<const T*>(Cb + ((unsigned)roff[b + q] + lb));
+    __builtin_amdgcn_sched_barrier(0);  // all U loads issue before the first use
 #pragma unroll
     for (int q = 0; q < U; q++) {
       acc[0] += cv[q] * chi[b + q];
@@ -141,7 +144,7 @@ __device__ inline void lcao_batch(const T* __restrict__ C, const int* roff, cons
     }
   }
   for (; b < nfun; b++) {
-    const T cv = Cp[roff[b]];
+    const T cv = *reinterpret_cast<const T*>(Cb + ((unsigned)roff[b] + lb));
     acc[0] += cv * chi[b];
     acc[1] += cv * chi[ld + b];
     acc[2] += cv * chi[2 * ld + b];

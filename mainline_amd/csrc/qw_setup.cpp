@@ -202,10 +202,21 @@ int sym_of(const std::string& s) {
   if (same(s, "5D")) return SK_5D;
   if (same(s, "D") || same(s, "6D")) return SK_6D;
   if (same(s, "F") || same(s, "10F")) return SK_10F;
+  if (same(s, "7F")) return SK_7F;
+  if (same(s, "7F_crystal")) return SK_7FC;
+  if (same(s, "G") || same(s, "15G")) return SK_15G;
+  if (same(s, "9G")) return SK_9G;
+  if (same(s, "9G_pyscf")) return SK_9GP;
   unsupported("AOSPLINE symmetry " + s + " is not supported");
 }
-int sym_nf(int s) { return s == SK_S ? 1 : s == SK_P ? 3 : s == SK_5D ? 5 : s == SK_6D ? 6 : 10; }
-int sym_l(int s) { return s == SK_S ? 0 : s == SK_P ? 1 : s == SK_10F ? 3 : 2; }
+int sym_nf(int s) {
+  static const int nf[] = {1, 3, 5, 6, 10, 7, 7, 9, 9, 15};
+  return nf[s];
+}
+int sym_l(int s) {
+  static const int l[] = {0, 1, 2, 2, 3, 3, 3, 4, 4, 4};
+  return l[s];
+}
 
 struct BasisObj {
   std::string label;
@@ -255,7 +266,8 @@ BasisObj read_aospline(const Words& w) {
         if (l == 0) nrm = std::sqrt(2. * f4 * fac);
         else if (l == 1) nrm = std::sqrt(2. * f4 * f4 * fac / 3.);
         else if (l == 2) nrm = std::sqrt(2. * f4 * f4 * f4 * fac / 15.);
-        else nrm = std::sqrt(2. * (f4 * f4) * (f4 * f4) * fac / 105.);
+        else if (l == 3) nrm = std::sqrt(2. * (f4 * f4) * (f4 * f4) * fac / 105.);
+        else nrm = std::sqrt(2. * (f4 * f4) * (f4 * f4) * f4 * fac / 945.);
         (void)pw;
         (void)dfact;
       } else if (norm == "CRYSTAL") {

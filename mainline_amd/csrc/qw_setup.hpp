@@ -11,7 +11,10 @@ constexpr int kMaxPPL = 6;    // pseudopotential channels
 constexpr int kMaxPPT = 6;    // Gaussian terms per channel
 constexpr int kMaxAIP = 12;   // quadrature points
 
-enum SymKind { SK_S = 0, SK_P = 1, SK_5D = 2, SK_6D = 3, SK_10F = 4 };
+// Shell symmetries (src/basis_function/Cubic_spline_build.cpp:26-140,664-800).  7F_crystal and
+// 9G_pyscf are the 7F and 9G functions in the orders of those programs.
+enum SymKind { SK_S = 0, SK_P = 1, SK_5D = 2, SK_6D = 3, SK_10F = 4, SK_7F = 5, SK_7FC = 6, SK_9G = 7, SK_9GP = 8,
+               SK_15G = 9 };
 
 // One radial spline shell placed on an atom.
 struct ShellDesc {

@@ -26,6 +26,10 @@ atoms is 12 * (N/16)^(1/3) bohr; all atoms of a molecule share one species and o
 import os
 import numpy as np
 
+# functions per AOSPLINE shell symmetry
+SHELL_NF = {"S": 1, "P": 3, "5D": 5, "6D": 6, "10F": 10, "7F": 7, "7F_crystal": 7, "9G": 9, "9G_pyscf": 9,
+            "15G": 15}
+
 JASTROW_CUSP_GROUP = """GROUP {
   EEBASIS { EE CUTOFF_CUSP GAMMA 24.0 CUSP 1.0 CUTOFF 7.5 }
   EEBASIS { EE CUTOFF_CUSP GAMMA 24.0 CUSP 1.0 CUTOFF 7.5 }
@@ -68,7 +72,7 @@ class Molecule:
 
     @property
     def nfunc_atom(self):
-        return sum({"S": 1, "P": 3, "5D": 5}[s] for s, _ in self.shells)
+        return sum(SHELL_NF[s] for s, _ in self.shells)
 
     @property
     def nbasis(self):
@@ -249,6 +253,22 @@ def z4_molecule(natoms, seed, name=None, pseudo=False):
     return Molecule(name or ("pp%d" % natoms if pseudo else "ae%d" % natoms), label, 4.0,
                     coords, nup, ndn, shells, mo_up, mo_dn, ei, ee, 4, pseudo=pp,
                     seed=(seed % 100000007, 7))
+
+
+def fg_molecule(seed=20261206):
+    """Three Z=4 atoms whose basis holds every supported shell symmetry (S, P, 6D, 7F,
+    7F_crystal, 10F, 9G, 9G_pyscf, 15G: 67 functions per atom), 6 + 6 electrons: the fixture
+    system for the F and G angular functions."""
+    rng = np.random.default_rng(seed)
+    coords = np.array([[0.0, 0.0, 0.0], [2.4, 0.3, -0.2], [-0.7, 2.1, 0.9]])
+    syms = ["S", "P", "6D", "7F", "7F_crystal", "10F", "9G", "9G_pyscf", "15G"]
+    shells = [(sy, [(float(a), 1.0)]) for sy, a in zip(syms, np.exp(rng.uniform(np.log(0.3), np.log(1.5), len(syms))))]
+    nb = 3 * sum(SHELL_NF[sy] for sy in syms)
+    mo_up = _lowdin(rng.standard_normal((nb, 6)))
+    mo_dn = _lowdin(rng.standard_normal((nb, 6)))
+    ei = rng.uniform(-0.1, 0.1, size=4)
+    ee = rng.uniform(-0.1, 0.1, size=4)
+    return Molecule("fg3", "Be", 4.0, coords, 6, 6, shells, mo_up, mo_dn, ei, ee, 4, seed=(seed % 100000007, 7))
 
 
 def pp16(seed=20261203):

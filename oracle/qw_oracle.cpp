@@ -233,13 +233,26 @@ struct Spline {
 };
 
 // Angular symmetries (src/basis_function/Cubic_spline_build.cpp:26-120,664-800).
-enum Sym { SYM_S, SYM_P, SYM_5D, SYM_6D, SYM_10F };
+enum Sym { SYM_S, SYM_P, SYM_5D, SYM_6D, SYM_10F, SYM_7F, SYM_7FC, SYM_9G, SYM_9GP, SYM_15G };
 enum ISym {
   I_S, I_PX, I_PY, I_PZ,
   I_DXX, I_DYY, I_DZZ, I_DXY, I_DXZ, I_DYZ, I_DX2Y2, I_DZ2R2,
   I_FXXX, I_FYYY, I_FZZZ, I_FXXY, I_FXXZ, I_FYYX, I_FYYZ, I_FZZX, I_FZZY, I_FXYZ
 };
-int sym_lvalue(Sym s) { return s == SYM_S ? 0 : s == SYM_P ? 1 : (s == SYM_10F ? 3 : 2); }
+constexpr int kPoly = 1000;  // individual-function codes >= kPoly: ang_polys()[code - kPoly]
+
+int sym_lvalue(Sym s) {
+  switch (s) {
+    case SYM_S: return 0;
+    case SYM_P: return 1;
+    case SYM_5D:
+    case SYM_6D: return 2;
+    case SYM_10F:
+    case SYM_7F:
+    case SYM_7FC: return 3;
+    default: return 4;
+  }
+}
 int sym_nfunc(Sym s) {
   switch (s) {
     case SYM_S: return 1;
@@ -247,8 +260,51 @@ int sym_nfunc(Sym s) {
     case SYM_5D: return 5;
     case SYM_6D: return 6;
     case SYM_10F: return 10;
+    case SYM_7F:
+    case SYM_7FC: return 7;
+    case SYM_9G:
+    case SYM_9GP: return 9;
+    case SYM_15G: return 15;
   }
   return 0;
+}
+
+// 7F and 9G real solid harmonics as the reference writes them (unnormalised) and the 15G
+// Cartesian monomials, as polynomials {coefficient, power of x, y, z}
+// (src/basis_function/Cubic_spline_calc.cpp:646-720,720-1070; orders
+// src/basis_function/Cubic_spline_build.cpp:702-776).  Indexed by function: F0 Fm3 Fp3mod
+// Fp2 Fxyz Fm1 Fp1 (0-6), G0..G8 (7-15), xxxx yyyy zzzz xxxy xxxz yyyx yyyz zzzx zzzy xxyy
+// xxzz yyzz xxyz yyxz zzxy (16-30).
+struct PolyTerm {
+  int c, a, b, d;
+};
+const std::vector<std::vector<PolyTerm>>& ang_polys() {
+  static const std::vector<std::vector<PolyTerm>> P = {
+      {{2, 0, 0, 3}, {-3, 2, 0, 1}, {-3, 0, 2, 1}},   // z(2z^2 - 3x^2 - 3y^2)
+      {{3, 2, 1, 0}, {-1, 0, 3, 0}},                  // y(3x^2 - y^2)
+      {{1, 3, 0, 0}, {-3, 1, 2, 0}},                  // x(x^2 - 3y^2)
+      {{1, 2, 0, 1}, {-1, 0, 2, 1}},                  // z(x^2 - y^2)
+      {{1, 1, 1, 1}},                                 // xyz
+      {{4, 0, 1, 2}, {-1, 2, 1, 0}, {-1, 0, 3, 0}},   // y(4z^2 - x^2 - y^2)
+      {{4, 1, 0, 2}, {-1, 3, 0, 0}, {-1, 1, 2, 0}},   // x(4z^2 - x^2 - y^2)
+      {{3, 4, 0, 0}, {3, 0, 4, 0}, {8, 0, 0, 4}, {6, 2, 2, 0}, {-24, 2, 0, 2}, {-24, 0, 2, 2}},  // 35z^4-30z^2r^2+3r^4
+      {{4, 1, 0, 3}, {-3, 3, 0, 1}, {-3, 1, 2, 1}},   // xz(7z^2 - 3r^2)
+      {{4, 0, 1, 3}, {-3, 2, 1, 1}, {-3, 0, 3, 1}},   // yz(7z^2 - 3r^2)
+      {{-1, 4, 0, 0}, {1, 0, 4, 0}, {6, 2, 0, 2}, {-6, 0, 2, 2}},  // (x^2 - y^2)(7z^2 - r^2)
+      {{6, 1, 1, 2}, {-1, 3, 1, 0}, {-1, 1, 3, 0}},   // xy(7z^2 - r^2)
+      {{1, 3, 0, 1}, {-3, 1, 2, 1}},                  // xz(x^2 - 3y^2)
+      {{3, 2, 1, 1}, {-1, 0, 3, 1}},                  // yz(3x^2 - y^2)
+      {{1, 4, 0, 0}, {-6, 2, 2, 0}, {1, 0, 4, 0}},    // x^4 - 6x^2y^2 + y^4
+      {{1, 3, 1, 0}, {-1, 1, 3, 0}},                  // xy(x^2 - y^2)
+      {{1, 4, 0, 0}}, {{1, 0, 4, 0}}, {{1, 0, 0, 4}}, {{1, 3, 1, 0}}, {{1, 3, 0, 1}}, {{1, 1, 3, 0}},
+      {{1, 0, 3, 1}}, {{1, 1, 0, 3}}, {{1, 0, 1, 3}}, {{1, 2, 2, 0}}, {{1, 2, 0, 2}}, {{1, 0, 2, 2}},
+      {{1, 2, 1, 1}}, {{1, 1, 2, 1}}, {{1, 1, 1, 2}}};
+  return P;
+}
+inline double ipow(double v, int k) {
+  double r = 1;
+  for (int i = 0; i < k; i++) r *= v;
+  return r;
 }
 Sym sym_lookup(const std::string& s) {
   if (ieq(s, "S")) return SYM_S;
@@ -256,6 +312,11 @@ Sym sym_lookup(const std::string& s) {
   if (ieq(s, "D") || ieq(s, "6D")) return SYM_6D;
   if (ieq(s, "5D")) return SYM_5D;
   if (ieq(s, "F") || ieq(s, "10F")) return SYM_10F;
+  if (ieq(s, "7F")) return SYM_7F;
+  if (ieq(s, "7F_crystal")) return SYM_7FC;
+  if (ieq(s, "G") || ieq(s, "15G")) return SYM_15G;
+  if (ieq(s, "9G")) return SYM_9G;
+  if (ieq(s, "9G_pyscf")) return SYM_9GP;
   fail("unsupported AOSPLINE symmetry " + s);
 }
 
@@ -318,6 +379,9 @@ struct AOSpline {
             case SYM_P: nrm = sqrt(2. * feg2 * fac / 3.); break;
             case SYM_5D:
             case SYM_6D: nrm = sqrt(2. * feg * feg2 * fac / 15.); break;
+            case SYM_9G:
+            case SYM_9GP:
+            case SYM_15G: nrm = sqrt(2. * feg2 * feg2 * feg * fac / 945.); break;
             default: nrm = sqrt(2. * feg2 * feg2 * fac / 105.); break;
           }
         } else if (norm == "CRYSTAL") {
@@ -364,6 +428,22 @@ struct AOSpline {
           indiv.insert(indiv.end(), {I_FXXX, I_FYYY, I_FZZZ, I_FXXY, I_FXXZ, I_FYYX, I_FYYZ,
                                      I_FZZX, I_FZZY, I_FXYZ});
           break;
+        // polynomial functions: kPoly + index into ang_polys()
+        case SYM_7F:
+          for (int q : {0, 1, 2, 3, 4, 5, 6}) indiv.push_back(kPoly + q);
+          break;
+        case SYM_7FC:
+          for (int q : {0, 6, 5, 3, 4, 2, 1}) indiv.push_back(kPoly + q);
+          break;
+        case SYM_9G:
+          for (int q = 7; q <= 15; q++) indiv.push_back(kPoly + q);
+          break;
+        case SYM_9GP:
+          for (int q : {15, 13, 11, 9, 7, 8, 10, 12, 14}) indiv.push_back(kPoly + q);
+          break;
+        case SYM_15G:
+          for (int q = 16; q <= 30; q++) indiv.push_back(kPoly + q);
+          break;
       }
     }
     nfunc = (int)indiv.size();
@@ -392,6 +472,34 @@ struct AOSpline {
       for (; tot < end; tot++) {
         double* o = out + 5 * tot;
         double x = r[2], y = r[3], z = r[4], v, h, v2, v3, rdp;
+        if (indiv[tot] >= kPoly) {
+          // psi = P f: grad = f grad P + (f'/r) P r_vec, lap = f lap P + 2 (f'/r) r.grad P + P (f'' + 2 f'/r)
+          const std::vector<PolyTerm>& T = ang_polys()[indiv[tot] - kPoly];
+          double P = 0, G[3] = {0, 0, 0}, L = 0, rg = 0;
+          for (const PolyTerm& t : T) {
+            const int e[3] = {t.a, t.b, t.d};
+            const double c[3] = {x, y, z};
+            P += t.c * ipow(x, t.a) * ipow(y, t.b) * ipow(z, t.d);
+            for (int k = 0; k < 3; k++) {
+              if (e[k] == 0) continue;
+              double term = t.c * e[k];
+              for (int q = 0; q < 3; q++) term *= ipow(c[q], q == k ? e[q] - 1 : e[q]);
+              G[k] += term;
+              if (e[k] > 1) {
+                double t2 = t.c * e[k] * (e[k] - 1);
+                for (int q = 0; q < 3; q++) t2 *= ipow(c[q], q == k ? e[q] - 2 : e[q]);
+                L += t2;
+              }
+            }
+          }
+          rg = x * G[0] + y * G[1] + z * G[2];
+          o[0] = P * f;
+          o[1] = f * G[0] + fd * P * x;
+          o[2] = f * G[1] + fd * P * y;
+          o[3] = f * G[2] + fd * P * z;
+          o[4] = f * L + 2 * fd * rg + P * (f2 + 2 * fd);
+          continue;
+        }
         switch (indiv[tot]) {
           case I_S:
             o[0] = f; o[1] = fd * x; o[2] = fd * y; o[3] = fd * z; o[4] = f2 + 2. * fd;

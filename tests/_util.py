@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 _DECK_DIR = os.path.join(tempfile.gettempdir(), "qw_test_decks_%d" % os.getuid())
 
-SYNTH = {"h4": synth.h4, "pp16": synth.pp16, "ae32": synth.ae32,
+SYNTH = {"h4": synth.h4, "pp16": synth.pp16, "ae32": synth.ae32, "fg3": synth.fg_molecule,
          # 80 + 80 electrons: the block-per-walker path for more than 64 electrons of a spin
          "ae40": lambda: synth.z4_molecule(40, 20261240, name="ae40")}
 

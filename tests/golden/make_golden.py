@@ -17,6 +17,8 @@ Cases:
   n2_pp     the reference's tests/n2 basis, orbitals and BFD pseudopotential (AIP 6), one
             determinant, with a POLYPADE Jastrow of nonzero coefficients
   h4, pp16, ae32   the seeded synthetic molecules of mainline_amd/synth.py (4, 64, 128 e)
+  fg3       three atoms with every supported shell symmetry (S P 6D 7F 7F_crystal 10F 9G
+            9G_pyscf 15G), 12 electrons
 
 Usage: python tests/golden/make_golden.py   (from the repository root)
 """
@@ -143,7 +145,7 @@ def main():
             res = run_ref(d, deck, pos)
             np.savez_compressed(os.path.join(OUT, name + ".npz"), deck=os.path.relpath(os.path.join(d, deck), OUT), **res)
             print(name, res["pos"].shape)
-        for mol, k in [(synth.h4(), 16), (synth.pp16(), 8), (synth.ae32(), 4)]:
+        for mol, k in [(synth.h4(), 16), (synth.pp16(), 8), (synth.ae32(), 4), (synth.fg_molecule(), 12)]:
             d = os.path.join(tmp, mol.name)
             deck = os.path.basename(mol.write(d))
             orb = open(os.path.join(d, mol.name + ".orb"), "rb").read()

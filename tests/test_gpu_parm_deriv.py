@@ -27,7 +27,7 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("name", ["h_atom", "h2_jast", "n2_pp", "h4", "pp16", "ae32"])
+@pytest.mark.parametrize("name", ["h_atom", "h2_jast", "n2_pp", "h4", "pp16", "ae32", "fg3"])
 def test_fp64_parm_deriv_matches_reference_golden(name):
     g, deck = golden(name)
     _say("parm deriv golden", name, g["pos"].shape)
