@@ -328,8 +328,10 @@ __device__ inline int ang_fn(int sym, int i) {
 }
 // psi = P f(r): grad psi = f grad P + (f'/r) P r_vec; lap psi = f lap P + P (f'' + (2 + 2l) f'/r)
 // for P homogeneous of degree l (r_vec . grad P = l P).
+// Kept out of line: F and G shells are rare, and inlining would add their registers to every
+// sweep kernel.
 template <typename T>
-__device__ inline void shell_poly_lap(int sym, double f, double fd, double f2, double x, double y, double z, T* out,
+__device__ __attribute__((noinline)) void shell_poly_lap(int sym, double f, double fd, double f2, double x, double y, double z, T* out,
                                       int ld) {
   const int nf = (sym == SK_7F || sym == SK_7FC) ? 7 : (sym == SK_15G ? 15 : 9);
   const double lr = (sym == SK_7F || sym == SK_7FC) ? 8.0 : 10.0;
@@ -358,7 +360,7 @@ __device__ inline void shell_poly_lap(int sym, double f, double fd, double f2, d
   }
 }
 template <typename T>
-__device__ inline void shell_poly_val(int sym, double f, double x, double y, double z, T* out) {
+__device__ __attribute__((noinline)) void shell_poly_val(int sym, double f, double x, double y, double z, T* out) {
   const int nf = (sym == SK_7F || sym == SK_7FC) ? 7 : (sym == SK_15G ? 15 : 9);
   for (int i = 0; i < nf; i++) {
     const AngMono* m;

@@ -3,6 +3,9 @@
 // (gosling, tests/qwtest.py) reads them unchanged:
 //   <deck>.log     block records (src/properties/Properties.cpp:578-641,
 //                  src/properties/Properties_block.cpp:73-138)
+//   <deck>.json    one record per block, each the one-block final average followed by <RS>
+//                  (src/method/Dmc_method.cpp:541-549, src/method/Vmc_method.cpp:526-534,
+//                  src/properties/Properties_average.cpp:449-520)
 //   <deck>.config  walker positions (+ DMC weight) (src/program_utils/qmc_io.h:371-423,
 //                  src/system/Sample_point.cpp:120-148, src/method/Dmc_method.cpp:1228-1265)
 // Walkers come from <deck>.config when it exists (READCONFIG), otherwise from the reference's
@@ -82,6 +85,8 @@ void write_log(const std::string& path, const char* label, double tau, bool dmc,
   lo << "#-------------------------------------------------\n";
   lo << "#" << (dmc ? "DMC" : "VMC") << " run: timestep " << tau << "\n";
   lo << "#-------------------------------------------------\n\n\n";
+  // the run's header record, with no average generators (src/properties/Properties.cpp:1020-1034)
+  lo << "init { \n   label " << label << "\n}\n";
   for (int b = 0; b < nblock; b++) {
     const double* r = &eb[(size_t)b * QW_BLOCK_FIELDS];
     auto sec = [&](const char* name, double v) { lo << "   " << name << " { " << v << "   } \n"; };
@@ -101,6 +106,35 @@ void write_log(const std::string& path, const char* label, double tau, bool dmc,
     sec("nonlocal_energyvar", r[9]);
     lo << "   autocorr_energy { } \n";
     lo << "} \n\n";
+  }
+}
+
+// Per block: the reduction of that one block alone, so value = the block average, error = 0,
+// sigma = the square root of the block variance, and the independent-point estimate
+// sigma² / error² (infinite, with the reference's warning, whenever the variance is nonzero).
+void write_json(const std::string& path, const std::vector<double>& eb, int nblock) {
+  std::ofstream js(path, std::ios::app);  // default stream precision, as the reference's
+  static const char* names[] = {"total_energy", "kinetic", "potential", "nonlocal", "weight"};
+  static const int avg_at[] = {0, 1, 2, 3, 10}, var_at[] = {6, 7, 8, 9, 11};
+  for (int b = 0; b < nblock; b++) {
+    const double* r = &eb[(size_t)b * QW_BLOCK_FIELDS];
+    const double err = 0.0;
+    js << "{\n\"warmup blocks\":0,\n\"properties\":{\n";
+    for (int i = 0; i < 5; i++) {
+      js << "\"" << names[i] << "\":{\n";
+      js << "\"value\":[" << r[avg_at[i]] << "],\n";
+      js << "\"error\":[" << std::sqrt(err) << "],\n";
+      js << "\"sigma\":[" << std::sqrt(r[var_at[i]]) << "]\n";
+      js << "}" << (i != 4 ? "," : "") << "\n";
+    }
+    js << "},\n";
+    const double indep = std::sqrt(r[6]) / std::sqrt(err), totpoints = indep * indep;
+    js << "\"independent points\":" << totpoints << ",\n";
+    js << "\"commments\": [\n";
+    if (totpoints > r[5])
+      js << "\"Warning: The estimated number of independent points is _greater_ that the points in the log "
+            "file. Error bars are probably unreliable! \"\n";
+    js << "]\n}\n<RS>\n";
   }
 }
 
@@ -172,6 +206,7 @@ int main(int argc, char** argv) {
   double stats[4];
   if (qw_run(e, &m, eb.data(), stats) != QW_OK) return fail("run", e);
   write_log(deck + ".log", dmc ? "dmc" : "vmc", m.timestep, dmc, eb, m.nblock);
+  write_json(deck + ".json", eb, m.nblock);
   if (qw_walkers_get(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
   write_configs(cfg, nw, ne, pos, w, dmc);
   double s = 0, s2 = 0;

@@ -20,6 +20,10 @@ Cases:
   fg3       three atoms with every supported shell symmetry (S P 6D 7F 7F_crystal 10F 9G
             9G_pyscf 15G), 12 electrons
 
+Output files: the reference program (oracle/_ref/qwalk) runs a short DMC of the tests/h deck
+and its block log and per-block JSON are kept as ref_output/h_dmc.log and h_dmc.json (the
+layout of the files the qwalk_amd driver writes is checked against them).
+
 Usage: python tests/golden/make_golden.py   (from the repository root)
 """
 import hashlib
@@ -34,7 +38,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 from mainline_amd import synth  # noqa: E402
-from oracle.oracle import REF_GOLDEN, parse_golden  # noqa: E402
+from oracle.oracle import REF_GOLDEN, REF_QWALK, parse_golden  # noqa: E402
 
 REF_TESTS = "/root/reference/tests"
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -136,10 +140,29 @@ def ref_deck_cases(tmp):
     return cases
 
 
+OUTPUT_DECK = "method { DMC timestep 0.02 nblock 2 nstep 5 nconfig 16 }\ninclude qw.sys\ntrialfunc { include qw.slater }\n"
+
+
+def ref_output_files(tmp):
+    """The reference's own .log and .json for a short DMC of the H atom."""
+    d = os.path.join(tmp, "h_out")
+    os.makedirs(d)
+    for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb"]:
+        shutil.copy(os.path.join(REF_TESTS, "h", f), d)
+    open(os.path.join(d, "qw.run"), "w").write(OUTPUT_DECK)
+    subprocess.run([REF_QWALK, "qw.run"], cwd=d, check=True, capture_output=True, timeout=300)
+    out = os.path.join(OUT, "ref_output")
+    os.makedirs(out, exist_ok=True)
+    open(os.path.join(out, "h_dmc.deck"), "w").write(OUTPUT_DECK)
+    for ext in ["log", "json"]:
+        shutil.copy(os.path.join(d, "qw.run." + ext), os.path.join(out, "h_dmc." + ext))
+
+
 def main():
     if not os.path.exists(REF_GOLDEN):
         sys.exit("build the reference first: make -f oracle/Makefile.ref -j8")
     with tempfile.TemporaryDirectory() as tmp:
+        ref_output_files(tmp)
         cases = ref_deck_cases(tmp)
         for name, (d, deck, pos) in cases.items():
             res = run_ref(d, deck, pos)

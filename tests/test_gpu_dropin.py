@@ -4,7 +4,9 @@ block log, and the reference's gosling (compiled from its sources under oracle/_
 it; the energies must match GAMESS HF (-0.4998098113) and the exact -0.5 within 3 sigma, with
 error bars below 0.1 (tests/qwtest.py:28-36)."""
 import json
+import math
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -46,3 +48,49 @@ def test_h_atom_vmc_through_driver_and_gosling(tmp_path):
 @pytest.mark.skipif(not os.path.exists(GOSLING), reason="reference gosling not built (oracle/Makefile.ref)")
 def test_h_atom_dmc_through_driver_and_gosling(tmp_path):
     _run_case(tmp_path, "method { DMC timestep 0.005 nblock 80 }", -0.5)
+
+
+NUM = re.compile(r"[-+]?(\d+\.?\d*|\.\d+)([eE][-+]?\d+)?|[-+]?\b(inf|nan)\b")
+
+
+def _skeleton(text):
+    return [NUM.sub("#", line) for line in text.splitlines()]
+
+
+def _log_blocks(text):
+    out = []
+    for blk in text.split("block {")[1:]:
+        out.append({k: float(v) for k, v in re.findall(r"(\w+) \{ (\S+)\s+\}", blk)})
+    return out
+
+
+def test_output_files_have_the_reference_layout(tmp_path):
+    """The driver's .log and per-block .json have the layout of the reference program's own
+    files for the same deck (tests/golden/ref_output, written by oracle/_ref/qwalk), numbers
+    aside; each .json record is that block alone: value = the block average, error 0, sigma =
+    the square root of the block variance (src/properties/Properties_average.cpp:449-520)."""
+    ref = os.path.join(GOLDEN, "ref_output")
+    d = os.path.join(GOLDEN, "decks", "h_atom")
+    for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb"]:
+        shutil.copy(os.path.join(d, f), tmp_path)
+    shutil.copy(os.path.join(ref, "h_dmc.deck"), tmp_path / "qw.run")
+    print("output files: h_dmc deck", flush=True)
+    out = subprocess.run([DRIVER, "qw.run"], cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    for ext in ["log", "json"]:
+        mine = open(tmp_path / ("qw.run." + ext)).read()
+        theirs = open(os.path.join(ref, "h_dmc." + ext)).read()
+        assert _skeleton(mine) == _skeleton(theirs), ext
+    blocks = _log_blocks(open(tmp_path / "qw.run.log").read())
+    recs = [json.loads(r.replace(":inf,", ":Infinity,")) for r in
+            open(tmp_path / "qw.run.json").read().split("<RS>") if r.strip()]
+    assert len(recs) == len(blocks) == 2
+    names = {"total_energy": "total_energy", "kinetic": "kinetic_energy", "potential": "potential_energy",
+             "nonlocal": "nonlocal_energy", "weight": "weight"}
+    for rec, blk in zip(recs, blocks):
+        for jname, lname in names.items():
+            p = rec["properties"][jname]
+            assert p["value"][0] == pytest.approx(blk[lname], rel=1e-5, abs=1e-9)
+            assert p["error"][0] == 0
+            assert p["sigma"][0] == pytest.approx(math.sqrt(blk[lname + "var"]), rel=1e-5, abs=1e-9)
