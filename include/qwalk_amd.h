@@ -76,6 +76,8 @@ typedef struct qw_method_desc {
   double max_weight;        /* DMC: MAX_POSS_WEIGHT */
   double branch_start_cutoff;
   int pp_stochastic;        /* 1 = the reference's default stochastic nonlocal evaluation */
+  int tmoves;               /* DMC: 0 = locality approximation only, 1 = TMOVES, 2 = TMOVESSC
+                               (Casula T-moves, src/method/Dmc_method.cpp:67-73,850-906) */
 } qw_method_desc;
 
 /* Engine for the SYSTEM / PSEUDO / TRIALFUNC sections of a QWalk deck.  INCLUDE and ORBFILE

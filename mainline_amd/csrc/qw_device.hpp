@@ -205,7 +205,8 @@ __device__ inline void philox(unsigned c0, unsigned c1, unsigned c2, unsigned c3
   out[3] = c3;
 }
 // Stream tags (counter word 3); the sweep number of a step sits in bits 8 and up.
-enum : unsigned { TAG_GAUSS1 = 0, TAG_ACC1 = 1, TAG_GAUSS2 = 2, TAG_ACC2 = 3, TAG_PP = 4, TAG_ROT = 5, TAG_BRANCH = 6 };
+enum : unsigned { TAG_GAUSS1 = 0, TAG_ACC1 = 1, TAG_GAUSS2 = 2, TAG_ACC2 = 3, TAG_PP = 4, TAG_ROT = 5, TAG_BRANCH = 6,
+                  TAG_TMOVE = 7 };
 __device__ inline double u53(unsigned a, unsigned b) {
   return ((a >> 5) * 67108864.0 + (b >> 6)) * (1.0 / 9007199254740992.0);
 }

@@ -111,7 +111,7 @@ struct qw_engine {
   int graph_mode = -1;     // QW_GRAPHS: 0 never, 1 always, otherwise for small step work
   unsigned* dstep = nullptr;  // device step counter read by the kernels
   hipGraphExec_t gx = nullptr;
-  int g_np = 0, g_nw = 0;
+  int g_np = 0, g_nw = 0, g_tm = 0, g_ppm = 0;
   double g_tau = 0;
   struct Ev {
     int kind;
@@ -130,6 +130,10 @@ struct qw_engine {
   int* lupiv = nullptr;
   int* luinfo = nullptr;
   int lucap = 0;
+  // T-moves: per-(atom, electron) negative nonlocal sums of the last local energy
+  double* tneg = nullptr;
+  int tnegcap = 0;
+  int tm_mode = 0;  // 0 off, 1 TMOVES, 2 TMOVESSC (for the current run)
 
   ~qw_engine() {
     (void)hipSetDevice(device);
@@ -379,10 +383,12 @@ struct qw_engine {
       for (void* p : walker_allocs) (void)hipFree(p);
     }
     walker_allocs.clear();
-    luw = nullptr;  // the solver buffers are walker allocations too
+    luw = nullptr;  // the solver and T-move buffers are walker allocations too
     lupiv = nullptr;
     luinfo = nullptr;
     lucap = 0;
+    tneg = nullptr;
+    tnegcap = 0;
     const size_t ne = S.ne, nsp = S.nsp;
     W.pos = walloc<double>((size_t)n * ne * 3);
     if (precision == QW_PREC_FP64) {
@@ -570,6 +576,22 @@ struct qw_engine {
     for (auto& x : tune_ev) pool.push_back(x.second.first), pool.push_back(x.second.second);
     tune_ev.clear();
   }
+  void ensure_tneg() {
+    if (tneg && tnegcap >= nw) return;
+    tneg = walloc<double>((size_t)nw * S.na * S.ne);
+    tnegcap = nw;
+  }
+  // Casula T-moves after the local energy of a DMC step (k_local wrote A.tneg)
+  void run_tmove(const LocalArgs& A) {
+    TmoveArgs t{tm_mode == 2 ? 1 : 0, A.pp_mode, A.tneg, A.seed, dstep, dctl};
+    const size_t lds = tmove_lds_bytes(S.ne, S.chunk_nf);
+    launch("k_tmove", [&] {
+      if (precision == QW_PREC_FP64)
+        hipLaunchKernelGGL(k_tmove<double>, dim3(nw), dim3(64), lds, stream, S, W, t);
+      else
+        hipLaunchKernelGGL(k_tmove<float>, dim3(nw), dim3(64), lds, stream, S, W, t);
+    });
+  }
   void run_local(const LocalArgs& A) {
     if (precision != QW_PREC_FP64) {
       mixed_local(*this, A);
@@ -680,7 +702,9 @@ struct qw_engine {
       A.mode = 2;
       A.use_dgl = 1;
       A.stepp = dstep;
+      A.tneg = tm_mode ? tneg : nullptr;
       run_local(A);
+      if (tm_mode) run_tmove(A);
       launch("k_step", [&] { hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, stream, dstep); });
     }
     launch("k_branch", [&] {
@@ -698,7 +722,7 @@ struct qw_engine {
   }
   // dmc_interval through a recorded graph (recorded again when np, tau or nw change)
   void dmc_interval_graph(int np, double tau, const LocalArgs& A) {
-    if (!gx || g_np != np || g_tau != tau || g_nw != nw) {
+    if (!gx || g_np != np || g_tau != tau || g_nw != nw || g_tm != tm_mode || g_ppm != A.pp_mode) {
       if (gx) {
         HIPCHK(hipGraphExecDestroy(gx));
         gx = nullptr;
@@ -722,6 +746,8 @@ struct qw_engine {
       g_np = np;
       g_tau = tau;
       g_nw = nw;
+      g_tm = tm_mode;
+      g_ppm = A.pp_mode;
     }
     HIPCHK(hipGraphLaunch(gx, stream));
   }
@@ -1084,6 +1110,7 @@ extern "C" qw_status qw_method_read(const char* deck_path, qw_method_desc* out) 
     out->max_weight = d.max_weight;
     out->branch_start_cutoff = d.branch_start_cutoff;
     out->pp_stochastic = d.pp_stochastic;
+    out->tmoves = d.tmoves;
     return QW_OK;
   } catch (std::exception& x) {
     g_create_error = x.what();
@@ -1110,6 +1137,8 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
     A.pp_mode = m->pp_stochastic ? 1 : 0;
     A.seed = e->seed;
     A.ctl = e->dctl;
+    e->tm_mode = dmc && e->S.has_pp ? (m->tmoves == 1 ? 1 : m->tmoves == 2 ? 2 : 0) : 0;
+    if (e->tm_mode) e->ensure_tneg();
     HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)nw * kBsum * sizeof(double), e->stream));
     HIPCHK(hipMemsetAsync(e->W.acc, 0, (size_t)nw * sizeof(int), e->stream));
     DmcCtl ctl{};

@@ -651,6 +651,7 @@ struct LocalArgs {
   double* out_lap;
   double* out_eparts;
   DmcCtl* ctl;
+  double* tneg;          // T-moves (mode 2): [nw][na][ne] sums of -vxx over the negative terms, or null
 };
 
 // TM: element type of the MO cache and inverse (double on the FP64 path, float on the mixed
@@ -849,6 +850,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
             want = strength > rr;
           }
           vnl += vl[P.L - 1];  // local channel for every electron
+          if (A.tneg && !want) A.tneg[((size_t)w * S.na + a) * ne + e] = 0.0;
         }
         unsigned long long mask = __ballot(want);
         while (mask) {
@@ -872,7 +874,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
           const double u_old = jastrow_e_val(S, ee, re, pos, lane);
           (void)gtmp;
           const double ivk = lane < n ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
-          double nl = 0;
+          double nl = 0, nneg = 0;
           for (int q = 0; q < P.aip; q++) {
             double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];
             double qy = P.q[q][0] * rot[1] + P.q[q][1] * rot[4] + P.q[q][2] * rot[7];
@@ -887,9 +889,14 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
             double ratio = Rq * exp(uq - u_old) * P.w[q];
             double ts = 0;
             for (int l = 0; l < P.L - 1; l++) ts += (2 * l + 1) * v[l] * legendre(cs, l);
-            nl += ts * ratio;
+            const double vxx = ts * ratio;
+            nl += vxx;
+            if (vxx < 0.0) nneg -= vxx;
           }
-          if (lane == 0) vnl += nl;
+          if (lane == 0) {
+            vnl += nl;
+            if (A.tneg) A.tneg[((size_t)w * S.na + a) * ne + ee] = nneg;
+          }
         }
       }
     }
@@ -943,6 +950,222 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
       bs[14] += wt * wt;
     }
   }
+}
+
+// ---------------------------------------------------------------- k_tmove
+struct TmoveArgs {
+  int sc;                // 0: one draw per walker (TMOVES), 1: one draw per electron (TMOVESSC)
+  int pp_mode;           // as in the k_local launch that wrote tneg
+  const double* tneg;    // [nw][na][ne] from k_local
+  unsigned long long seed;
+  const unsigned* stepp; // device step counter (the step k_local used)
+  const DmcCtl* ctl;     // tau
+};
+
+// The quadrature point of atom a that the draw t selects for electron e, scanning the atom's
+// negative terms from the cumulative value c (the order and terms of k_local's nonlocal loop);
+// writes the new position.  All lanes take part (LCAO over the MOs, Jastrow over electrons).
+template <typename TM>
+__device__ inline void tmove_target(const DevSys& S, const DevWalkers& W, int w, const double* pos, int e, int a,
+                                    double t, double c, double tau, const double rot[9], int pp_mode, double* chi,
+                                    int* flist, double np_out[3], int lane) {
+  const int nsp = S.nsp, s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0), n = s ? S.ndn : S.nup;
+  const QW_CONST PPType& P = cptr(S.pp)[cptr(S.atom_pp)[a]];
+  const QW_CONST double* ap = cptr(S.atom_pos) + 3 * a;
+  const double* re = pos + 3 * e;
+  const double od[3] = {re[0] - ap[0], re[1] - ap[1], re[2] - ap[2]};
+  const double orr = sqrt(od[0] * od[0] + od[1] * od[1] + od[2] * od[2]);
+  double v[kMaxPPL];
+  pp_radial(P, cptr(S.atom_Z)[a], orr, v);
+  if (pp_mode == 1) {
+    double strength = 0;
+    for (int l = 0; l < P.L - 1; l++) strength += 10.0 * (2 * l + 1) * fabs(v[l]);
+    strength = fmin(1.0, strength);
+    if (fabs(strength) > 0)
+      for (int l = 0; l < P.L - 1; l++) v[l] /= strength;
+  }
+  const double u_old = jastrow_e_val(S, e, re, pos, lane);
+  const double ivk = lane < n ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
+  int pick = -1;
+  for (int q = 0; q < P.aip && pick < 0; q++) {
+    const double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];
+    const double qy = P.q[q][0] * rot[1] + P.q[q][1] * rot[4] + P.q[q][2] * rot[7];
+    const double qz = P.q[q][0] * rot[2] + P.q[q][1] * rot[5] + P.q[q][2] * rot[8];
+    const double np_[3] = {re[0] + (qx * orr - od[0]), re[1] + (qy * orr - od[1]), re[2] + (qz * orr - od[2])};
+    const double nd[3] = {np_[0] - ap[0], np_[1] - ap[1], np_[2] - ap[2]};
+    const double nr = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+    const double cs = (nd[0] * od[0] + nd[1] * od[1] + nd[2] * od[2]) / (nr * orr);
+    const double phi = lcao_val(S, s, np_, chi, flist, lane);
+    const double Rq = wsum(ivk * phi);
+    const double uq = jastrow_e_val(S, e, np_, pos, lane);
+    const double ratio = Rq * exp(uq - u_old) * P.w[q];
+    double ts = 0;
+    for (int l = 0; l < P.L - 1; l++) ts += (2 * l + 1) * v[l] * legendre(cs, l);
+    const double vxx = ts * ratio;
+    if (vxx < 0.0) {
+      c += tau * (-vxx);
+      // the last negative term stands in when rounding leaves t just above the running sum
+      for (int d = 0; d < 3; d++) np_out[d] = np_[d];
+      if (t < c) pick = q;
+    }
+  }
+}
+
+// Casula T-moves after the local energy of a DMC step (src/method/Dmc_method.cpp:850-906,
+// src/system/Pseudopotential.cpp:371-389,752-763), one wavefront per walker.  From k_local's
+// per-(atom, electron) sums S_ae of -vxx over the negative nonlocal terms:
+//   TMOVES    one uniform x (1 + tau sum S_ae) picks at most one term of the whole list, in
+//             k_local's order (atom, electron, quadrature point);
+//   TMOVESSC  per electron, one uniform x (1 + tau sum_a S_ae) over that electron's terms.
+// The chosen terms are found again at the unmoved configuration (the atom from the sums, the
+// quadrature point by recomputing that atom's terms), then applied in electron order as
+// accepted moves: Sherman-Morrison on the inverse, the MO cache row, log|det| and sign.
+template <typename TM>
+__global__ __launch_bounds__(64) void k_tmove(DevSys S, DevWalkers W, TmoveArgs A) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x, w = blockIdx.x;
+  if (w >= W.nw) return;
+  const int ne = S.ne, na = S.na, nsp = S.nsp;
+  double* pos = lds;
+  double* chi = pos + 3 * ne;
+  double* newp = chi + 5 * S.chunk_nf;
+  double* selt = newp + 3 * ne;
+  double* selc = selt + ne;
+  double* phib = selc + ne;
+  double* rowk = phib + kWave;
+  double* xs = rowk + kWave;
+  int* sela = (int*)(xs + ne);
+  int* flist = sela + ne;
+  for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  const unsigned step = *A.stepp;
+  const double tau = A.ctl->tau;
+  const double* T = A.tneg + (size_t)w * na * ne;
+  auto has_pp = [&](int a) { return cptr(S.atom_pp)[a] >= 0; };
+  for (int e = lane; e < ne; e += kWave) sela[e] = -1;
+  __syncthreads();
+  if (A.sc) {
+    for (int e = lane; e < ne; e += kWave) {
+      double se = 0;
+      for (int a = 0; a < na; a++)
+        if (has_pp(a)) se += tau * T[(size_t)a * ne + e];
+      const double t = uniform(A.seed, (unsigned)w, step, (unsigned)e, TAG_TMOVE) * (1.0 + se);
+      if (t > 1.0) {
+        double c = 1.0;
+        for (int a = 0; a < na; a++) {
+          const double x = has_pp(a) ? tau * T[(size_t)a * ne + e] : 0.0;
+          if (x > 0) {
+            sela[e] = a;
+            selc[e] = c;
+            if (t < c + x) break;
+          }
+          c += x;
+        }
+        selt[e] = t;
+      }
+    }
+  } else {
+    double part = 0;
+    for (int a = 0; a < na; a++)
+      if (has_pp(a))
+        for (int e = lane; e < ne; e += kWave) part += tau * T[(size_t)a * ne + e];
+    const double stot = wsum(part);
+    const double t = uniform(A.seed, (unsigned)w, step, 0u, TAG_TMOVE) * (1.0 + stot);
+    if (t > 1.0) {
+      // the atom, from per-atom totals in order; then the electron, scanning that atom's sums
+      double c = 1.0;
+      int as = -1;
+      double cs = 1.0;
+      for (int a = 0; a < na; a++) {
+        if (!has_pp(a)) continue;
+        double xa = 0;
+        for (int e = lane; e < ne; e += kWave) xa += tau * T[(size_t)a * ne + e];
+        xa = wsum(xa);
+        if (xa > 0) {
+          as = a;
+          cs = c;
+          if (t < c + xa) break;
+        }
+        c += xa;
+      }
+      if (as >= 0) {
+        __syncthreads();
+        for (int e = lane; e < ne; e += kWave) xs[e] = tau * T[(size_t)as * ne + e];
+        __syncthreads();
+        if (lane == 0) {
+          int es = -1;
+          double ce = cs, c2 = cs;
+          for (int i = 0; i < ne; i++) {
+            const double x = xs[i];
+            if (x > 0) {
+              es = i;  // the last candidate stands in when rounding leaves t above the sum
+              ce = c2;
+              if (t < c2 + x) break;
+            }
+            c2 += x;
+          }
+          if (es >= 0) {
+            sela[es] = as;
+            selc[es] = ce;
+            selt[es] = t;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double rot[9];
+  random_rotation(A.seed, (unsigned)w, step, rot);
+  // targets at the unmoved configuration
+  int* flst = flist;
+  for (int e = 0; e < ne; e++) {
+    if (sela[e] < 0) continue;
+    double np_[3];
+    tmove_target<TM>(S, W, w, pos, e, sela[e], selt[e], selc[e], tau, rot, A.pp_mode, chi, flst, np_, lane);
+    if (lane == 0)
+      for (int d = 0; d < 3; d++) newp[3 * e + d] = np_[d];
+  }
+  __syncthreads();
+  // apply in electron order
+  TM* invw = (sizeof(TM) == sizeof(double) ? (TM*)W.inv : (TM*)W.invf) + (size_t)w * 2 * nsp * nsp;
+  TM* mow = (sizeof(TM) == sizeof(double) ? (TM*)W.mo : (TM*)W.mof) + (size_t)w * ne * 5 * nsp;
+  for (int e = 0; e < ne; e++) {
+    if (sela[e] < 0) continue;
+    const int s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0), n = s ? S.ndn : S.nup;
+    const double np_[3] = {newp[3 * e], newp[3 * e + 1], newp[3 * e + 2]};
+    double phi[5];
+    lcao_lap(S, s, np_, chi, flist, phi, lane);
+    TM* ig = invw + (size_t)s * nsp * nsp;
+    const double rk = lane < n ? (double)ig[(size_t)k * nsp + lane] : 0.0;
+    const double R = wsum(rk * (lane < n ? phi[0] : 0.0));
+    phib[lane] = lane < n ? phi[0] : 0.0;
+    rowk[lane] = rk;
+    __syncthreads();
+    if (lane < n) {
+      TM* row = ig + (size_t)lane * nsp;
+      double tk = 0;
+      for (int j = 0; j < n; j++) tk += (double)row[j] * phib[j];
+      if (lane == k) {
+        for (int j = 0; j < n; j++) row[j] = (TM)(rowk[j] / R);
+      } else {
+        const double pk = -tk / R;
+        for (int j = 0; j < n; j++) row[j] = (TM)((double)row[j] + rowk[j] * pk);
+      }
+      for (int c = 0; c < 5; c++) mow[((size_t)e * 5 + c) * nsp + lane] = (TM)phi[c];
+    }
+    if (lane == 0) {
+      W.logdet[2 * w + s] += log(fabs(R));
+      if (R < 0) W.sgn[2 * w + s] = -W.sgn[2 * w + s];
+    }
+    if (lane < 3) pos[3 * e + lane] = np_[lane];
+    __syncthreads();
+  }
+  for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
+}
+
+// Dynamic LDS of k_tmove.
+inline size_t tmove_lds_bytes(int ne, int chunk_nf) {
+  return ((size_t)3 * ne + 5 * chunk_nf + 3 * ne + 3 * ne + 2 * kWave) * sizeof(double) +
+         ((size_t)ne + chunk_nf) * sizeof(int) + 16;
 }
 
 // ---------------------------------------------------------------- k_branch

@@ -766,8 +766,10 @@ MethodDesc read_method(const std::string& path) {
     ivalue(m, d.feedback_interval, "FEEDBACK_INTERVAL");
     dvalue(m, d.feedback, "FEEDBACK");
     dvalue(m, d.branch_start_cutoff, "BRANCH_START_CUTOFF");
-    if (keyword(m, "TMOVES") || keyword(m, "TMOVESSC") || keyword(m, "PURE_DMC") || keyword(m, "ALL_ELECTRON"))
-      throw std::runtime_error("DMC options TMOVES / PURE_DMC / ALL_ELECTRON are not supported");
+    // TMOVES takes precedence when both are given (src/method/Dmc_method.cpp:70-73,869)
+    d.tmoves = keyword(m, "TMOVES") ? 1 : keyword(m, "TMOVESSC") ? 2 : 0;
+    if (keyword(m, "PURE_DMC") || keyword(m, "ALL_ELECTRON"))
+      throw std::runtime_error("DMC options PURE_DMC / ALL_ELECTRON are not supported");
   } else {
     throw std::runtime_error("METHOD must be VMC or DMC, found " + m[0]);
   }

@@ -99,6 +99,7 @@ struct MethodDesc {
   double max_weight = 7.0;
   double branch_start_cutoff = 10.0;
   int pp_stochastic = 1;
+  int tmoves = 0;  // 1 TMOVES, 2 TMOVESSC
 };
 
 // Throws std::runtime_error with a message on bad or unsupported input.

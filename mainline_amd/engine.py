@@ -29,7 +29,7 @@ class MethodDesc(ctypes.Structure):
                 ("nstep", ctypes.c_int), ("ndecorr", ctypes.c_int), ("timestep", ctypes.c_double),
                 ("feedback_interval", ctypes.c_int), ("feedback", ctypes.c_double),
                 ("max_weight", ctypes.c_double), ("branch_start_cutoff", ctypes.c_double),
-                ("pp_stochastic", ctypes.c_int)]
+                ("pp_stochastic", ctypes.c_int), ("tmoves", ctypes.c_int)]
 
 
 class ServerOpts(ctypes.Structure):
@@ -113,9 +113,10 @@ def read_method(deck):
 
 
 def method(kind, nconfig, nblock, nstep, timestep, ndecorr=2, feedback_interval=5, feedback=1.0,
-           max_weight=7.0, branch_start_cutoff=10.0, pp_stochastic=1):
+           max_weight=7.0, branch_start_cutoff=10.0, pp_stochastic=1, tmoves=0):
+    """tmoves (DMC): 0 = off, 1 = TMOVES, 2 = TMOVESSC (Casula T-moves)."""
     return MethodDesc(1 if kind.upper() == "DMC" else 0, nconfig, nblock, nstep, ndecorr, timestep,
-                      feedback_interval, feedback, max_weight, branch_start_cutoff, pp_stochastic)
+                      feedback_interval, feedback, max_weight, branch_start_cutoff, pp_stochastic, tmoves)
 
 
 class Engine:

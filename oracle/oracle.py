@@ -45,6 +45,10 @@ def lib():
         L.orc_inverse_update_row.restype = ctypes.c_double
         L.orc_inverse_update_row.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.orc_philox.argtypes = [ctypes.POINTER(ctypes.c_uint)] * 3
+        L.orc_tmoves.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp]
+        L.orc_tmove_step.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, _dp, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint]
         L.orc_run.restype = ctypes.c_double
         L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, _dp, _dp]
         _lib = L
@@ -61,7 +65,7 @@ class RunOpts(ctypes.Structure):
                 ("seed", ctypes.c_ulonglong), ("pp_stochastic", ctypes.c_int),
                 ("feedback_interval", ctypes.c_int), ("feedback", ctypes.c_double),
                 ("max_weight", ctypes.c_double), ("branch_start_cutoff", ctypes.c_double),
-                ("nthreads", ctypes.c_int), ("step0", ctypes.c_uint)]
+                ("nthreads", ctypes.c_int), ("step0", ctypes.c_uint), ("tmoves", ctypes.c_int)]
 
 
 class Oracle:
@@ -113,6 +117,27 @@ class Oracle:
         lib().orc_parm_deriv(self.h, K, _p(pos), _p(g), _p(gd), _p(vg))
         return dict(gradient=g[:, :n], gradderiv=gd[:, :n], val_gradient=vg)
 
+    def tmoves(self, pos, rot=None, pp_mode=0, pp_rand=None):
+        """T-move candidates of one configuration pos[Ne][3]: electron[M], vxx[M],
+        displacement[M][3] in the reference's order, and the nonlocal energy without them."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        r = None if rot is None else np.ascontiguousarray(rot, dtype=np.float64)
+        pr = None if pp_rand is None else np.ascontiguousarray(pp_rand, dtype=np.float64)
+        cap = max(1, self.natoms * self.ne * 32)
+        e = (ctypes.c_int * cap)()
+        vxx, disp, vnl = np.zeros(cap), np.zeros((cap, 3)), np.zeros(1)
+        m = lib().orc_tmoves(self.h, _p(pos), _p(r), pp_mode, _p(pr), cap, e, _p(vxx), _p(disp), _p(vnl))
+        assert m >= 0
+        return dict(e=np.array(e[:m], dtype=np.int64), vxx=vxx[:m], disp=disp[:m], vnl=vnl[0])
+
+    def tmove_step(self, pos, tau, sc, seed, walker, step, rot=None, pp_mode=0, pp_rand=None):
+        """One T-move step of configuration pos[Ne][3]; returns (new positions, electrons moved)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+        r = None if rot is None else np.ascontiguousarray(rot, dtype=np.float64)
+        pr = None if pp_rand is None else np.ascontiguousarray(pp_rand, dtype=np.float64)
+        n = lib().orc_tmove_step(self.h, _p(pos), _p(r), pp_mode, _p(pr), tau, 1 if sc else 0, seed, walker, step)
+        return pos, n
+
     def mo(self, spin, pts):
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         nmo = self.nup if spin == 0 else self.ndn
@@ -144,7 +169,7 @@ class Oracle:
 
     def run(self, pos, method, nblock, nstep, timestep, seed, weights=None, ndecorr=2,
             pp_stochastic=1, nthreads=1, step0=0, state=None, feedback_interval=5,
-            feedback=1.0, max_weight=7.0, branch_start_cutoff=10.0):
+            feedback=1.0, max_weight=7.0, branch_start_cutoff=10.0, tmoves=0):
         pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
         nw = pos.shape[0]
         w = np.ones(nw) if weights is None else np.ascontiguousarray(weights, dtype=np.float64).copy()
@@ -152,7 +177,7 @@ class Oracle:
         st = np.zeros(4) if state is None else np.ascontiguousarray(state, dtype=np.float64)
         o = RunOpts(1 if method.upper() == "DMC" else 0, nw, nblock, nstep, ndecorr, timestep,
                     seed, pp_stochastic, feedback_interval, feedback, max_weight,
-                    branch_start_cutoff, nthreads, step0)
+                    branch_start_cutoff, nthreads, step0, tmoves)
         acc = lib().orc_run(self.h, ctypes.byref(o), _p(pos), _p(w), _p(eb), _p(st))
         return dict(pos=pos, weights=w, eblock=eb, acceptance=acc, state=st)
 
@@ -191,7 +216,7 @@ def parse_golden(text):
         if not f:
             continue
         if f[0] == "config":
-            cur = dict(elec=[], parm=[], pdgrad=[], pdgd=[], pdvg=[])
+            cur = dict(elec=[], parm=[], pdgrad=[], pdgd=[], pdvg=[], tmove=[])
             out.append(cur)
         elif f[0] == "elec":
             cur["elec"].append([float(x) for x in f[2:6]])
@@ -201,6 +226,8 @@ def parse_golden(text):
             cur["pdgd"].append([float(x) for x in f[3:7]])
         elif f[0] == "pdvg":
             cur["pdvg"].append([float(x) for x in f[2:5]])
+        elif f[0] == "tmove":
+            cur["tmove"].append([float(x) for x in f[1:6]])
         elif f[0] in ("pdok", "nparm"):
             cur[f[0]] = int(f[1])
         elif f[0] == "sign":
@@ -215,4 +242,5 @@ def parse_golden(text):
         c["pdgrad"] = np.array(c["pdgrad"])
         c["pdgd"] = np.array(c["pdgd"]).reshape(npar, ne, 4)
         c["pdvg"] = np.array(c["pdvg"]).reshape(-1, 3)
+        c["tmove"] = np.array(c["tmove"]).reshape(-1, 5)  # e, vxx, displacement
     return out

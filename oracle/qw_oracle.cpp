@@ -1438,10 +1438,21 @@ struct EvalOut {
   std::vector<double> grad, lap;
 };
 
+// A T-move candidate (src/system/Pseudopotential.h:60-65): electron e displaced by disp onto
+// a quadrature point, with its negative nonlocal matrix element vxx.
+struct Tmove {
+  int e;
+  double vxx;
+  double disp[3];
+};
+
 // Nonlocal pseudopotential for one walker (src/system/Pseudopotential.cpp:596-815).
 // qrot: rotated quadrature points per PP atom type; rnd: uniforms [natoms][ne] or NULL.
+// With tmoves set (Tmoves::negative_tmove, src/system/Pseudopotential.cpp:752-763), terms with
+// vxx < 0 are left out of the sum and appended to *tmoves in the order they are met (atom,
+// electron, quadrature point).
 double pp_nonlocal(const System& S, Walker& w, const double* rot, int stochastic, const double* rnd,
-                   std::vector<double>& tmp, JScratch& js) {
+                   std::vector<double>& tmp, JScratch& js, std::vector<Tmove>* tmoves = nullptr) {
   int ne = S.ne(), na = (int)S.atoms.size();
   double tot = 0;
   std::vector<double> phi;
@@ -1501,7 +1512,13 @@ double pp_nonlocal(const System& S, Walker& w, const double* rot, int stochastic
           double ratio = R * exp(u_new - u_old) * P.qw[i];
           double ts = 0;
           for (int l = 0; l < L - 1; l++) ts += (2 * l + 1) * vl[l] * legendre(rdr, l);
-          nonlocal += ts * ratio;
+          const double vxx = ts * ratio;
+          if (tmoves && vxx < 0.0) {
+            Tmove t{e, vxx, {q[0] * old[0] - old[2], q[1] * old[0] - old[3], q[2] * old[0] - old[4]}};
+            tmoves->push_back(t);
+          } else {
+            nonlocal += vxx;
+          }
         }
       }
       tot += vl[L - 1] + nonlocal;
@@ -1556,7 +1573,7 @@ inline void philox(const unsigned* ctr, const unsigned* key, unsigned* out) {
 }
 // RNG streams of the walker engine: key = seed, counter = (walker, step, index, tag).
 // The sweep number within a step goes into bits 8.. of the tag.
-enum { TAG_GAUSS1 = 0, TAG_ACC1 = 1, TAG_GAUSS2 = 2, TAG_ACC2 = 3, TAG_PP = 4, TAG_ROT = 5, TAG_BRANCH = 6 };
+enum { TAG_GAUSS1 = 0, TAG_ACC1 = 1, TAG_GAUSS2 = 2, TAG_ACC2 = 3, TAG_PP = 4, TAG_ROT = 5, TAG_BRANCH = 6, TAG_TMOVE = 7 };
 inline double u53(unsigned a, unsigned b) {
   return ((a >> 5) * 67108864.0 + (b >> 6)) * (1.0 / 9007199254740992.0);
 }
@@ -1725,8 +1742,12 @@ struct Energy {
   double total() const { return ke + vloc + vnl; }
 };
 
+// With tmoves set, the nonlocal part is evaluated as Pseudopotential::calcNonlocTmove does
+// (negative terms collected into *tmoves) and then restored as Dmc_method::doTmove does,
+// nonlocal -= (sum - 1) / tau with sum = 1 - tau * sum(vxx) (src/method/Dmc_method.cpp:850-866).
 Energy local_energy(const System& S, Walker& w, unsigned long long seed, unsigned wid, unsigned step,
-                    int pp_stochastic, std::vector<double>& tmp, JScratch& js) {
+                    int pp_stochastic, std::vector<double>& tmp, JScratch& js, std::vector<Tmove>* tmoves = nullptr,
+                    double tau = 0.0) {
   Energy E;
   E.ke = kinetic(S, w, nullptr, nullptr, js);
   E.vloc = coulomb(S, w.pos.data());
@@ -1741,9 +1762,66 @@ Energy local_energy(const System& S, Walker& w, unsigned long long seed, unsigne
       for (int a = 0; a < na; a++)
         for (int e = 0; e < ne; e++) rnd[a * ne + e] = uniform(seed, wid, step, (unsigned)(a * ne + e), TAG_PP);
     }
-    E.vnl = pp_nonlocal(S, w, rot, pp_stochastic, pp_stochastic ? rnd.data() : nullptr, tmp, js);
+    E.vnl = pp_nonlocal(S, w, rot, pp_stochastic, pp_stochastic ? rnd.data() : nullptr, tmp, js, tmoves);
+    if (tmoves) {
+      double sum = 1;
+      for (const Tmove& m : *tmoves) sum -= tau * m.vxx;
+      E.vnl -= (sum - 1) / tau;
+    }
   }
   return E;
+}
+
+// Casula T-moves after the local energy (src/method/Dmc_method.cpp:867-905).  sc = 0 (TMOVES):
+// one draw scaled by sum = 1 - tau * sum(vxx) chooses at most one candidate of the whole list,
+// in list order; sc = 1 (TMOVESSC): the same per electron over its own candidates.  A chosen
+// electron is translated by the candidate's displacement and the walker's inverse, MO cache and
+// log|det| are updated as for an accepted move.  Uniforms: TAG_TMOVE, index 0 (TMOVES) or the
+// electron (TMOVESSC).
+int do_tmove(const System& S, Walker& w, const std::vector<Tmove>& tm, double tau, int sc, unsigned long long seed,
+             unsigned wid, unsigned step, std::vector<double>& tmp, JScratch& js, Point& P) {
+  auto move = [&](const Tmove& m) {
+    for (int d = 0; d < 3; d++) P.r[d] = w.pos[3 * m.e + d] + m.disp[d];
+    eval_point(S, w, m.e, P, 0.0, tmp, js);
+    accept_point(S, w, m.e, P);
+  };
+  int moved = 0;
+  if (!sc) {
+    double sum = 1;
+    for (const Tmove& m : tm) sum -= tau * m.vxx;
+    const double rand = uniform(seed, wid, step, 0, TAG_TMOVE) * sum;
+    sum = 1;
+    if (rand > sum) {
+      for (const Tmove& m : tm) {
+        sum -= tau * m.vxx;
+        if (rand < sum) {
+          move(m);
+          moved++;
+          break;
+        }
+      }
+    }
+    return moved;
+  }
+  for (int e = 0; e < S.ne(); e++) {
+    double sum_e = 1;
+    for (const Tmove& m : tm)
+      if (m.e == e) sum_e -= tau * m.vxx;
+    const double rand = uniform(seed, wid, step, (unsigned)e, TAG_TMOVE) * sum_e;
+    double sel = 1;
+    if (rand > sel) {
+      for (const Tmove& m : tm) {
+        if (m.e != e) continue;
+        sel -= tau * m.vxx;
+        if (rand < sel) {
+          move(m);
+          moved++;
+          break;
+        }
+      }
+    }
+  }
+  return moved;
 }
 
 // DMC weight factor (src/method/Dmc_method.cpp:801-824).
@@ -1972,6 +2050,51 @@ extern "C" int orc_eval(orc_system* o, int K, const double* pos, const double* r
   return 0;
 }
 
+// T-move candidates of fixed configurations (deterministic or stochastic PP as in orc_eval).
+// Returns the number of candidates of configuration k written to e_out[cap], vxx_out[cap],
+// disp_out[cap][3] (-1 if cap is too small); vnl_out = the nonlocal energy without them.
+extern "C" int orc_tmoves(orc_system* o, const double* pos, const double* rot, int pp_mode, const double* pp_rand,
+                          int cap, int* e_out, double* vxx_out, double* disp_out, double* vnl_out) {
+  const System& S = o->S;
+  int ne = S.ne();
+  std::vector<double> tmp;
+  JScratch js;
+  Walker w;
+  walker_init(S, w);
+  w.pos.assign(pos, pos + (size_t)ne * 3);
+  walker_refresh(S, w, tmp);
+  std::vector<Tmove> tl;
+  *vnl_out = S.pp.empty() ? 0.0 : pp_nonlocal(S, w, rot, pp_mode, pp_rand, tmp, js, &tl);
+  if ((int)tl.size() > cap) return -1;
+  for (size_t t = 0; t < tl.size(); t++) {
+    e_out[t] = tl[t].e;
+    vxx_out[t] = tl[t].vxx;
+    for (int d = 0; d < 3; d++) disp_out[3 * t + d] = tl[t].disp[d];
+  }
+  return (int)tl.size();
+}
+
+// T-move step of one walker at a fixed configuration (tests of the selection rule): the
+// candidates at pos, then do_tmove with the given seed / walker / step counters.  pos is
+// updated in place; returns the number of electrons moved.
+extern "C" int orc_tmove_step(orc_system* o, double* pos, const double* rot, int pp_mode, const double* pp_rand,
+                              double tau, int sc, unsigned long long seed, unsigned wid, unsigned step) {
+  const System& S = o->S;
+  int ne = S.ne();
+  std::vector<double> tmp;
+  JScratch js;
+  Walker w;
+  walker_init(S, w);
+  w.pos.assign(pos, pos + (size_t)ne * 3);
+  walker_refresh(S, w, tmp);
+  std::vector<Tmove> tl;
+  if (!S.pp.empty()) pp_nonlocal(S, w, rot, pp_mode, pp_rand, tmp, js, &tl);
+  Point P;
+  int moved = do_tmove(S, w, tl, tau, sc, seed, wid, step, tmp, js, P);
+  std::copy(w.pos.begin(), w.pos.end(), pos);
+  return moved;
+}
+
 // Number of linear Jastrow parameters, their values (may be NULL).
 extern "C" int orc_nparm(orc_system* o, double* values) {
   const System& S = o->S;
@@ -2172,11 +2295,15 @@ extern "C" double orc_run(orc_system* o, const orc_run_opts* opt, double* pos, d
               a += move_electron(S, W[i], e, tau, dmc ? 1 : 2, dmc ? 1 : 0, opt->seed, (unsigned)i, cur,
                                  (unsigned)sw2, tmps[t], jss[t], ptss[t].data()) > 0;
           acc[i] = a;
-          Energy E = local_energy(S, W[i], opt->seed, (unsigned)i, cur, opt->pp_stochastic, tmps[t], jss[t]);
+          const bool tmv = dmc && opt->tmoves && !S.pp.empty();
+          std::vector<Tmove> tl;
+          Energy E = local_energy(S, W[i], opt->seed, (unsigned)i, cur, opt->pp_stochastic, tmps[t], jss[t],
+                                  tmv ? &tl : nullptr, tau);
           enew[i] = E.total();
           ekin[i] = E.ke;
           evl[i] = E.vloc;
           evn[i] = E.vnl;
+          if (tmv) do_tmove(S, W[i], tl, tau, opt->tmoves == 2, opt->seed, (unsigned)i, cur, tmps[t], jss[t], ptss[t][0]);
         });
         for (int i = 0; i < nw; i++) {
           acc_chunk += acc[i];

@@ -90,7 +90,17 @@ typedef struct orc_run_opts {
   double branch_start_cutoff;
   int nthreads;
   unsigned int step0;  /* first global step index (RNG counter) */
+  int tmoves;          /* DMC: 0 none, 1 TMOVES, 2 TMOVESSC */
 } orc_run_opts;
+
+/* T-move candidates of one configuration pos[Ne][3] (Pseudopotential::calcNonlocTmove):
+ * electron, vxx (< 0) and displacement per candidate, in the reference's order; vnl_out = the
+ * nonlocal energy without them.  Returns the count, or -1 if it exceeds cap. */
+int orc_tmoves(orc_system* s, const double* pos, const double* rot, int pp_mode, const double* pp_rand,
+               int cap, int* e_out, double* vxx_out, double* disp_out, double* vnl_out);
+/* One T-move step (Dmc_method::doTmove) of one configuration, in place; returns electrons moved. */
+int orc_tmove_step(orc_system* s, double* pos, const double* rot, int pp_mode, const double* pp_rand,
+                   double tau, int sc, unsigned long long seed, unsigned wid, unsigned step);
 
 double orc_run(orc_system* s, const orc_run_opts* o, double* pos, double* weights,
                double* eblock, double* state /* [4]: etrial, eref, teff, cutoff sigma */);

@@ -6,8 +6,10 @@
 // carries over) and prints, with %.17g,
 //   ln|Psi|, sign, per electron grad(Psi)/Psi (3) and lap(Psi)/Psi,
 //   kinetic energy, local (Coulomb) potential, nonlocal pseudopotential energy,
-//   and the linear-parameter derivatives of Wavefunction::getParmDeriv (gradient, gradderiv,
-//   val_gradient) with the parameter values of Wavefunction_data::getVarParms.
+//   the linear-parameter derivatives of Wavefunction::getParmDeriv (gradient, gradderiv,
+//   val_gradient) with the parameter values of Wavefunction_data::getVarParms,
+//   and the T-move candidates of Pseudopotential::calcNonlocTmove (electron, displacement,
+//   vxx) with the nonlocal energy it returns (the terms with vxx >= 0 only).
 // The pseudopotential is made deterministic with the identity quadrature rotation, as the
 // reference allows (src/system/Pseudopotential.h:80 setDeterministic,
 // src/system/Pseudopotential.cpp:828 rotateQuadrature).
@@ -122,6 +124,14 @@ int main(int argc, char** argv) {
     vnl = 0.0;
     psp->calcNonlocWithTest(wfdata, sys, sample, wf, test, vnl);
     print_config(k, ne, val, lap, kin(0), vloc, vnl(0));
+    // T-move candidates (src/system/Pseudopotential.cpp:371-389,752-763)
+    vector<Tmove> tmov;
+    Array1<doublevar> vnl_t(nwf);
+    psp->calcNonlocTmove(wfdata, sys, sample, wf, vnl_t, tmov);
+    printf("vnl_tmove %.17g\n", vnl_t(0));
+    for (size_t t = 0; t < tmov.size(); t++)
+      printf("tmove %d %.17g %.17g %.17g %.17g\n", tmov[t].e, tmov[t].vxx, tmov[t].pos(0), tmov[t].pos(1),
+             tmov[t].pos(2));
     // linear-parameter derivatives (Wavefunction::getParmDeriv) and the parameter values
     Parm_deriv_return pd;
     pd.need_hessian = 0;

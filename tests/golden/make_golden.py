@@ -8,7 +8,9 @@ as an .npz fixture:
 
   pos[K][Ne][3], logpsi[K], sign[K], grad[K][Ne][3], lap[K][Ne], kinetic[K], vloc[K],
   vnonloc[K], and from Wavefunction::getParmDeriv: parms[np] (getVarParms),
-  pd_gradient[K][np], pd_gradderiv[K][np][Ne][4], pd_valgrad[K][Ne][3]
+  pd_gradient[K][np], pd_gradderiv[K][np][Ne][4], pd_valgrad[K][Ne][3],
+  and from Pseudopotential::calcNonlocTmove: vnl_tmove[K] and the candidate list
+  tmove[M][6] = configuration, electron, vxx, displacement (3), in the reference's order
   (+ orb_sha256 for generated decks, to detect generator drift)
 
 Cases:
@@ -21,7 +23,7 @@ Cases:
             9G_pyscf 15G), 12 electrons
 
 Output files: the reference program (oracle/_ref/qwalk) runs a short DMC of the tests/h deck
-and its block log and per-block JSON are kept as ref_output/h_dmc.log and h_dmc.json (the
+and its block log and per-block JSON are kept as ref_output/h_dmc.qwlog and h_dmc.json (the
 layout of the files the qwalk_amd driver writes is checked against them).
 
 Usage: python tests/golden/make_golden.py   (from the repository root)
@@ -95,6 +97,8 @@ def run_ref(deck_dir, deck, pos):
         pd_gradient=np.array([c["pdgrad"] for c in g]),
         pd_gradderiv=np.array([c["pdgd"] for c in g]),
         pd_valgrad=np.array([c["pdvg"] for c in g]),
+        vnl_tmove=np.array([c["vnl_tmove"] for c in g]),
+        tmove=np.array([[k] + list(t) for k, c in enumerate(g) for t in c["tmove"]]).reshape(-1, 6),
     ) if K else None
 
 
@@ -154,8 +158,9 @@ def ref_output_files(tmp):
     out = os.path.join(OUT, "ref_output")
     os.makedirs(out, exist_ok=True)
     open(os.path.join(out, "h_dmc.deck"), "w").write(OUTPUT_DECK)
-    for ext in ["log", "json"]:
-        shutil.copy(os.path.join(d, "qw.run." + ext), os.path.join(out, "h_dmc." + ext))
+    # the block log is kept as .qwlog (".log" files are treated as run logs elsewhere)
+    for ext, keep in [("log", "qwlog"), ("json", "json")]:
+        shutil.copy(os.path.join(d, "qw.run." + ext), os.path.join(out, "h_dmc." + keep))
 
 
 def main():

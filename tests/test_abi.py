@@ -52,3 +52,12 @@ def test_method_read_applies_reference_defaults(tmp_path):
     p.write_text("METHOD { VMC NCONFIG 256 TIMESTEP 0.5 NBLOCK 50 NSTEP 20 }\n")
     m = E.read_method(str(p))
     assert (m.method, m.nconfig, m.nblock, m.nstep, m.ndecorr, m.timestep) == (0, 256, 50, 20, 2, 0.5)
+
+
+def test_method_read_tmoves_keywords(tmp_path):
+    """TMOVES (one draw per walker) and TMOVESSC (per electron); TMOVES wins when both are
+    given, as doTmove tests it first (src/method/Dmc_method.cpp:70-73,869)."""
+    p = tmp_path / "m.dmc"
+    for text, want in [("", 0), ("TMOVES", 1), ("TMOVESSC", 2), ("TMOVESSC TMOVES", 1)]:
+        p.write_text("METHOD { DMC TIMESTEP 0.01 %s }\n" % text)
+        assert E.read_method(str(p)).tmoves == want

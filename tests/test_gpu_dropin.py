@@ -60,7 +60,7 @@ def _skeleton(text):
 def _log_blocks(text):
     out = []
     for blk in text.split("block {")[1:]:
-        out.append({k: float(v) for k, v in re.findall(r"(\w+) \{ (\S+)\s+\}", blk)})
+        out.append({k: float(v) for k, v in re.findall(r"(\w+) \{ ([-+0-9.eE]+|[-+]?nan|[-+]?inf)\s+\}", blk)})
     return out
 
 
@@ -78,9 +78,9 @@ def test_output_files_have_the_reference_layout(tmp_path):
     out = subprocess.run([DRIVER, "qw.run"], cwd=tmp_path, capture_output=True, text=True, timeout=200)
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0
-    for ext in ["log", "json"]:
+    for ext, kept in [("log", "qwlog"), ("json", "json")]:
         mine = open(tmp_path / ("qw.run." + ext)).read()
-        theirs = open(os.path.join(ref, "h_dmc." + ext)).read()
+        theirs = open(os.path.join(ref, "h_dmc." + kept)).read()
         assert _skeleton(mine) == _skeleton(theirs), ext
     blocks = _log_blocks(open(tmp_path / "qw.run.log").read())
     recs = [json.loads(r.replace(":inf,", ":Infinity,")) for r in
