@@ -53,17 +53,12 @@ def run_gpu(deck, mol, precision, nw, steps, warmup, tau):
     log("[%s] warmup %d steps" % (precision, warmup))
     eng.run(E.method("DMC", nw, nblock=1, nstep=max(1, warmup), timestep=tau))
     log("[%s] timed %d steps" % (precision, steps))
-    kt0 = eng.kernel_times()
     t0 = time.perf_counter()
     res = eng.run(E.method("DMC", nw, nblock=1, nstep=steps, timestep=tau))
     dt = time.perf_counter() - t0
-    kt1 = eng.kernel_times()
-    # per-kernel average over the timed steps only (event times on the engine's stream)
-    kt = {}
-    for k, (a1, c1) in kt1.items():
-        a0, c0 = kt0.get(k, (0.0, 0))
-        if c1 > c0:
-            kt[k] = [(a1 * c1 - a0 * c0) / (c1 - c0), int(c1 - c0)]
+    # per-kernel average and launch count over the timed steps only (event times on the
+    # engine's stream; the engine restarts its kernel times at every run)
+    kt = {k: [a, int(c)] for k, (a, c) in eng.kernel_times().items() if c > 0}
     moves = nw * mol.nelec * steps
     return dict(moves_per_s=moves / dt, seconds=dt, ms_per_step=1e3 * dt / steps, energy=float(res["eblock"][0, 0]),
                 acceptance=float(res["acceptance"]), kernels=kt, tuning=eng.tuning())
@@ -286,8 +281,8 @@ def main():
     p_acc = R["acceptance"] if R["acceptance"] > 0 else 0.99
     kname = "k_sweep" if "k_sweep" in R["kernels"] else next(iter(R["kernels"]))
     avg_ms = R["kernels"][kname][0]
-    # moves per sweep launch: a spin channel's electrons (FP64), the refresh interval (mixed)
-    moves_per_launch = a.walkers * (ns if head == "fp64" else min(ns, 32))
+    # moves per sweep launch, from the launches counted over the timed steps
+    moves_per_launch = a.walkers * ne * a.steps / R["kernels"][kname][1]
     f_em = flops_per_move(nb, ns, ne, na, 5 * na, 5, 4, p_acc)
     dt = "f64" if head == "fp64" else "f32"
     achieved = f_em * moves_per_launch / (avg_ms * 1e-3)

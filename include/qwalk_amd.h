@@ -61,7 +61,8 @@ typedef struct qw_engine_opts {
   int precision;            /* QW_PREC_* */
   int device;               /* HIP device ordinal */
   unsigned long long seed;  /* key of the counter-based random streams */
-  int refresh_interval;     /* mixed precision: FP64 LU recompute every this many moves (0 = 32) */
+  int refresh_interval;     /* mixed precision: FP64 LU recompute every this many moves
+                               (0 = once per sweep of a spin channel) */
 } qw_engine_opts;
 
 typedef struct qw_method_desc {

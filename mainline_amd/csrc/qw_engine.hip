@@ -87,7 +87,9 @@ std::vector<JRad<R>> jrad_table(const std::vector<JObj>& objs) {
 
 struct qw_engine {
   std::string err;
-  int device = 0, precision = 0, refresh = 32;
+  // mixed precision: FP64 recompute of a spin's inverse after this many moves; by default once
+  // per sweep of the channel (FP32 rank-1 updates for at most one sweep between recomputes)
+  int device = 0, precision = 0, refresh = 1 << 20;
   unsigned long long seed = 1;
   SysTables T;
   DevSys S{};
