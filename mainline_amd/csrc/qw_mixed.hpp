@@ -118,6 +118,55 @@ __device__ inline float jastrow_e_f(const DevSys& S, int e, const double p[3], c
   return jastrow_e_f(S, e, p, pos, g, lane, S.eit_pos, 3);
 }
 
+// The same FP32 Jastrow value and gradient for a sweep that keeps the electron positions in
+// registers: lane l holds electrons l and l + 64 (pe[0], pe[1]; at most 128 electrons).  Each
+// partner's distance is formed once and every object is evaluated on it (objects innermost),
+// so the pair loop reads no LDS.  Same terms as jastrow_e_t; the per-lane sums run in another
+// order.
+__device__ inline float jastrow_e_freg(const DevSys& S, int e, const double p[3], const double (*pe)[3], float g[3],
+                                       int lane, const double* epos, int estride) {
+  float u = 0, gx = 0, gy = 0, gz = 0, l = 0;
+  const QW_CONST int* ea0 = cptr(S.eit_a0);
+  const QW_CONST int* eo0 = cptr(S.eit_o0);
+  const QW_CONST JRad<float>* eit = cptr(S.eitf);
+  const QW_CONST JRad<float>* ee = cptr(S.eef);
+  for (int t = 0; t < S.n_eit; t++) {
+    const int a0 = ea0[t], a1 = ea0[t + 1], o0 = eo0[t], o1 = eo0[t + 1];
+    for (int k = a0 + lane; k < a1; k += kWave) {
+      const double* ap = epos + estride * k;
+      const float x = (float)(p[0] - ap[0]), y = (float)(p[1] - ap[1]), z = (float)(p[2] - ap[2]);
+      float r, ri;
+      dist_inv(x * x + y * y + z * z, r, ri);
+      float gf = 0;
+      for (int o = o0; o < o1; o++) with_jrad<float, false>(eit[o], [&](const auto& ev) { ev(0.f, r, ri, u, gf, l); });
+      gx += gf * x;
+      gy += gf * y;
+      gz += gf * z;
+    }
+  }
+  const bool eup = e < S.nup;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int j = lane + h * kWave;
+    if (j < S.ne && j != e) {
+      const float x = (float)(p[0] - pe[h][0]), y = (float)(p[1] - pe[h][1]), z = (float)(p[2] - pe[h][2]);
+      float r, ri;
+      dist_inv(x * x + y * y + z * z, r, ri);
+      const float uf = ((j < S.nup) != eup) ? 1.f : 0.f;
+      float gf = 0;
+      for (int o = 0; o < S.nee; o++) with_jrad<float, false>(ee[o], [&](const auto& ev) { ev(uf, r, ri, u, gf, l); });
+      gx += gf * x;
+      gy += gf * y;
+      gz += gf * z;
+    }
+  }
+  u = wsumf(u);
+  g[0] = wsumf(gx);
+  g[1] = wsumf(gy);
+  g[2] = wsumf(gz);
+  return u;
+}
+
 // LCAO of one point in the mixed path.  On return lane j < n holds phi[c] (c = value,
 // gradient, laplacian) of MO j.  chi: LDS tile 5 x ld floats (FP32 basis values);
 // phis: LDS 5 x 64 floats used to move MFMA accumulators to lane = MO.
@@ -308,6 +357,14 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys 
   float* phib = rowb + NS;
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   sweep_randoms<DEPTH>(A, (unsigned)w, off, rnd, lane);
+  // electron positions of this lane's Jastrow partners, kept in registers for the sweep
+  double pe[2][3];
+#pragma unroll
+  for (int h = 0; h < 2; h++)
+    for (int d = 0; d < 3; d++) {
+      const int j = lane + h * kWave;
+      pe[h][d] = j < ne ? W.pos[((size_t)w * ne + j) * 3 + d] : 0.0;
+    }
   float inv[NS];
   const float* ig = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
 #pragma unroll
@@ -335,7 +392,7 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys 
     walker_sync<WPB>();
     const float rk = lane < n ? rowb[lane] : 0.f;
     float gj0[3];
-    const float u0 = jastrow_e_f(S, e, r0, pos, gj0, lane, V.epos, 4);
+    const float u0 = jastrow_e_freg(S, e, r0, pe, gj0, lane, V.epos, 4);
     double d0[3] = {(double)(wsumf(rk * m1) + gj0[0]), (double)(wsumf(rk * m2) + gj0[1]), (double)(wsumf(rk * m3) + gj0[2])};
     float phi[5];
     double r1[3], d1[3], ln1 = 0;
@@ -357,7 +414,7 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys 
       float t0 = wsumf(rk * phi[0]);
       float t1 = wsumf(rk * phi[1]), t2 = wsumf(rk * phi[2]), t3 = wsumf(rk * phi[3]);
       float gj[3];
-      float un = jastrow_e_f(S, e, rn, pos, gj, lane, V.epos, 4);
+      float un = jastrow_e_freg(S, e, rn, pe, gj, lane, V.epos, 4);
       float it0 = 1.f / t0;
       dn[0] = (double)(t1 * it0 + gj[0]);
       dn[1] = (double)(t2 * it0 + gj[1]);
@@ -423,6 +480,12 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys 
       if (R < 0) sgn = -sgn;
       nacc++;
       if (lane < 3) pos[3 * e + lane] = r1[lane];
+      if (lane == (e & (kWave - 1))) {
+        if (e < kWave)
+          for (int d = 0; d < 3; d++) pe[0][d] = r1[d];
+        else
+          for (int d = 0; d < 3; d++) pe[1][d] = r1[d];
+      }
     }
     if (lane < n && k + 1 < A.k1) {
       const size_t e1 = e + 1;
