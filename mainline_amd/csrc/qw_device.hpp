@@ -76,6 +76,13 @@ struct DevSys {
   const int* atom_pp;
   double vii;
   int chunk_nf;           // max functions per chunk (LDS tile width)
+  // several determinants (ndet > 1): weights, per-spin MO list lengths (the MO cache row
+  // holds nsp = max(nmo) values), occupations [2][ndet][nsig] as indices into the lists
+  int ndet, nsig;         // nsig = max(nup, ndn): row stride of the determinant inverses
+  int nmo[2];
+  const double* detwt;    // [ndet]
+  const double* logc;     // [ndet] log|detwt|
+  const int* occ;         // [2][ndet][nsig]
 };
 
 // Per-walker state in device memory (walker-major).
@@ -94,6 +101,9 @@ struct DevWalkers {
   int* acc;        // [nw] accepted moves since the last reset
   double* bsum;    // [nw][16] block sums (see kBsum)
   double* dgl;     // [nw][ne][4] determinant gradient / laplacian ratios from the last sweep
+  double* invd;    // several determinants: [nw][2][ndet][nsig][nsig] transposed inverses
+  double* ldd;     // [nw][2][ndet] log|det|
+  double* sgd;     // [nw][2][ndet] sign
 };
 
 // ---------------------------------------------------------------- wave reductions

@@ -20,6 +20,7 @@
 #include "qw_big.hpp"
 #include "qw_kernels.hpp"
 #include "qw_mixed.hpp"
+#include "qw_multidet.hpp"
 #include "qw_parm.hpp"
 #include "qw_setup.hpp"
 
@@ -35,6 +36,10 @@ thread_local std::string g_create_error;
 std::mutex g_devmem_mu;
 
 struct DevErr : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+// Valid input outside what this engine supports (QW_ERR_UNSUPPORTED).
+struct Unsupported : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 #define HIPCHK(x)                                                                                  \
@@ -127,6 +132,7 @@ struct qw_engine {
 
   // more than 64 electrons of one spin: 256-thread walker blocks, solver-library refresh
   bool big = false;
+  bool md = false;  // several determinants (FP64, k_inv_md / k_sweep_md / k_local<double, true>)
   rocblas_handle rb = nullptr;
   double* luw = nullptr;
   int* lupiv = nullptr;
@@ -242,7 +248,12 @@ struct qw_engine {
     S.nint = T.nint;
     S.nee = (int)T.ee.size();
     S.has_pp = T.pptypes.empty() ? 0 : 1;
-    S.nsp = std::max(T.nup, T.ndn);
+    // MO cache row: the spin's MO list (its electrons' orbitals with one determinant)
+    S.nsp = std::max(T.nmo[0], T.nmo[1]);
+    S.nsig = std::max(T.nup, T.ndn);
+    S.ndet = T.ndet;
+    S.nmo[0] = T.nmo[0];
+    S.nmo[1] = T.nmo[1];
     S.vii = T.vii;
     // shell chunks whose functions fit one LDS tile
     std::vector<int> ch{0};
@@ -270,6 +281,19 @@ struct qw_engine {
       S.C[s] = upload(T.C[s]);
       std::vector<float> cf(T.C[s].begin(), T.C[s].end());
       S.Cf[s] = upload(cf);
+    }
+    {
+      std::vector<double> lc(T.ndet);
+      for (int d = 0; d < T.ndet; d++) lc[d] = std::log(std::fabs(T.detwt[d]));
+      std::vector<int> oc((size_t)2 * T.ndet * S.nsig, 0);
+      for (int s = 0; s < 2; s++) {
+        const int n = s ? T.ndn : T.nup;
+        for (int d = 0; d < T.ndet; d++)
+          for (int i = 0; i < n; i++) oc[((size_t)s * T.ndet + d) * S.nsig + i] = T.occ[s][(size_t)d * n + i];
+      }
+      S.detwt = upload(T.detwt);
+      S.logc = upload(lc);
+      S.occ = upload(oc);
     }
     // linear Jastrow parameters: one unit-coefficient object each, atom lists for one-body ones
     {
@@ -370,6 +394,7 @@ struct qw_engine {
       const unsigned long long nm = std::max(T.nup, T.ndn);
       b += N * nm * nm * 8 + N * nm * 4 + N * 4;
     }
+    if (md) b += N * 2 * S.ndet * (S.nsig * S.nsig + 2) * 8;
     return b;
   }
 
@@ -413,6 +438,13 @@ struct qw_engine {
     W.acc = walloc<int>(n);
     W.bsum = walloc<double>((size_t)n * kBsum);
     W.dgl = walloc<double>((size_t)n * ne * 4);
+    if (md) {
+      W.invd = walloc<double>((size_t)n * 2 * S.ndet * S.nsig * S.nsig);
+      W.ldd = walloc<double>((size_t)n * 2 * S.ndet);
+      W.sgd = walloc<double>((size_t)n * 2 * S.ndet);
+    } else {
+      W.invd = W.ldd = W.sgd = nullptr;
+    }
     int P = 1;
     while (P < n) P <<= 1;
     keys = walloc<unsigned long long>(P);
@@ -471,6 +503,14 @@ struct qw_engine {
       big_refresh_all();
       return;
     }
+    if (md) {
+      run_mo_rows();
+      launch("k_inv", [&] {
+        hipLaunchKernelGGL(k_inv_md, dim3((unsigned)((size_t)nw * 2 * S.ndet)), dim3(64), inv_md_lds_bytes(S.nsig),
+                           stream, S, W, nw);
+      });
+      return;
+    }
     if (precision == QW_PREC_FP64) {
       run_mo_rows();
       run_inv<double, double>(W.mo, W.inv, -1);
@@ -482,6 +522,14 @@ struct qw_engine {
     SweepArgs A{s, k0, k1, tau, seed, step, sweep, restrict_nodes, dets, dstep};
     if (big) {
       big_sweep(A, depth);
+      return;
+    }
+    if (md) {
+      const size_t lds = sweep_md_lds_bytes(S.ne, S.ndet, S.nsig, S.nsp, S.chunk_nf, depth);
+      launch("k_sweep", [&] {
+        if (depth == 1) hipLaunchKernelGGL(k_sweep_md<1>, dim3(nw), dim3(64), lds, stream, S, W, A);
+        else hipLaunchKernelGGL(k_sweep_md<2>, dim3(nw), dim3(64), lds, stream, S, W, A);
+      });
       return;
     }
     if (precision != QW_PREC_FP64) {
@@ -595,6 +643,11 @@ struct qw_engine {
     });
   }
   void run_local(const LocalArgs& A) {
+    if (md) {
+      const size_t lds = lds_local() + ((size_t)5 * S.ndet + S.nsp + 2) * sizeof(double);
+      launch("k_local", [&] { hipLaunchKernelGGL((k_local<double, true>), dim3(nw), dim3(64), lds, stream, S, W, A); });
+      return;
+    }
     if (precision != QW_PREC_FP64) {
       mixed_local(*this, A);
       return;
@@ -773,6 +826,9 @@ qw_status guarded(qw_engine* e, F f) {
   } catch (DevErr& x) {
     if (e) e->err = x.what();
     return QW_ERR_DEVICE;
+  } catch (Unsupported& x) {
+    if (e) e->err = x.what();
+    return QW_ERR_UNSUPPORTED;
   } catch (std::bad_alloc&) {
     if (e) e->err = "host allocation failed";
     return QW_ERR_NOMEM;
@@ -823,6 +879,17 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
       return QW_ERR_UNSUPPORTED;
     }
     e->big = true;
+  }
+  if (e->T.ndet > 1) {
+    if (e->precision != QW_PREC_FP64) {
+      g_create_error = "several determinants are supported with the fp64 precision only";
+      return QW_ERR_UNSUPPORTED;
+    }
+    if (nmax > kMdMaxSpin) {
+      g_create_error = "several determinants are supported up to 32 electrons of one spin";
+      return QW_ERR_UNSUPPORTED;
+    }
+    e->md = true;
   }
   try {
     int ndev = 0;
@@ -974,6 +1041,7 @@ extern "C" qw_status qw_parm_deriv(qw_engine* e, int K, const double* pos, doubl
                                    double* gradderiv, double* val_gradient) {
   if (!e || K <= 0 || !pos || !gradient || !gradderiv || !val_gradient) return QW_ERR_ARG;
   return guarded(e, [&] {
+    if (e->md) throw Unsupported("parameter derivatives are supported with one determinant only");
     e->ensure_walkers(K);
     e->dmc_ready = false;
     const size_t ne = e->S.ne, np = (size_t)e->npar;
@@ -1140,6 +1208,7 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
     A.seed = e->seed;
     A.ctl = e->dctl;
     e->tm_mode = dmc && e->S.has_pp ? (m->tmoves == 1 ? 1 : m->tmoves == 2 ? 2 : 0) : 0;
+    if (e->tm_mode && e->md) throw Unsupported("T-moves are supported with one determinant only");
     if (e->tm_mode) e->ensure_tneg();
     HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)nw * kBsum * sizeof(double), e->stream));
     HIPCHK(hipMemsetAsync(e->W.acc, 0, (size_t)nw * sizeof(int), e->stream));

@@ -35,9 +35,9 @@ struct DmcCtl {
 // chi is an LDS tile of 5 x S.chunk_nf doubles.  Block-wide barriers separate the tile
 // producer (basis_lap) from the consumer.
 // flist: LDS tile of S.chunk_nf ints (indices of the active functions).
-__device__ inline void lcao_lap(const DevSys& S, int s, const double p[3], double* chi, int* flist, double phi[5],
-                                int lane) {
-  const int n = s ? S.ndn : S.nup, ld = S.chunk_nf;
+__device__ inline void lcao_lap_n(const DevSys& S, int s, int n, const double p[3], double* chi, int* flist,
+                                  double phi[5], int lane) {
+  const int ld = S.chunk_nf;
   for (int c = 0; c < 5; c++) phi[c] = 0;
   for (int ch = 0; ch < S.nchunk; ch++) {
     int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
@@ -65,8 +65,14 @@ __device__ inline void lcao_lap(const DevSys& S, int s, const double p[3], doubl
   }
 }
 
-__device__ inline double lcao_val(const DevSys& S, int s, const double p[3], double* chi, int* flist, int lane) {
-  const int n = s ? S.ndn : S.nup;
+__device__ inline void lcao_lap(const DevSys& S, int s, const double p[3], double* chi, int* flist, double phi[5],
+                                int lane) {
+  lcao_lap_n(S, s, s ? S.ndn : S.nup, p, chi, flist, phi, lane);
+}
+
+// n: MOs per row of C[s] (the spin's electrons, or its MO list with several determinants).
+__device__ inline double lcao_val_n(const DevSys& S, int s, int n, const double p[3], double* chi, int* flist,
+                                    int lane) {
   double phi = 0;
   for (int ch = 0; ch < S.nchunk; ch++) {
     int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
@@ -83,6 +89,9 @@ __device__ inline double lcao_val(const DevSys& S, int s, const double p[3], dou
   }
   return phi;
 }
+__device__ inline double lcao_val(const DevSys& S, int s, const double p[3], double* chi, int* flist, int lane) {
+  return lcao_val_n(S, s, s ? S.ndn : S.nup, p, chi, flist, lane);
+}
 
 // ---------------------------------------------------------------- k_mo_rows
 __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) {
@@ -91,11 +100,11 @@ __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) 
   const long blk = blockIdx.x;
   const int w = (int)(blk / S.ne), e = (int)(blk % S.ne);
   if (w >= nw) return;
-  const int s = e < S.nup ? 0 : 1, n = s ? S.ndn : S.nup;
+  const int s = e < S.nup ? 0 : 1, n = S.nmo[s];
   double p[3];
   for (int d = 0; d < 3; d++) p[d] = W.pos[((size_t)w * S.ne + e) * 3 + d];
   double phi[5];
-  lcao_lap(S, s, p, lds, (int*)(lds + 5 * S.chunk_nf), phi, lane);
+  lcao_lap_n(S, s, n, p, lds, (int*)(lds + 5 * S.chunk_nf), phi, lane);
   if (lane < n) {
     double* row = W.mo + ((size_t)w * S.ne + e) * 5 * S.nsp;
     for (int c = 0; c < 5; c++) row[c * S.nsp + lane] = phi[c];
@@ -717,7 +726,65 @@ __device__ inline void jastrow_local(const DevSys& S, const JRad<R>* ee, const J
   }
 }
 
-template <typename TM>
+// ---------------------------------------------------------------- several determinants
+// Weights b_d = sign_d sign_0 exp(l_d - l_0) of the terms c_d D_up,d D_dn,d (log l_d, sign
+// sign_d), pivoting on the first term as the reference's sum of log values does
+// (src/program_utils/MatrixAlgebra.h:92-99); determinant parts as weighted sums over the
+// determinants of each one's inverse row times its orbitals (src/wavefunction/Slat_wf.h:1021-1139).
+constexpr int kMdMaxSpin = 32;
+
+// b[d] for the walker's per-determinant log|det| and sign ([2][ndet] each, in LDS); returns
+// sum_d b_d in every lane and the pivot term's log l0 and sign s0.
+__device__ inline double md_weights(const DevSys& S, const double* ld, const double* sg, double* b, double& l0,
+                                    double& s0, int lane) {
+  const int nd = S.ndet;
+  const QW_CONST double* lc = cptr(S.logc);
+  const QW_CONST double* cw = cptr(S.detwt);
+  l0 = lc[0] + ld[0] + ld[nd];
+  s0 = (cw[0] < 0 ? -1.0 : 1.0) * sg[0] * sg[nd];
+  double part = 0;
+  for (int d = lane; d < nd; d += kWave) {
+    const double l = lc[d] + ld[d] + ld[nd + d];
+    const double sd = (cw[d] < 0 ? -1.0 : 1.0) * sg[d] * sg[nd + d];
+    const double v = d == 0 ? 1.0 : sd * s0 * exp(l - l0);
+    b[d] = v;
+    part += v;
+  }
+  return wsum(part);
+}
+
+// out[c] = sum_d b_d sum_j inv_d(k, j) v[c][occ_d(j)] / den for NC components (v: LDS rows of
+// length ldv indexed by the spin's MO list), lanes over determinants.
+template <int NC>
+__device__ inline void md_contract(const DevSys& S, const double* invw, int s, int k, const double* v, int ldv,
+                                   const double* b, double den, double out[NC], int lane) {
+  const int nd = S.ndet, n = s ? S.ndn : S.nup, ns = S.nsig;
+  const QW_CONST int* oc = cptr(S.occ) + (size_t)s * nd * ns;
+  double part[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) part[c] = 0;
+  for (int d = lane; d < nd; d += kWave) {
+    const double* row = invw + (((size_t)s * nd + d) * ns + k) * ns;
+    const QW_CONST int* o = oc + (size_t)d * ns;
+    double t[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) t[c] = 0;
+    for (int j = 0; j < n; j++) {
+      const double r = row[j];
+      const int m = o[j];
+#pragma unroll
+      for (int c = 0; c < NC; c++) t[c] += v[c * ldv + m] * r;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) part[c] += b[d] * t[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; c++) out[c] = wsum(part[c]) / den;
+}
+
+// MD: several determinants (FP64): determinant parts and quadrature ratios are weighted sums
+// over the determinants (md_contract), from their inverses in W.invd.
+template <typename TM, bool MD = false>
 __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs A) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, w = blockIdx.x;
@@ -728,10 +795,37 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
   double* lD = gD + 3 * ne;
   double* chi = lD + ne;
   int* flist = (int*)(chi + 5 * S.chunk_nf);
+  // MD: determinant weights, their log/sign, and MO values at a quadrature point
+  double* bw = (double*)(flist + ((S.chunk_nf + 1) & ~1));
+  double* mdl = bw + (MD ? S.ndet : 0);
+  double* phq = mdl + (MD ? 4 * S.ndet : 0);
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   const TM* mog = mo_cache<TM>(W) + (size_t)w * ne * 5 * nsp;
+  double md_den = 1, md_l0 = 0, md_s0 = 1;
+  const double* md_inv = nullptr;
+  if constexpr (MD) {
+    for (int i = lane; i < 2 * S.ndet; i += kWave) {
+      mdl[i] = W.ldd[(size_t)w * 2 * S.ndet + i];
+      mdl[2 * S.ndet + i] = W.sgd[(size_t)w * 2 * S.ndet + i];
+    }
+    __syncthreads();
+    md_den = md_weights(S, mdl, mdl + 2 * S.ndet, bw, md_l0, md_s0, lane);
+    md_inv = W.invd + (size_t)w * 2 * S.ndet * S.nsig * S.nsig;
+    __syncthreads();
+    for (int e = 0; e < ne; e++) {
+      const int s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0);
+      double t[4];
+      md_contract<4>(S, md_inv, s, k, (const double*)mog + ((size_t)e * 5 + 1) * nsp, nsp, bw, md_den, t, lane);
+      if (lane == 0) {
+        gD[3 * e] = t[0];
+        gD[3 * e + 1] = t[1];
+        gD[3 * e + 2] = t[2];
+        lD[e] = t[3];
+      }
+    }
+  }
   // determinant gradient / laplacian per electron: lane k owns row k of its spin's inverse
-  if (A.use_dgl) {
+  if (A.use_dgl && !MD) {
     for (int e = lane; e < ne; e += kWave) {
       const double* d = W.dgl + ((size_t)w * ne + e) * 4;
       gD[3 * e] = d[0];
@@ -742,7 +836,7 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
   }
   // otherwise from the inverse and the MO cache: electron by electron, lanes over the MOs, so
   // every row is read coalesced
-  for (int s = 0; s < 2 && !A.use_dgl; s++) {
+  for (int s = 0; s < 2 && !A.use_dgl && !MD; s++) {
     const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0;
     const TM* ig = inv_store<TM>(W) + ((size_t)w * 2 + s) * nsp * nsp;
     for (int k = 0; k < n; k++) {
@@ -873,7 +967,8 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
           double gtmp[3];
           const double u_old = jastrow_e_val(S, ee, re, pos, lane);
           (void)gtmp;
-          const double ivk = lane < n ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
+          const double ivk =
+              (!MD && lane < n) ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
           double nl = 0, nneg = 0;
           for (int q = 0; q < P.aip; q++) {
             double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];
@@ -883,8 +978,17 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
             double nd[3] = {np_[0] - ap[0], np_[1] - ap[1], np_[2] - ap[2]};
             double nr = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
             double cs = (nd[0] * od[0] + nd[1] * od[1] + nd[2] * od[2]) / (nr * orr);
-            double phi = lcao_val(S, s, np_, chi, flist, lane);
-            double Rq = wsum(ivk * phi);
+            double Rq;
+            if constexpr (MD) {
+              const double phm = lcao_val_n(S, s, S.nmo[s], np_, chi, flist, lane);
+              __syncthreads();
+              if (lane < S.nmo[s]) phq[lane] = phm;
+              __syncthreads();
+              md_contract<1>(S, md_inv, s, k, phq, nsp, bw, md_den, &Rq, lane);
+            } else {
+              double phi = lcao_val(S, s, np_, chi, flist, lane);
+              Rq = wsum(ivk * phi);
+            }
             double uq = jastrow_e_val(S, ee, np_, pos, lane);
             double ratio = Rq * exp(uq - u_old) * P.w[q];
             double ts = 0;
@@ -910,8 +1014,13 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
     el[2] = vnl;
     el[3] = etot;
     if (A.mode == 0) {
-      A.out_logpsi[w] = W.logdet[2 * w] + W.logdet[2 * w + 1] + utot;
-      A.out_sign[w] = W.sgn[2 * w] * W.sgn[2 * w + 1];
+      if constexpr (MD) {
+        A.out_logpsi[w] = md_l0 + log(fabs(md_den)) + utot;
+        A.out_sign[w] = md_s0 * (md_den < 0 ? -1.0 : 1.0);
+      } else {
+        A.out_logpsi[w] = W.logdet[2 * w] + W.logdet[2 * w + 1] + utot;
+        A.out_sign[w] = W.sgn[2 * w] * W.sgn[2 * w + 1];
+      }
       A.out_eparts[3 * w] = ke;
       A.out_eparts[3 * w + 1] = vloc;
       A.out_eparts[3 * w + 2] = vnl;
@@ -1284,6 +1393,14 @@ __global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcC
     for (size_t i = t; i < nmo; i += blockDim.x) W.mo[dst * nmo + i] = W.mo[src * nmo + i];
   if (W.mof)
     for (size_t i = t; i < nmo; i += blockDim.x) W.mof[dst * nmo + i] = W.mof[src * nmo + i];
+  if (W.invd) {
+    const size_t nd = (size_t)2 * S.ndet * S.nsig * S.nsig, nl = (size_t)2 * S.ndet;
+    for (size_t i = t; i < nd; i += blockDim.x) W.invd[dst * nd + i] = W.invd[src * nd + i];
+    for (size_t i = t; i < nl; i += blockDim.x) {
+      W.ldd[dst * nl + i] = W.ldd[src * nl + i];
+      W.sgd[dst * nl + i] = W.sgd[src * nl + i];
+    }
+  }
   if (t < 2) {
     W.logdet[2 * dst + t] = W.logdet[2 * src + t];
     W.sgn[2 * dst + t] = W.sgn[2 * src + t];

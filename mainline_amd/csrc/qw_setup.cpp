@@ -511,11 +511,15 @@ SysTables read_deck(const std::string& path, bool* unsup) {
     }
     if (!keyword(wf1, "SLATER")) unsupported("WF1 must be a SLATER determinant");
     Words dw, st, orb;
-    section0(wf1, dw, "DETWT");
-    if (dw.size() != 1) unsupported("only single-determinant SLATER is supported");
     if (keyword(wf1, "OPTIMIZE_MO") || keyword(wf1, "CSF")) unsupported("OPTIMIZE_MO / CSF are not supported");
+    if (!section0(wf1, dw, "DETWT") || dw.empty()) bad("need DETWT");
+    T.ndet = (int)dw.size();
+    if (T.ndet > kMaxDet) unsupported("at most " + std::to_string(kMaxDet) + " determinants are supported");
+    T.detwt.resize(T.ndet);
+    for (int d = 0; d < T.ndet; d++) T.detwt[d] = std::atof(dw[d].c_str());
+    if (T.detwt[0] == 0.0) bad("the first determinant must have a nonzero weight");
     if (!section0(wf1, st, "STATES")) bad("need STATES");
-    if ((int)st.size() != T.ne()) bad("STATES must list one orbital per electron");
+    if ((int)st.size() != T.ndet * T.ne()) bad("STATES must list one orbital per electron for every determinant");
     if (!section0(wf1, orb, "ORBITALS")) unsupported("only ORBITALS (real MOs) are supported");
     if (!keyword(orb, "CUTOFF_MO")) unsupported("only CUTOFF_MO orbitals are supported");
     int nmo_tot = 0;
@@ -605,14 +609,33 @@ SysTables read_deck(const std::string& path, bool* unsup) {
     }
     for (auto& row : moc)
       for (double& v : row) v = std::fabs(v) > 1e-12 ? magnify * v : 0.0;
+    // per spin: the MOs any determinant occupies, in order of first appearance, and each
+    // determinant's occupation as indices into that list (src/wavefunction/Slat_wf_data.cpp:192-263)
     for (int s = 0; s < 2; s++) {
-      int n = s ? T.ndn : T.nup;
-      T.C[s].assign((size_t)T.nb * n, 0.0);
-      for (int i = 0; i < n; i++) {
-        int mo = std::atoi(st[(s ? T.nup : 0) + i].c_str()) - 1;
-        if (mo < 0 || mo >= nmo_tot) bad("STATES index out of range");
-        for (int f = 0; f < T.nb; f++) T.C[s][(size_t)f * n + i] = moc[mo][f];
-      }
+      const int n = s ? T.ndn : T.nup;
+      std::vector<int> list;
+      T.occ[s].assign((size_t)T.ndet * n, 0);
+      for (int d = 0; d < T.ndet; d++)
+        for (int i = 0; i < n; i++) {
+          const int mo = std::atoi(st[(size_t)d * T.ne() + (s ? T.nup : 0) + i].c_str()) - 1;
+          if (mo < 0 || mo >= nmo_tot) bad("STATES index out of range");
+          int place = -1;
+          for (size_t q = 0; q < list.size(); q++)
+            if (list[q] == mo) place = (int)q;
+          if (place < 0) {
+            place = (int)list.size();
+            list.push_back(mo);
+          }
+          for (int j = 0; j < i; j++)
+            if (T.occ[s][(size_t)d * n + j] == place) bad("repeated orbital in one spin channel of a determinant");
+          T.occ[s][(size_t)d * n + i] = place;
+        }
+      const int nm = (int)list.size();
+      if (T.ndet > 1 && nm > 64) unsupported("several determinants with more than 64 orbitals of one spin are not supported");
+      T.nmo[s] = nm;
+      T.C[s].assign((size_t)T.nb * nm, 0.0);
+      for (int i = 0; i < nm; i++)
+        for (int f = 0; f < T.nb; f++) T.C[s][(size_t)f * nm + i] = moc[list[i]][f];
     }
 
     // ---- Jastrow

@@ -10,6 +10,7 @@ constexpr int kMaxJF = 8;     // functions per Jastrow radial object
 constexpr int kMaxPPL = 6;    // pseudopotential channels
 constexpr int kMaxPPT = 6;    // Gaussian terms per channel
 constexpr int kMaxAIP = 12;   // quadrature points
+constexpr int kMaxDet = 256;  // determinants of a multi-determinant expansion
 
 // Shell symmetries (src/basis_function/Cubic_spline_build.cpp:26-140,664-800).  7F_crystal and
 // 9G_pyscf are the 7F and 9G functions in the orders of those programs.
@@ -75,7 +76,13 @@ struct SysTables {
   int ntab = 0, nint = 0;         // spline tables, intervals per table (padded to the max)
   std::vector<double> spline;     // [ntab][nint][4]
   std::vector<double> tab_invspacing, tab_spacing;
-  std::vector<double> C[2];       // per spin [nb][nspin]
+  std::vector<double> C[2];       // per spin [nb][nmo[s]]: the spin's MO list
+  // determinants: weights, and each determinant's orbitals per spin as indices into that
+  // spin's MO list ([ndet][nspin]); one determinant: nmo = nspin, occupation 0..n-1
+  int ndet = 1;
+  std::vector<double> detwt;
+  std::vector<int> occ[2];
+  int nmo[2] = {0, 0};
   std::vector<JObj> ee;           // electron-electron objects (merged over groups)
   std::vector<JObj> ei;           // electron-ion objects, atom ranges in atom_ei0
   std::vector<int> atom_ei0;      // [na+1]

@@ -710,6 +710,11 @@ struct System {
   std::vector<int> fobj;                     // basis object of each function (for obj cutoff)
   std::vector<std::vector<double>> C;        // per spin [totbasis][nmo_s] dense
   std::vector<int> nmo;                      // per spin list length
+  // determinants (src/wavefunction/Slat_wf_data.cpp:34-263): weights and, per spin, each
+  // determinant's occupied orbitals as indices into that spin's MO list
+  int ndet = 1;
+  std::vector<double> detwt;
+  std::vector<std::vector<int>> occ[2];      // [spin][det][n]
   // Jastrow
   std::vector<JGroup> groups;
   // PP
@@ -878,10 +883,13 @@ void read_system(System& S, const Tok& words) {
 void read_slater(System& S, const Tok& wf1, const std::string& dir) {
   if (!haskeyword(wf1, "SLATER")) fail("WF1 must be SLATER");
   Tok detwt, states, orbs;
-  readsection0(wf1, detwt, "DETWT");
-  if (detwt.size() != 1) fail("only a single determinant is supported");
+  if (haskeyword(wf1, "CSF")) fail("CSF input is not supported");
+  if (!readsection0(wf1, detwt, "DETWT") || detwt.empty()) fail("need DETWT");
+  S.ndet = (int)detwt.size();
+  S.detwt.resize(S.ndet);
+  for (int d = 0; d < S.ndet; d++) S.detwt[d] = atof(detwt[d].c_str());
   if (!readsection0(wf1, states, "STATES")) fail("need STATES");
-  if ((int)states.size() != S.ne()) fail("STATES count must equal the number of electrons");
+  if ((int)states.size() != S.ndet * S.ne()) fail("STATES count must be ndet x the number of electrons");
   if (!readsection0(wf1, orbs, "ORBITALS")) fail("need ORBITALS");
   if (!haskeyword(orbs, "CUTOFF_MO")) fail("only CUTOFF_MO orbitals are supported");
   std::string v;
@@ -962,23 +970,37 @@ void read_slater(System& S, const Tok& wf1, const std::string& dir) {
         tot++;
       }
   }
-  // per-spin lists (src/wavefunction/Slat_wf_data.cpp:223-263, MO_matrix_cutoff::buildLists)
+  // per-spin MO lists: every orbital a determinant of that spin occupies, in order of first
+  // appearance, and each determinant's occupation as indices into the list
+  // (src/wavefunction/Slat_wf_data.cpp:192-263, MO_matrix_cutoff::buildLists)
   S.C.assign(2, {});
   S.nmo.assign(2, 0);
   for (int s = 0; s < 2; s++) {
     int n = S.nspin(s);
-    std::vector<int> occ(n);
-    for (int i = 0; i < n; i++) {
-      occ[i] = atoi(states[(s == 0 ? 0 : S.nup) + i].c_str()) - 1;
-      if (occ[i] < 0 || occ[i] >= nmo_total) fail("STATES index out of range");
+    std::vector<int> list;
+    S.occ[s].assign(S.ndet, std::vector<int>(n));
+    for (int d = 0; d < S.ndet; d++) {
+      std::vector<int> orig(n);
+      for (int i = 0; i < n; i++) {
+        orig[i] = atoi(states[(size_t)d * S.ne() + (s == 0 ? 0 : S.nup) + i].c_str()) - 1;
+        if (orig[i] < 0 || orig[i] >= nmo_total) fail("STATES index out of range");
+        for (int j = 0; j < i; j++)
+          if (orig[i] == orig[j]) fail("repeated orbital in one spin channel");
+        int place = -1;
+        for (size_t q = 0; q < list.size(); q++)
+          if (list[q] == orig[i]) place = (int)q;
+        if (place < 0) {
+          place = (int)list.size();
+          list.push_back(orig[i]);
+        }
+        S.occ[s][d][i] = place;
+      }
     }
-    for (int i = 0; i < n; i++)
-      for (int j = 0; j < i; j++)
-        if (occ[i] == occ[j]) fail("repeated orbital in one spin channel");
-    S.nmo[s] = n;
-    S.C[s].assign((size_t)S.totbasis * n, 0.0);
+    int nm = (int)list.size();
+    S.nmo[s] = nm;
+    S.C[s].assign((size_t)S.totbasis * nm, 0.0);
     for (int f = 0; f < S.totbasis; f++)
-      for (int i = 0; i < n; i++) S.C[s][(size_t)f * n + i] = full[occ[i]][f];
+      for (int i = 0; i < nm; i++) S.C[s][(size_t)f * nm + i] = full[list[i]][f];
   }
 }
 
@@ -1388,13 +1410,36 @@ double legendre(double x, int n) {
 struct Walker {
   std::vector<double> pos;              // ne*3
   std::vector<double> mo;               // [ne][5][nmo_s]
-  std::vector<double> inv[2];           // [n][n] transposed inverse
-  double logdet[2] = {0, 0}, sign[2] = {1, 1};
+  std::vector<std::vector<double>> inv[2];  // [det][n][n] transposed inverse per determinant
+  std::vector<double> logdet[2], sign[2];   // [det] log|det| and sign per spin
 };
 
 void walker_init(const System& S, Walker& w) {
   w.mo.assign((size_t)S.ne() * 5 * std::max(S.nmo[0], S.nmo[1]), 0.0);
-  for (int s = 0; s < 2; s++) w.inv[s].assign((size_t)S.nspin(s) * S.nspin(s), 0.0);
+  for (int s = 0; s < 2; s++) {
+    w.inv[s].assign(S.ndet, std::vector<double>((size_t)S.nspin(s) * S.nspin(s), 0.0));
+    w.logdet[s].assign(S.ndet, 0.0);
+    w.sign[s].assign(S.ndet, 1.0);
+  }
+}
+
+// Relative determinant weights b_d = sign_d sign_0 exp(l_d - l_0) of the terms
+// c_d D_up,d D_dn,d (log l_d, sign sign_d), pivoting on the first term as the reference's
+// sum of log values does (src/program_utils/MatrixAlgebra.h:92-99); the total is
+// sign_0 exp(l_0) sum_d b_d.  One determinant: b_0 = 1 exactly.
+void det_terms(const System& S, const Walker& w, std::vector<double>& b, double& l0, double& s0) {
+  b.resize(S.ndet);
+  auto term = [&](int d, double& l, double& sg) {
+    l = log(fabs(S.detwt[d])) + w.logdet[0][d] + w.logdet[1][d];
+    sg = (S.detwt[d] < 0 ? -1.0 : 1.0) * w.sign[0][d] * w.sign[1][d];
+  };
+  term(0, l0, s0);
+  b[0] = 1.0;
+  for (int d = 1; d < S.ndet; d++) {
+    double l, sg;
+    term(d, l, sg);
+    b[d] = sg * s0 * exp(l - l0);
+  }
 }
 inline double* morow(const System& S, Walker& w, int e) {
   return &w.mo[(size_t)e * 5 * std::max(S.nmo[0], S.nmo[1])];
@@ -1405,30 +1450,49 @@ void walker_refresh(const System& S, Walker& w, std::vector<double>& tmp) {
   for (int e = 0; e < S.ne(); e++) mo_lap(S, S.spin_of(e), &w.pos[3 * e], morow(S, w, e), tmp);
   for (int s = 0; s < 2; s++) {
     int n = S.nspin(s);
-    if (n == 0) {
-      w.logdet[s] = 0;
-      w.sign[s] = 1;
-      continue;
+    for (int d = 0; d < S.ndet; d++) {
+      if (n == 0) {
+        w.logdet[s][d] = 0;
+        w.sign[s][d] = 1;
+        continue;
+      }
+      const std::vector<int>& oc = S.occ[s][d];
+      std::vector<double> m((size_t)n * n);
+      for (int k = 0; k < n; k++) {
+        double* r = morow(S, w, s * S.nup + k);
+        for (int i = 0; i < n; i++) m[k * n + i] = r[oc[i]];  // value row, component 0
+      }
+      transpose_inverse(m.data(), w.inv[s][d].data(), n, w.logdet[s][d], w.sign[s][d]);
     }
-    int nm = S.nmo[s];
-    std::vector<double> m((size_t)n * n);
-    for (int k = 0; k < n; k++) {
-      double* r = morow(S, w, s * S.nup + k);
-      for (int i = 0; i < n; i++) m[k * n + i] = r[i];  // value row, component 0
-    }
-    (void)nm;
-    transpose_inverse(m.data(), w.inv[s].data(), n, w.logdet[s], w.sign[s]);
   }
+}
+
+// sum_d b_d sum_j inv_d(k, j) v[occ_d(j)] / sum_d b_d for the row k of spin s: the
+// determinant part of Psi(r')/Psi (v = MO values at r') or of grad Psi / Psi and
+// lap Psi / Psi (v = MO derivatives at the electron) (src/wavefunction/Slat_wf.h:1021-1139).
+double det_contract(const System& S, const Walker& w, int s, int k, const double* v) {
+  const int n = S.nspin(s);
+  std::vector<double> b;
+  double l0, s0;
+  det_terms(S, w, b, l0, s0);
+  double num = 0, den = 0;
+  for (int d = 0; d < S.ndet; d++) {
+    const double* iv = &w.inv[s][d][(size_t)k * n];
+    const std::vector<int>& oc = S.occ[s][d];
+    double t = 0;
+    for (int j = 0; j < n; j++) t += v[oc[j]] * iv[j];
+    num += b[d] * t;
+    den += b[d];
+  }
+  return num / den;
 }
 
 // det part of grad/lap for electron e from the cache (src/wavefunction/Slat_wf.h:1021-1139)
 void det_gradlap(const System& S, Walker& w, int e, double* g, double* l) {
-  int s = S.spin_of(e), n = S.nspin(s), k = e - s * S.nup, nm = S.nmo[s];
+  int s = S.spin_of(e), k = e - s * S.nup, nm = S.nmo[s];
   const double* r = morow(S, w, e);
-  const double* iv = &w.inv[s][(size_t)k * n];
   double t[5] = {0, 0, 0, 0, 0};
-  for (int d = 1; d < 5; d++)
-    for (int j = 0; j < n; j++) t[d] += r[d * nm + j] * iv[j];
+  for (int d = 1; d < 5; d++) t[d] = det_contract(S, w, s, k, r + d * nm);
   for (int d = 0; d < 3; d++) g[d] = t[d + 1];
   *l = t[4];
 }
@@ -1505,9 +1569,7 @@ double pp_nonlocal(const System& S, Walker& w, const double* rot, int stochastic
           for (int d = 0; d < 3; d++) rdr += nd[d + 2] * old[d + 2];
           rdr /= (nd[0] * old[0]);
           mo_val(S, s, np, phi.data(), tmp);
-          double R = 0;
-          const double* iv = &w.inv[s][(size_t)k * n];
-          for (int j = 0; j < n; j++) R += iv[j] * phi[j];
+          const double R = det_contract(S, w, s, k, phi.data());
           double u_new = jastrow_e(S, e, np, w.pos.data(), nullptr, nullptr, js);
           double ratio = R * exp(u_new - u_old) * P.qw[i];
           double ts = 0;
@@ -1649,10 +1711,9 @@ void eval_point(const System& S, Walker& w, int e, Point& P, double u_old, std::
   int s = S.spin_of(e), n = S.nspin(s), k = e - s * S.nup, nm = S.nmo[s];
   P.phi.resize(5 * nm);
   mo_lap(S, s, P.r, P.phi.data(), tmp);
-  const double* iv = &w.inv[s][(size_t)k * n];
-  double t[4] = {0, 0, 0, 0};
-  for (int d = 0; d < 4; d++)
-    for (int j = 0; j < n; j++) t[d] += iv[j] * P.phi[d * nm + j];
+  double t[4];
+  for (int d = 0; d < 4; d++) t[d] = det_contract(S, w, s, k, P.phi.data() + d * nm);
+  (void)n;
   P.R = t[0];
   double gj[3];
   double u = jastrow_e(S, e, P.r, w.pos.data(), gj, nullptr, js);
@@ -1665,11 +1726,13 @@ void eval_point(const System& S, Walker& w, int e, Point& P, double u_old, std::
 void accept_point(const System& S, Walker& w, int e, const Point& P) {
   int s = S.spin_of(e), n = S.nspin(s), k = e - s * S.nup, nm = S.nmo[s];
   std::vector<double> v(n);
-  for (int j = 0; j < n; j++) v[j] = P.phi[j];
-  double f = inverse_update_row(w.inv[s].data(), v.data(), k, n);
-  double ratio = 1. / f;
-  w.logdet[s] += log(fabs(ratio));
-  if (ratio < 0) w.sign[s] = -w.sign[s];
+  for (int d = 0; d < S.ndet; d++) {
+    for (int j = 0; j < n; j++) v[j] = P.phi[S.occ[s][d][j]];
+    double f = inverse_update_row(w.inv[s][d].data(), v.data(), k, n);
+    double ratio = 1. / f;
+    w.logdet[s][d] += log(fabs(ratio));
+    if (ratio < 0) w.sign[s][d] = -w.sign[s][d];
+  }
   double* row = morow(S, w, e);
   for (int d = 0; d < 5; d++)
     for (int j = 0; j < nm; j++) row[d * nm + j] = P.phi[d * nm + j];
@@ -2038,8 +2101,12 @@ extern "C" int orc_eval(orc_system* o, int K, const double* pos, const double* r
     w.pos.assign(pos + (size_t)k * ne * 3, pos + (size_t)(k + 1) * ne * 3);
     walker_refresh(S, w, tmp);
     double U = jastrow_total(S, w.pos.data(), js);
-    logpsi[k] = w.logdet[0] + w.logdet[1] + U;
-    sign[k] = w.sign[0] * w.sign[1];
+    std::vector<double> b;
+    double l0, s0, sb = 0;
+    det_terms(S, w, b, l0, s0);
+    for (double x : b) sb += x;
+    logpsi[k] = l0 + log(fabs(sb)) + U;
+    sign[k] = s0 * (sb < 0 ? -1.0 : 1.0);
     double ke = kinetic(S, w, grad + (size_t)k * ne * 3, lap + (size_t)k * ne, js);
     eparts[3 * k + 0] = ke;
     eparts[3 * k + 1] = coulomb(S, w.pos.data());

@@ -18,6 +18,8 @@ Cases:
   h2_jast   the reference's tests/h2 deck with its two-group Jastrow (cusp + POLYPADE)
   n2_pp     the reference's tests/n2 basis, orbitals and BFD pseudopotential (AIP 6), one
             determinant, with a POLYPADE Jastrow of nonzero coefficients
+  n2_ci     the reference's tests/n2 CI wave function (28 determinants, qw.ci + cidet);
+  n2_cij    the same with the POLYPADE Jastrow of n2_pp
   h4, pp16, ae32   the seeded synthetic molecules of mainline_amd/synth.py (4, 64, 128 e)
   fg3       three atoms with every supported shell symmetry (S P 6D 7F 7F_crystal 10F 9G
             9G_pyscf 15G), 12 electrons
@@ -141,6 +143,23 @@ def ref_deck_cases(tmp):
         f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW WF1 { INCLUDE n2.slater } "
                 "WF2 { INCLUDE n2.jast2 } }\n")
     cases["n2_pp"] = (d, "n2.wfeval", clouds([[0, 0, -1.60626709], [0, 0, 1.60626709]], [5, 5], 12, 13, 0.9))
+    # N2 CI (tests/n2/qw.ci): 28 determinants over 8 MOs, BFD pseudopotential; without and
+    # with the Jastrow above
+    d = os.path.join(DECKS, "n2_ci")
+    os.makedirs(d, exist_ok=True)
+    for f in ["qw.sys", "qw.basis", "qw.orb", "cidet"]:
+        shutil.copy(os.path.join(REF_TESTS, "n2", f), d)
+    with open(os.path.join(d, "n2_jast.jast2"), "w") as f:
+        f.write(N2_JAST)
+    slater = ("SLATER\n ORBITALS {\n  CUTOFF_MO\n  MAGNIFY 1\n  NMO 8\n  ORBFILE qw.orb\n"
+              "  INCLUDE qw.basis\n  CENTERS { USEGLOBAL }\n }\n include cidet\n")
+    with open(os.path.join(d, "n2ci.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC {\n" + slater + "}\n")
+    with open(os.path.join(d, "n2cij.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW\nWF1 {\n" + slater + "}\nWF2 { INCLUDE n2_jast.jast2 }\n}\n")
+    n2c = clouds([[0, 0, -1.60626709], [0, 0, 1.60626709]], [5, 5], 12, 17, 0.9)
+    cases["n2_ci"] = (d, "n2ci.wfeval", n2c)
+    cases["n2_cij"] = (d, "n2cij.wfeval", n2c)
     return cases
 
 

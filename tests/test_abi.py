@@ -30,16 +30,24 @@ def test_missing_deck_is_an_input_error():
     assert ex.value.status == -2
 
 
-def test_multideterminant_is_reported_unsupported(tmp_path):
+def test_determinant_input_checks(tmp_path):
+    """STATES must hold ndet x Ne orbitals (bad input); CSF input, and several determinants
+    with a mixed precision, are valid QWalk the engine does not run (unsupported)."""
     g, deck = golden("h_atom")
     d = os.path.dirname(deck)
-    txt = open(os.path.join(d, "qw.slater")).read().replace("DETWT { 1.0 }", "DETWT { 1.0 0.5 }")
-    (tmp_path / "qw.slater").write_text(txt)
+    base = open(os.path.join(d, "qw.slater")).read()
     for f in ["qw.sys", "qw.basis", "qw.orb"]:
         (tmp_path / f).write_text(open(os.path.join(d, f)).read())
     (tmp_path / "x.wfeval").write_text("INCLUDE qw.sys\nTRIALFUNC { INCLUDE qw.slater }\n")
+    for txt, status in [(base.replace("DETWT { 1.0 }", "DETWT { 1.0 0.5 }"), -2),
+                        (base.replace("DETWT { 1.0 }", "CSF { 1.0 1.0 }"), -3)]:
+        (tmp_path / "qw.slater").write_text(txt)
+        with pytest.raises(E.QwError) as ex:
+            E.Engine(str(tmp_path / "x.wfeval"))
+        assert ex.value.status == status, ex.value
+    g, deck = golden("n2_ci")
     with pytest.raises(E.QwError) as ex:
-        E.Engine(str(tmp_path / "x.wfeval"))
+        E.Engine(deck, "mixed_fp32")
     assert ex.value.status == -3
 
 

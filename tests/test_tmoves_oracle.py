@@ -18,7 +18,7 @@ import pytest
 from oracle import oracle as O
 from tests._util import golden
 
-PP_CASES = ["n2_pp", "pp16"]
+PP_CASES = ["n2_pp", "n2_ci", "pp16"]
 
 
 @pytest.mark.parametrize("name", PP_CASES + ["h_atom", "ae32"])
