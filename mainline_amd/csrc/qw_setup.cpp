@@ -511,12 +511,29 @@ SysTables read_deck(const std::string& path, bool* unsup) {
     }
     if (!keyword(wf1, "SLATER")) unsupported("WF1 must be a SLATER determinant");
     Words dw, st, orb;
-    if (keyword(wf1, "OPTIMIZE_MO") || keyword(wf1, "CSF")) unsupported("OPTIMIZE_MO / CSF are not supported");
-    if (!section0(wf1, dw, "DETWT") || dw.empty()) bad("need DETWT");
-    T.ndet = (int)dw.size();
+    if (keyword(wf1, "OPTIMIZE_MO")) unsupported("OPTIMIZE_MO is not supported");
+    // determinant weights: DETWT, or CSF sections { c w_1 w_2 ... } giving c * w_j to the next
+    // determinants in STATES order (src/wavefunction/Slat_wf_data.cpp:69-118)
+    T.detwt.clear();
+    {
+      size_t cp = 0;
+      Words cs;
+      int ncsf = 0;
+      while (section(wf1, cp, cs, "CSF")) {
+        if (cs.size() < 2) bad("a CSF needs a coefficient and at least one determinant weight");
+        const double c = std::atof(cs[0].c_str());
+        for (size_t j = 1; j < cs.size(); j++) T.detwt.push_back(c * std::atof(cs[j].c_str()));
+        ncsf++;
+      }
+      if (ncsf) {
+        if (section0(wf1, dw, "DETWT")) bad("CSF is incompatible with DETWT");
+      } else {
+        if (!section0(wf1, dw, "DETWT") || dw.empty()) bad("need DETWT");
+        for (auto& x : dw) T.detwt.push_back(std::atof(x.c_str()));
+      }
+    }
+    T.ndet = (int)T.detwt.size();
     if (T.ndet > kMaxDet) unsupported("at most " + std::to_string(kMaxDet) + " determinants are supported");
-    T.detwt.resize(T.ndet);
-    for (int d = 0; d < T.ndet; d++) T.detwt[d] = std::atof(dw[d].c_str());
     if (T.detwt[0] == 0.0) bad("the first determinant must have a nonzero weight");
     if (!section0(wf1, st, "STATES")) bad("need STATES");
     if ((int)st.size() != T.ndet * T.ne()) bad("STATES must list one orbital per electron for every determinant");

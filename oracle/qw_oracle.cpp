@@ -883,11 +883,24 @@ void read_system(System& S, const Tok& words) {
 void read_slater(System& S, const Tok& wf1, const std::string& dir) {
   if (!haskeyword(wf1, "SLATER")) fail("WF1 must be SLATER");
   Tok detwt, states, orbs;
-  if (haskeyword(wf1, "CSF")) fail("CSF input is not supported");
-  if (!readsection0(wf1, detwt, "DETWT") || detwt.empty()) fail("need DETWT");
-  S.ndet = (int)detwt.size();
-  S.detwt.resize(S.ndet);
-  for (int d = 0; d < S.ndet; d++) S.detwt[d] = atof(detwt[d].c_str());
+  // DETWT, or CSF { c w_1 w_2 ... } sections: determinant weights c * w_j in STATES order
+  // (src/wavefunction/Slat_wf_data.cpp:69-118)
+  S.detwt.clear();
+  size_t cpos = 0;
+  Tok csf;
+  int ncsf = 0;
+  while (readsection(wf1, cpos, csf, "CSF")) {
+    if (csf.size() < 2) fail("wrong number of elements in a CSF");
+    for (size_t j = 1; j < csf.size(); j++) S.detwt.push_back(atof(csf[0].c_str()) * atof(csf[j].c_str()));
+    ncsf++;
+  }
+  if (ncsf) {
+    if (readsection0(wf1, detwt, "DETWT")) fail("CSF is incompatible with DETWT");
+  } else {
+    if (!readsection0(wf1, detwt, "DETWT") || detwt.empty()) fail("need DETWT");
+    for (auto& x : detwt) S.detwt.push_back(atof(x.c_str()));
+  }
+  S.ndet = (int)S.detwt.size();
   if (!readsection0(wf1, states, "STATES")) fail("need STATES");
   if ((int)states.size() != S.ndet * S.ne()) fail("STATES count must be ndet x the number of electrons");
   if (!readsection0(wf1, orbs, "ORBITALS")) fail("need ORBITALS");

@@ -20,6 +20,7 @@ Cases:
             determinant, with a POLYPADE Jastrow of nonzero coefficients
   n2_ci     the reference's tests/n2 CI wave function (28 determinants, qw.ci + cidet);
   n2_cij    the same with the POLYPADE Jastrow of n2_pp
+  n2_csf    the n2_ci expansion written as CSF sections
   h4, pp16, ae32   the seeded synthetic molecules of mainline_amd/synth.py (4, 64, 128 e)
   fg3       three atoms with every supported shell symmetry (S P 6D 7F 7F_crystal 10F 9G
             9G_pyscf 15G), 12 electrons
@@ -111,6 +112,23 @@ def clouds(centers, counts, k, seed, spread):
     return centers[idx][None] + spread * rng.standard_normal((k, len(idx), 3))
 
 
+def csf_text(cidet):
+    """CSF sections for a DETWT + STATES text: consecutive determinants of equal |weight|
+    grouped into CSF { |w| sign ... }, STATES kept."""
+    import re
+    body = re.sub(r"#[^\n]*", "", cidet)
+    w = [float(x) for x in re.search(r"DETWT\s*\{([^}]*)\}", body).group(1).split()]
+    states = re.search(r"STATES\s*\{[^}]*\}", body).group(0)
+    out, i = [], 0
+    while i < len(w):
+        j = i
+        while j + 1 < len(w) and abs(abs(w[j + 1]) - abs(w[i])) < 1e-15:
+            j += 1
+        out.append("CSF { %r %s }" % (abs(w[i]), " ".join("1" if w[k] > 0 else "-1" for k in range(i, j + 1))))
+        i = j + 1
+    return "\n".join(out) + "\n" + states + "\n"
+
+
 def ref_deck_cases(tmp):
     cases = {}
     # H atom (tests/h): qw.sys + qw.slater (+ qw.basis, qw.orb)
@@ -160,6 +178,12 @@ def ref_deck_cases(tmp):
     n2c = clouds([[0, 0, -1.60626709], [0, 0, 1.60626709]], [5, 5], 12, 17, 0.9)
     cases["n2_ci"] = (d, "n2ci.wfeval", n2c)
     cases["n2_cij"] = (d, "n2cij.wfeval", n2c)
+    # the same expansion as CSFs: runs of equal |weight| become one CSF { |w| +-1 ... }
+    with open(os.path.join(d, "cidet_csf"), "w") as f:
+        f.write(csf_text(open(os.path.join(d, "cidet")).read()))
+    with open(os.path.join(d, "n2csf.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC {\n" + slater.replace("include cidet", "include cidet_csf") + "}\n")
+    cases["n2_csf"] = (d, "n2csf.wfeval", n2c)
     return cases
 
 

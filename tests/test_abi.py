@@ -3,6 +3,7 @@ before it touches a device (no GPU needed)."""
 import os
 import re
 
+import numpy as np
 import pytest
 
 from mainline_amd import engine as E
@@ -31,8 +32,9 @@ def test_missing_deck_is_an_input_error():
 
 
 def test_determinant_input_checks(tmp_path):
-    """STATES must hold ndet x Ne orbitals (bad input); CSF input, and several determinants
-    with a mixed precision, are valid QWalk the engine does not run (unsupported)."""
+    """STATES must hold ndet x Ne orbitals, a CSF a coefficient and weights, and CSF excludes
+    DETWT (bad input); several determinants with a mixed precision are valid QWalk the
+    engine does not run (unsupported)."""
     g, deck = golden("h_atom")
     d = os.path.dirname(deck)
     base = open(os.path.join(d, "qw.slater")).read()
@@ -40,7 +42,8 @@ def test_determinant_input_checks(tmp_path):
         (tmp_path / f).write_text(open(os.path.join(d, f)).read())
     (tmp_path / "x.wfeval").write_text("INCLUDE qw.sys\nTRIALFUNC { INCLUDE qw.slater }\n")
     for txt, status in [(base.replace("DETWT { 1.0 }", "DETWT { 1.0 0.5 }"), -2),
-                        (base.replace("DETWT { 1.0 }", "CSF { 1.0 1.0 }"), -3)]:
+                        (base.replace("DETWT { 1.0 }", "CSF { 1.0 }"), -2),
+                        (base.replace("DETWT { 1.0 }", "DETWT { 1.0 } CSF { 1.0 1.0 }"), -2)]:
         (tmp_path / "qw.slater").write_text(txt)
         with pytest.raises(E.QwError) as ex:
             E.Engine(str(tmp_path / "x.wfeval"))
@@ -69,3 +72,11 @@ def test_method_read_tmoves_keywords(tmp_path):
     for text, want in [("", 0), ("TMOVES", 1), ("TMOVESSC", 2), ("TMOVESSC TMOVES", 1)]:
         p.write_text("METHOD { DMC TIMESTEP 0.01 %s }\n" % text)
         assert E.read_method(str(p)).tmoves == want
+
+
+def test_csf_fixture_is_the_same_expansion():
+    """The reference evaluates the CSF form of the N2 CI expansion exactly as its DETWT form."""
+    a, _ = golden("n2_ci")
+    b, _ = golden("n2_csf")
+    for k in ["logpsi", "sign", "grad", "lap", "kinetic", "vloc", "vnonloc"]:
+        assert np.array_equal(a[k], b[k]), k

@@ -94,3 +94,29 @@ def test_output_files_have_the_reference_layout(tmp_path):
             assert p["value"][0] == pytest.approx(blk[lname], rel=1e-5, abs=1e-9)
             assert p["error"][0] == 0
             assert p["sigma"][0] == pytest.approx(math.sqrt(blk[lname + "var"]), rel=1e-5, abs=1e-9)
+
+
+@pytest.mark.skipif(not os.path.exists(GOSLING), reason="reference gosling not built (oracle/Makefile.ref)")
+def test_n2_ci_vmc_through_driver_and_gosling(tmp_path):
+    """The reference's N2 CI test (tests/n2/run_test.py: qw.ci with its 28-determinant cidet,
+    BFD pseudopotential) through the driver, reblocked by the reference's gosling: total and
+    kinetic energies against GAMESS CI within 3 sigma (tests/n2/run_test.py:20-23,28-32)."""
+    d = os.path.join(GOLDEN, "decks", "n2_ci")
+    for f in ["qw.sys", "qw.basis", "qw.orb", "cidet"]:
+        shutil.copy(os.path.join(d, f), tmp_path)
+    (tmp_path / "qw.ci").write_text(
+        "method { vmc nblock 40 nstep 20 timestep 0.5 }\ninclude qw.sys\n"
+        "trialfunc { SLATER ORBITALS { CUTOFF_MO MAGNIFY 1 NMO 8 ORBFILE qw.orb INCLUDE qw.basis "
+        "CENTERS { USEGLOBAL } } include cidet }\n")
+    print("drop-in: N2 CI VMC", flush=True)
+    out = subprocess.run([DRIVER, "qw.ci", "--walkers", "2048"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=200)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    js = subprocess.run([GOSLING, "-json", "qw.ci.log"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert js.returncode == 0, js.stderr
+    props = json.loads(js.stdout)["properties"]
+    for key, ref in [("total_energy", -19.3080264611), ("kinetic", 12.2497549662)]:
+        v, e = props[key]["value"][0], props[key]["error"][0]
+        print("gosling", key, v, "+/-", e, flush=True)
+        assert abs(v - ref) <= 3 * e + (2e-3 if key == "total_energy" else 2e-2)

@@ -27,7 +27,7 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("name", ["n2_ci", "n2_cij"])
+@pytest.mark.parametrize("name", ["n2_ci", "n2_cij", "n2_csf"])
 def test_multidet_eval_matches_reference_golden(name):
     g, deck = golden(name)
     _say("multidet eval golden", name)
