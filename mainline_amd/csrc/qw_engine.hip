@@ -634,9 +634,11 @@ struct qw_engine {
   // Casula T-moves after the local energy of a DMC step (k_local wrote A.tneg)
   void run_tmove(const LocalArgs& A) {
     TmoveArgs t{tm_mode == 2 ? 1 : 0, A.pp_mode, A.tneg, A.seed, dstep, dctl};
-    const size_t lds = tmove_lds_bytes(S.ne, S.chunk_nf);
+    const size_t lds = md ? tmove_lds_bytes(S.ne, S.chunk_nf, S.ndet, S.nsig, S.nsp) : tmove_lds_bytes(S.ne, S.chunk_nf);
     launch("k_tmove", [&] {
-      if (precision == QW_PREC_FP64)
+      if (md)
+        hipLaunchKernelGGL((k_tmove<double, true>), dim3(nw), dim3(64), lds, stream, S, W, t);
+      else if (precision == QW_PREC_FP64)
         hipLaunchKernelGGL(k_tmove<double>, dim3(nw), dim3(64), lds, stream, S, W, t);
       else
         hipLaunchKernelGGL(k_tmove<float>, dim3(nw), dim3(64), lds, stream, S, W, t);
@@ -1208,7 +1210,6 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
     A.seed = e->seed;
     A.ctl = e->dctl;
     e->tm_mode = dmc && e->S.has_pp ? (m->tmoves == 1 ? 1 : m->tmoves == 2 ? 2 : 0) : 0;
-    if (e->tm_mode && e->md) throw Unsupported("T-moves are supported with one determinant only");
     if (e->tm_mode) e->ensure_tneg();
     HIPCHK(hipMemsetAsync(e->W.bsum, 0, (size_t)nw * kBsum * sizeof(double), e->stream));
     HIPCHK(hipMemsetAsync(e->W.acc, 0, (size_t)nw * sizeof(int), e->stream));

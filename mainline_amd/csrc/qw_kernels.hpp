@@ -1074,10 +1074,13 @@ struct TmoveArgs {
 // The quadrature point of atom a that the draw t selects for electron e, scanning the atom's
 // negative terms from the cumulative value c (the order and terms of k_local's nonlocal loop);
 // writes the new position.  All lanes take part (LCAO over the MOs, Jastrow over electrons).
-template <typename TM>
+// MD: several determinants, ratios as weighted sums over the determinants (weights bw, sum
+// den, the walker's inverses md_inv; phq: LDS row for the MO values at a quadrature point).
+template <typename TM, bool MD>
 __device__ inline void tmove_target(const DevSys& S, const DevWalkers& W, int w, const double* pos, int e, int a,
                                     double t, double c, double tau, const double rot[9], int pp_mode, double* chi,
-                                    int* flist, double np_out[3], int lane) {
+                                    int* flist, double np_out[3], const double* bw, double den,
+                                    const double* md_inv, double* phq, int lane) {
   const int nsp = S.nsp, s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0), n = s ? S.ndn : S.nup;
   const QW_CONST PPType& P = cptr(S.pp)[cptr(S.atom_pp)[a]];
   const QW_CONST double* ap = cptr(S.atom_pos) + 3 * a;
@@ -1094,7 +1097,8 @@ __device__ inline void tmove_target(const DevSys& S, const DevWalkers& W, int w,
       for (int l = 0; l < P.L - 1; l++) v[l] /= strength;
   }
   const double u_old = jastrow_e_val(S, e, re, pos, lane);
-  const double ivk = lane < n ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
+  const double ivk =
+      (!MD && lane < n) ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
   int pick = -1;
   for (int q = 0; q < P.aip && pick < 0; q++) {
     const double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];
@@ -1104,8 +1108,17 @@ __device__ inline void tmove_target(const DevSys& S, const DevWalkers& W, int w,
     const double nd[3] = {np_[0] - ap[0], np_[1] - ap[1], np_[2] - ap[2]};
     const double nr = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
     const double cs = (nd[0] * od[0] + nd[1] * od[1] + nd[2] * od[2]) / (nr * orr);
-    const double phi = lcao_val(S, s, np_, chi, flist, lane);
-    const double Rq = wsum(ivk * phi);
+    double Rq;
+    if constexpr (MD) {
+      const double phm = lcao_val_n(S, s, S.nmo[s], np_, chi, flist, lane);
+      __syncthreads();
+      if (lane < S.nmo[s]) phq[lane] = phm;
+      __syncthreads();
+      md_contract<1>(S, md_inv, s, k, phq, nsp, bw, den, &Rq, lane);
+    } else {
+      const double phi = lcao_val(S, s, np_, chi, flist, lane);
+      Rq = wsum(ivk * phi);
+    }
     const double uq = jastrow_e_val(S, e, np_, pos, lane);
     const double ratio = Rq * exp(uq - u_old) * P.w[q];
     double ts = 0;
@@ -1129,7 +1142,7 @@ __device__ inline void tmove_target(const DevSys& S, const DevWalkers& W, int w,
 // The chosen terms are found again at the unmoved configuration (the atom from the sums, the
 // quadrature point by recomputing that atom's terms), then applied in electron order as
 // accepted moves: Sherman-Morrison on the inverse, the MO cache row, log|det| and sign.
-template <typename TM>
+template <typename TM, bool MD = false>
 __global__ __launch_bounds__(64) void k_tmove(DevSys S, DevWalkers W, TmoveArgs A) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, w = blockIdx.x;
@@ -1145,6 +1158,12 @@ __global__ __launch_bounds__(64) void k_tmove(DevSys S, DevWalkers W, TmoveArgs 
   double* xs = rowk + kWave;
   int* sela = (int*)(xs + ne);
   int* flist = sela + ne;
+  // MD: determinant logs/signs [2][2][nd], weights, quadrature / trial MO rows, rows k
+  double* mdl = (double*)(flist + ((S.chunk_nf + 1) & ~1));
+  double* bw = mdl + (MD ? 4 * S.ndet : 0);
+  double* phq = bw + (MD ? S.ndet : 0);
+  double* phn = phq + (MD ? nsp : 0);
+  double* rwk = phn + (MD ? 5 * nsp : 0);
   for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
   const unsigned step = *A.stepp;
   const double tau = A.ctl->tau;
@@ -1224,12 +1243,26 @@ __global__ __launch_bounds__(64) void k_tmove(DevSys S, DevWalkers W, TmoveArgs 
   __syncthreads();
   double rot[9];
   random_rotation(A.seed, (unsigned)w, step, rot);
+  double md_den = 1, md_l0, md_s0;
+  double* md_inv = nullptr;
+  const int nd = S.ndet, nsg = S.nsig;
+  if constexpr (MD) {
+    for (int i = lane; i < 2 * nd; i += kWave) {
+      mdl[i] = W.ldd[(size_t)w * 2 * nd + i];
+      mdl[2 * nd + i] = W.sgd[(size_t)w * 2 * nd + i];
+    }
+    __syncthreads();
+    md_den = md_weights(S, mdl, mdl + 2 * nd, bw, md_l0, md_s0, lane);
+    md_inv = W.invd + (size_t)w * 2 * nd * nsg * nsg;
+    __syncthreads();
+  }
   // targets at the unmoved configuration
   int* flst = flist;
   for (int e = 0; e < ne; e++) {
     if (sela[e] < 0) continue;
     double np_[3];
-    tmove_target<TM>(S, W, w, pos, e, sela[e], selt[e], selc[e], tau, rot, A.pp_mode, chi, flst, np_, lane);
+    tmove_target<TM, MD>(S, W, w, pos, e, sela[e], selt[e], selc[e], tau, rot, A.pp_mode, chi, flst, np_, bw, md_den,
+                         md_inv, phq, lane);
     if (lane == 0)
       for (int d = 0; d < 3; d++) newp[3 * e + d] = np_[d];
   }
@@ -1242,6 +1275,46 @@ __global__ __launch_bounds__(64) void k_tmove(DevSys S, DevWalkers W, TmoveArgs 
     const int s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0), n = s ? S.ndn : S.nup;
     const double np_[3] = {newp[3 * e], newp[3 * e + 1], newp[3 * e + 2]};
     double phi[5];
+    if constexpr (MD) {
+      // every determinant's inverse by Sherman-Morrison, as an accepted move of k_sweep_md
+      const int nm = S.nmo[s];
+      lcao_lap_n(S, s, nm, np_, chi, flist, phi, lane);
+      __syncthreads();
+      if (lane < nm)
+        for (int c = 0; c < 5; c++) phn[c * nsp + lane] = phi[c];
+      const QW_CONST int* oc = cptr(S.occ) + (size_t)s * nd * nsg;
+      for (int i = lane; i < nd * n; i += kWave) {
+        const int d = i / n, j = i % n;
+        rwk[d * nsg + j] = md_inv[(((size_t)s * nd + d) * nsg + k) * nsg + j];
+      }
+      __syncthreads();
+      for (int i = lane; i < nd * n; i += kWave) {
+        const int d = i / n, r = i % n;
+        const QW_CONST int* o = oc + (size_t)d * nsg;
+        const double* rk = rwk + d * nsg;
+        double* row = md_inv + (((size_t)s * nd + d) * nsg + r) * nsg;
+        double Rd = 0, tr = 0;
+        for (int j = 0; j < n; j++) {
+          const double vj = phn[o[j]];
+          Rd += rk[j] * vj;
+          tr += row[j] * vj;
+        }
+        if (r == k) {
+          const double f = 1.0 / Rd;
+          for (int j = 0; j < n; j++) row[j] = rk[j] * f;
+          W.ldd[(size_t)w * 2 * nd + s * nd + d] += log(fabs(Rd));
+          if (Rd < 0) W.sgd[(size_t)w * 2 * nd + s * nd + d] = -W.sgd[(size_t)w * 2 * nd + s * nd + d];
+        } else {
+          const double pk = -tr / Rd;
+          for (int j = 0; j < n; j++) row[j] += rk[j] * pk;
+        }
+      }
+      if (lane < nm)
+        for (int c = 0; c < 5; c++) mow[((size_t)e * 5 + c) * nsp + lane] = (TM)phn[c * nsp + lane];
+      if (lane < 3) pos[3 * e + lane] = np_[lane];
+      __syncthreads();
+      continue;
+    }
     lcao_lap(S, s, np_, chi, flist, phi, lane);
     TM* ig = invw + (size_t)s * nsp * nsp;
     const double rk = lane < n ? (double)ig[(size_t)k * nsp + lane] : 0.0;
@@ -1272,9 +1345,11 @@ __global__ __launch_bounds__(64) void k_tmove(DevSys S, DevWalkers W, TmoveArgs 
 }
 
 // Dynamic LDS of k_tmove.
-inline size_t tmove_lds_bytes(int ne, int chunk_nf) {
-  return ((size_t)3 * ne + 5 * chunk_nf + 3 * ne + 3 * ne + 2 * kWave) * sizeof(double) +
-         ((size_t)ne + chunk_nf) * sizeof(int) + 16;
+// With several determinants add (ndet, nsig, nsp): their logs, weights and row copies.
+inline size_t tmove_lds_bytes(int ne, int chunk_nf, int ndet = 0, int nsig = 0, int nsp = 0) {
+  const size_t md = ndet ? (size_t)5 * ndet + 6 * (size_t)nsp + (size_t)ndet * nsig + 1 : 0;
+  return ((size_t)3 * ne + 5 * chunk_nf + 3 * ne + 3 * ne + 2 * kWave + md) * sizeof(double) +
+         ((size_t)ne + chunk_nf + 1) * sizeof(int) + 16;
 }
 
 // ---------------------------------------------------------------- k_branch

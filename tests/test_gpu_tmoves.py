@@ -1,7 +1,8 @@
 """Casula T-moves on the device (k_local's negative nonlocal sums + k_tmove) against the
 oracle's restatement of Dmc_method::doTmove (src/method/Dmc_method.cpp:850-906): the same
 counter-based random streams give the same DMC trajectory, move by move, for TMOVES (one
-draw per walker) and TMOVESSC (one draw per electron); graph replay equals direct launches;
+draw per walker) and TMOVESSC (one draw per electron), with one determinant and with the
+reference's 28-determinant N2 CI expansion; graph replay equals direct launches;
 the mixed path tracks FP64 with common random numbers."""
 import sys
 
@@ -22,8 +23,8 @@ def _say(*a):
 
 
 def _case(name, nw):
-    if name == "n2_pp":
-        g, deck = golden("n2_pp")
+    if name in ("n2_pp", "n2_cij"):
+        g, deck = golden(name)
         pos = np.concatenate([g["pos"]] * (nw // len(g["pos"]) + 1))[:nw]
         return deck, pos, 0.05
     deck, mol = synth_deck(name)
@@ -31,7 +32,7 @@ def _case(name, nw):
 
 
 @pytest.mark.parametrize("tm", [1, 2])
-@pytest.mark.parametrize("name", ["n2_pp", "pp16"])
+@pytest.mark.parametrize("name", ["n2_pp", "pp16", "n2_cij"])
 def test_tmove_trajectory_matches_oracle(name, tm):
     nw = 16
     deck, pos, tau = _case(name, nw)
