@@ -715,6 +715,12 @@ struct System {
   int ndet = 1;
   std::vector<double> detwt;
   std::vector<std::vector<int>> occ[2];      // [spin][det][n]
+  // CSF structure: coefficient and determinant weights of each CSF, over consecutive
+  // determinants (DETWT input: one CSF {detwt, 1} per determinant); OPTIMIZE_DET makes the
+  // coefficients of CSFs 1.. parameters (src/wavefunction/Slat_wf_data.cpp:69-118,574-640)
+  std::vector<double> csf_c;
+  std::vector<std::vector<double>> csf_w;
+  bool optimize_det = false;
   // Jastrow
   std::vector<JGroup> groups;
   // PP
@@ -889,18 +895,30 @@ void read_slater(System& S, const Tok& wf1, const std::string& dir) {
   size_t cpos = 0;
   Tok csf;
   int ncsf = 0;
+  S.csf_c.clear();
+  S.csf_w.clear();
   while (readsection(wf1, cpos, csf, "CSF")) {
     if (csf.size() < 2) fail("wrong number of elements in a CSF");
-    for (size_t j = 1; j < csf.size(); j++) S.detwt.push_back(atof(csf[0].c_str()) * atof(csf[j].c_str()));
+    S.csf_c.push_back(atof(csf[0].c_str()));
+    S.csf_w.emplace_back();
+    for (size_t j = 1; j < csf.size(); j++) {
+      S.csf_w.back().push_back(atof(csf[j].c_str()));
+      S.detwt.push_back(atof(csf[0].c_str()) * atof(csf[j].c_str()));
+    }
     ncsf++;
   }
   if (ncsf) {
     if (readsection0(wf1, detwt, "DETWT")) fail("CSF is incompatible with DETWT");
   } else {
     if (!readsection0(wf1, detwt, "DETWT") || detwt.empty()) fail("need DETWT");
-    for (auto& x : detwt) S.detwt.push_back(atof(x.c_str()));
+    for (auto& x : detwt) {
+      S.detwt.push_back(atof(x.c_str()));
+      S.csf_c.push_back(S.detwt.back());
+      S.csf_w.push_back({1.0});
+    }
   }
   S.ndet = (int)S.detwt.size();
+  S.optimize_det = haskeyword(wf1, "OPTIMIZE_DET");
   if (!readsection0(wf1, states, "STATES")) fail("need STATES");
   if ((int)states.size() != S.ndet * S.ne()) fail("STATES count must be ndet x the number of electrons");
   if (!readsection0(wf1, orbs, "ORBITALS")) fail("need ORBITALS");
@@ -2081,24 +2099,72 @@ void parm_deriv(const System& S, Walker& w, PDeriv& out, JScratch& js) {
       if (!coef.empty()) pd_append(out, func, coef, ne);
     }
   }
-  // the determinant: no parameters, gradient of ln D per electron
-  std::vector<double> gD((size_t)ne * 3);
-  for (int e = 0; e < ne; e++) {
-    double l;
-    det_gradlap(S, w, e, &gD[3 * e], &l);
+  // the determinants: gradient of ln Psi_S per electron, and with OPTIMIZE_DET the CSF
+  // coefficients 1.. as parameters (src/wavefunction/Slat_wf.h:555-686): for CSF p with
+  // determinants j and weights w_j,
+  //   gradient(p) = sum_j w_j D_j / Psi_S,
+  //   gradderiv(p, e, c) = sum_j w_j D_j (d_c D_j / D_j - d_c Psi_S / Psi_S) / Psi_S,
+  // d_c = d/dx, d/dy, d/dz and the laplacian of electron e
+  std::vector<double> gD((size_t)ne * 3), lD(ne);
+  for (int e = 0; e < ne; e++) det_gradlap(S, w, e, &gD[3 * e], &lD[e]);
+  PDeriv sl;
+  if (S.optimize_det && S.csf_c.size() > 1) {
+    std::vector<double> b;
+    double l0, s0, den = 0;
+    det_terms(S, w, b, l0, s0);
+    for (double x : b) den += x;
+    // e_d = D_d / (c_0 D_0) without the weight c_d
+    std::vector<double> ed(S.ndet);
+    for (int d = 0; d < S.ndet; d++) {
+      const double l = w.logdet[0][d] + w.logdet[1][d], sg = w.sign[0][d] * w.sign[1][d];
+      ed[d] = sg * s0 * exp(l - l0);
+    }
+    const int ncsf = (int)S.csf_c.size();
+    sl.g.assign(ncsf - 1, 0.0);
+    sl.gd.assign((size_t)(ncsf - 1) * ne * 4, 0.0);
+    int det = (int)S.csf_w[0].size();
+    for (int c = 1; c < ncsf; c++)
+      for (double wj : S.csf_w[c]) {
+        sl.g[c - 1] += wj * ed[det] / den;
+        for (int e = 0; e < ne; e++) {
+          const int s = S.spin_of(e), n = S.nspin(s), k = e - s * S.nup, nm = S.nmo[s];
+          const double* r = morow(S, w, e);
+          const double* iv = &w.inv[s][det][(size_t)k * n];
+          for (int q = 0; q < 4; q++) {
+            double t = 0;
+            for (int j = 0; j < n; j++) t += r[(q + 1) * nm + S.occ[s][det][j]] * iv[j];
+            const double tot = q < 3 ? gD[3 * e + q] : lD[e];
+            sl.gd[((size_t)(c - 1) * ne + e) * 4 + q] += wj * ed[det] * (t - tot) / den;
+          }
+        }
+        det++;
+      }
+    for (int c = 1; c < ncsf; c++) sl.parms.push_back(S.csf_c[c]);
   }
-  const int np = (int)out.g.size();
-  if (np == 0) {
-    out.vg = gD;
-    return;
-  }
+  const int np = (int)out.g.size(), ns = (int)sl.g.size();
+  // extend_parm_deriv(Slater, Jastrow) (src/wavefunction/Wavefunction.cpp:27-113)
   for (int p = 0; p < np; p++)
     for (int e = 0; e < ne; e++) {
       double dot = 0;
       for (int d = 0; d < 3; d++) dot += out.gd[((size_t)p * ne + e) * 4 + d] * gD[3 * e + d];
       out.gd[((size_t)p * ne + e) * 4 + 3] += 2 * dot;
     }
-  for (int i = 0; i < 3 * ne; i++) out.vg[i] += gD[i];
+  for (int p = 0; p < ns && np > 0; p++)
+    for (int e = 0; e < ne; e++) {
+      double dot = 0;
+      for (int d = 0; d < 3; d++) dot += sl.gd[((size_t)p * ne + e) * 4 + d] * out.vg[3 * e + d];
+      sl.gd[((size_t)p * ne + e) * 4 + 3] += 2 * dot;
+    }
+  if (np == 0) {
+    out.vg = gD;
+  } else {
+    for (int i = 0; i < 3 * ne; i++) out.vg[i] += gD[i];
+  }
+  if (ns) {
+    out.g.insert(out.g.begin(), sl.g.begin(), sl.g.end());
+    out.gd.insert(out.gd.begin(), sl.gd.begin(), sl.gd.end());
+    out.parms.insert(out.parms.begin(), sl.parms.begin(), sl.parms.end());
+  }
 }
 
 extern "C" int orc_eval(orc_system* o, int K, const double* pos, const double* rot, int pp_mode,
@@ -2197,6 +2263,12 @@ extern "C" int orc_nparm(orc_system* o, double* values) {
 extern "C" int orc_set_parms(orc_system* o, const double* values) {
   System& S = o->S;
   int q = 0;
+  if (S.optimize_det && S.csf_c.size() > 1) {
+    for (size_t c = 1; c < S.csf_c.size(); c++) S.csf_c[c] = values[q++];
+    int det = 0;
+    for (size_t c = 0; c < S.csf_c.size(); c++)
+      for (double wj : S.csf_w[c]) S.detwt[det++] = S.csf_c[c] * wj;
+  }
   for (JGroup& G : S.groups) {
     if (G.has_one && !G.one_frozen)
       for (size_t sc = 0; sc < G.sec_label.size(); sc++)

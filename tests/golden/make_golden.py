@@ -21,6 +21,8 @@ Cases:
   n2_ci     the reference's tests/n2 CI wave function (28 determinants, qw.ci + cidet);
   n2_cij    the same with the POLYPADE Jastrow of n2_pp
   n2_csf    the n2_ci expansion written as CSF sections
+  n2_cio    n2_ci with OPTIMIZE_DET (the determinant weights 2.. as parameters)
+  n2_csfj   n2_csf with OPTIMIZE_DET and the Jastrow (CSF coefficients, then Jastrow parameters)
   h4, pp16, ae32   the seeded synthetic molecules of mainline_amd/synth.py (4, 64, 128 e)
   fg3       three atoms with every supported shell symmetry (S P 6D 7F 7F_crystal 10F 9G
             9G_pyscf 15G), 12 electrons
@@ -184,6 +186,15 @@ def ref_deck_cases(tmp):
     with open(os.path.join(d, "n2csf.wfeval"), "w") as f:
         f.write("INCLUDE qw.sys\nTRIALFUNC {\n" + slater.replace("include cidet", "include cidet_csf") + "}\n")
     cases["n2_csf"] = (d, "n2csf.wfeval", n2c)
+    # OPTIMIZE_DET: the CSF coefficients as parameters, with DETWT alone and with CSFs and a Jastrow
+    with open(os.path.join(d, "n2cio.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC {\n" + slater.replace("SLATER\n", "SLATER\n OPTIMIZE_DET\n") + "}\n")
+    with open(os.path.join(d, "n2csfj.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW\nWF1 {\n"
+                + slater.replace("SLATER\n", "SLATER\n OPTIMIZE_DET\n").replace("include cidet", "include cidet_csf")
+                + "}\nWF2 { INCLUDE n2_jast.jast2 }\n}\n")
+    cases["n2_cio"] = (d, "n2cio.wfeval", n2c)
+    cases["n2_csfj"] = (d, "n2csfj.wfeval", n2c)
     return cases
 
 
