@@ -15,10 +15,12 @@
  *                        at fixed configurations (src/wavefunction/Wavefunction.h:117-256,
  *                        src/properties/Properties_gather.cpp:40-74,
  *                        src/method/Test_method.cpp:128-176)
- *   qw_parm_count        Wavefunction_data::nparms + getVarParms for the linear Jastrow
- *                        parameters (src/wavefunction/Jastrow2_wf.cpp:477-520,678-687)
- *   qw_parm_deriv        Wavefunction::getParmDeriv of SLATER-JASTROW (one determinant) with
- *                        JASTROW2 (src/wavefunction/Slat_Jastrow.cpp:171-193,
+ *   qw_parm_count        Wavefunction_data::nparms + getVarParms for the CSF coefficients
+ *                        (OPTIMIZE_DET) and the linear Jastrow parameters
+ *                        (src/wavefunction/Slat_wf_data.cpp:574-604,
+ *                        src/wavefunction/Jastrow2_wf.cpp:477-520,678-687)
+ *   qw_parm_deriv        Wavefunction::getParmDeriv of SLATER(-JASTROW) with JASTROW2
+ *                        (src/wavefunction/Slat_wf.h:555-686, src/wavefunction/Slat_Jastrow.cpp:171-193,
  *                        src/wavefunction/Jastrow2_wf.cpp:1486-1553,1900-1945,
  *                        src/wavefunction/Wavefunction.cpp:27-113)
  *   qw_method_read       METHOD { VMC | DMC } keyword reading with the reference defaults
@@ -100,17 +102,21 @@ qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const double* ro
                         const double* pp_rand, double* logpsi, double* sign, double* grad,
                         double* lap, double* eparts);
 
-/* Linear Jastrow parameters in the reference's order (group by group: one-body coefficient
- * sections in input order, then two-body coefficients, TWOBODY_SPIN like before unlike;
- * FREEZE pieces have none): *np = their number; values[np] = their values when not NULL. */
+/* Linear parameters in the reference's order: with OPTIMIZE_DET and several determinants the
+ * coefficients of CSFs 2.. first (DETWT input: each determinant is a CSF), then the Jastrow's
+ * (group by group: one-body coefficient sections in input order, then two-body coefficients,
+ * TWOBODY_SPIN like before unlike; FREEZE pieces have none): *np = their number; values[np] =
+ * their values when not NULL. */
 qw_status qw_parm_count(const qw_engine* e, int* np, double* values);
 
-/* Derivatives of ln Psi with respect to the linear Jastrow parameters at fixed
+/* Derivatives of ln Psi with respect to the parameters of qw_parm_count at fixed
  * configurations pos[K][Ne][3] (the Parm_deriv_return of the reference):
  *   gradient[K][np]           d ln Psi / dp
- *   hessian[K][np][np]        gradient_i * gradient_j (may be NULL)
+ *   hessian[K][np][np]        gradient_i * gradient_j, zero between two CSF coefficients
+ *                             (may be NULL)
  *   gradderiv[K][np][Ne][4]   d/dp of grad_e ln Psi (3) and of lap_e Psi / Psi
- *   val_gradient[K][Ne][3]    gradient of ln D plus that of the Jastrow pieces with parameters
+ *   val_gradient[K][Ne][3]    gradient of ln Psi_S (the determinants) plus that of the Jastrow
+ *                             pieces with parameters
  * As in the reference, FREEZE pieces take no part in val_gradient or in the laplacian
  * cross term 2 gradderiv . val_gradient. */
 qw_status qw_parm_deriv(qw_engine* e, int K, const double* pos, double* gradient, double* hessian,

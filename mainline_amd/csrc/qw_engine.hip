@@ -314,6 +314,13 @@ struct qw_engine {
       d_parm = upload(pd);
       d_punit = upload(jrad_table<double>(units));
       d_patoms = upload(patoms);
+      // OPTIMIZE_DET: the coefficients of CSFs 1.. come first (Slat_Jastrow order)
+      if (T.optimize_det && T.ndet > 1 && T.csf_c.size() > 1) {
+        nslp = (int)T.csf_c.size() - 1;
+        parm_values.insert(parm_values.begin(), T.csf_c.begin() + 1, T.csf_c.end());
+        d_csf_d0 = upload(T.csf_d0);
+        d_csf_w = upload(T.csf_w);
+      }
     }
     S.ee = upload(jrad_table<double>(T.ee));
     S.eef = upload(jrad_table<float>(T.ee));
@@ -493,8 +500,11 @@ struct qw_engine {
   }
   bool inv_rows = false;  // 33..64 per spin: row-per-lane inversion instead of 8x8 blocks (QW_INV=rows)
   // linear Jastrow parameters (getParmDeriv)
-  int npar = 0;
+  int npar = 0;   // Jastrow parameters
+  int nslp = 0;   // CSF coefficients (OPTIMIZE_DET), before the Jastrow parameters
   std::vector<double> parm_values;
+  int* d_csf_d0 = nullptr;
+  double* d_csf_w = nullptr;
   JParmDev* d_parm = nullptr;
   JRad<double>* d_punit = nullptr;
   int* d_patoms = nullptr;
@@ -1033,9 +1043,9 @@ extern "C" qw_status qw_walkers_get(qw_engine* e, int n, double* pos, double* we
 
 extern "C" qw_status qw_parm_count(const qw_engine* e, int* np, double* values) {
   if (!e || !np) return QW_ERR_ARG;
-  *np = e->npar;
+  *np = e->nslp + e->npar;
   if (values)
-    for (int p = 0; p < e->npar; p++) values[p] = e->parm_values[p];
+    for (int p = 0; p < *np; p++) values[p] = e->parm_values[p];
   return QW_OK;
 }
 
@@ -1043,10 +1053,9 @@ extern "C" qw_status qw_parm_deriv(qw_engine* e, int K, const double* pos, doubl
                                    double* gradderiv, double* val_gradient) {
   if (!e || K <= 0 || !pos || !gradient || !gradderiv || !val_gradient) return QW_ERR_ARG;
   return guarded(e, [&] {
-    if (e->md) throw Unsupported("parameter derivatives are supported with one determinant only");
     e->ensure_walkers(K);
     e->dmc_ready = false;
-    const size_t ne = e->S.ne, np = (size_t)e->npar;
+    const size_t ne = e->S.ne, np = (size_t)e->npar, nps = (size_t)e->nslp, nt = np + nps;
     HIPCHK(hipMemcpyAsync(e->W.pos, pos, (size_t)K * ne * 3 * sizeof(double), hipMemcpyHostToDevice, e->stream));
     e->refresh_all();
     std::vector<void*> tmp;
@@ -1071,29 +1080,65 @@ extern "C" qw_status qw_parm_deriv(qw_engine* e, int K, const double* pos, doubl
     double* d_g = dalloc((size_t)K * np * sizeof(double));
     double* d_gd = dalloc((size_t)K * np * ne * 4 * sizeof(double));
     double* d_vg = dalloc((size_t)K * ne * 3 * sizeof(double));
-    const size_t lds = (size_t)9 * ne * sizeof(double);
+    double* d_vj = nps && np ? dalloc((size_t)K * ne * 3 * sizeof(double)) : nullptr;
+    const size_t lds = (size_t)9 * ne * sizeof(double) + (e->md ? (size_t)5 * e->S.ndet * sizeof(double) : 0);
     e->launch("k_parm_deriv", [&] {
-      if (e->precision == QW_PREC_FP64)
+      if (e->md)
+        hipLaunchKernelGGL((k_parm_deriv<double, true>), dim3(K), dim3(64), lds, e->stream, e->S, e->W,
+                           (const JParmDev*)e->d_parm, (const JRad<double>*)e->d_punit, (const int*)e->d_patoms,
+                           (int)np, K, d_g, d_gd, d_vg, d_vj);
+      else if (e->precision == QW_PREC_FP64)
         hipLaunchKernelGGL(k_parm_deriv<double>, dim3(K), dim3(64), lds, e->stream, e->S, e->W,
                            (const JParmDev*)e->d_parm, (const JRad<double>*)e->d_punit, (const int*)e->d_patoms,
-                           (int)np, K, d_g, d_gd, d_vg);
+                           (int)np, K, d_g, d_gd, d_vg, (double*)nullptr);
       else
         hipLaunchKernelGGL(k_parm_deriv<float>, dim3(K), dim3(64), lds, e->stream, e->S, e->W,
                            (const JParmDev*)e->d_parm, (const JRad<double>*)e->d_punit, (const int*)e->d_patoms,
-                           (int)np, K, d_g, d_gd, d_vg);
+                           (int)np, K, d_g, d_gd, d_vg, (double*)nullptr);
     });
+    double* d_gs = nps ? dalloc((size_t)K * nps * sizeof(double)) : nullptr;
+    double* d_gds = nps ? dalloc((size_t)K * nps * ne * 4 * sizeof(double)) : nullptr;
+    if (nps) {
+      e->launch("k_parm_det", [&] {
+        hipLaunchKernelGGL(k_parm_det, dim3(K), dim3(64), parm_det_lds_bytes((int)ne, e->S.ndet), e->stream, e->S,
+                           e->W, (const int*)e->d_csf_d0, (const double*)e->d_csf_w, (int)nps + 1, K,
+                           (const double*)d_vj, d_gs, d_gds);
+      });
+    }
+    std::vector<double> gj((size_t)K * np), gdj((size_t)K * np * ne * 4), gs((size_t)K * nps),
+        gds((size_t)K * nps * ne * 4);
     if (np) {
-      HIPCHK(hipMemcpyAsync(gradient, d_g, (size_t)K * np * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-      HIPCHK(hipMemcpyAsync(gradderiv, d_gd, (size_t)K * np * ne * 4 * sizeof(double), hipMemcpyDeviceToHost,
-                            e->stream));
+      HIPCHK(hipMemcpyAsync(gj.data(), d_g, gj.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipMemcpyAsync(gdj.data(), d_gd, gdj.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    }
+    if (nps) {
+      HIPCHK(hipMemcpyAsync(gs.data(), d_gs, gs.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+      HIPCHK(hipMemcpyAsync(gds.data(), d_gds, gds.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     }
     HIPCHK(hipMemcpyAsync(val_gradient, d_vg, (size_t)K * ne * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    // the caller's order: CSF coefficients, then Jastrow parameters
+    const size_t row = ne * 4;
+    for (int k = 0; k < K; k++) {
+      for (size_t p = 0; p < nps; p++) {
+        gradient[(size_t)k * nt + p] = gs[(size_t)k * nps + p];
+        std::copy(&gds[((size_t)k * nps + p) * row], &gds[((size_t)k * nps + p) * row] + row,
+                  gradderiv + ((size_t)k * nt + p) * row);
+      }
+      for (size_t p = 0; p < np; p++) {
+        gradient[(size_t)k * nt + nps + p] = gj[(size_t)k * np + p];
+        std::copy(&gdj[((size_t)k * np + p) * row], &gdj[((size_t)k * np + p) * row] + row,
+                  gradderiv + ((size_t)k * nt + nps + p) * row);
+      }
+    }
+    // hessian: g_i g_j, except between two CSF coefficients (zero), as the reference's
+    // Slater determinant leaves its block zero and extend_parm_deriv fills the rest
     if (hessian)
       for (int k = 0; k < K; k++)
-        for (size_t i = 0; i < np; i++)
-          for (size_t j = 0; j < np; j++)
-            hessian[((size_t)k * np + i) * np + j] = gradient[(size_t)k * np + i] * gradient[(size_t)k * np + j];
+        for (size_t i = 0; i < nt; i++)
+          for (size_t j = 0; j < nt; j++)
+            hessian[((size_t)k * nt + i) * nt + j] =
+                (i < nps && j < nps) ? 0.0 : gradient[(size_t)k * nt + i] * gradient[(size_t)k * nt + j];
     return QW_OK;
   });
 }

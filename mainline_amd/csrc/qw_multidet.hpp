@@ -240,4 +240,70 @@ __global__ __launch_bounds__(64) void k_sweep_md(DevSys S, DevWalkers W, SweepAr
   if (lane == 0) W.acc[w] += nacc;
 }
 
+// OPTIMIZE_DET: derivatives with respect to the coefficients of CSFs 1.. (CSF c covers
+// determinants csf_d0[c] .. csf_d0[c+1]-1 with weights csf_w), one wavefront per configuration
+// (src/wavefunction/Slat_wf.h:555-686, the cross term of src/wavefunction/Wavefunction.cpp:27-113):
+//   grad(p) = sum_j w_j D_j / Psi_S,
+//   gd(p, e, c) = sum_j w_j D_j (d_c D_j / D_j - d_c Psi_S / Psi_S) / Psi_S
+//                 (+ 2 gd(p, e, 0:3) . vj(e) on the laplacian when the Jastrow has parameters),
+// D_j = D_up,j D_dn,j, Psi_S = sum_d c_d D_d, d_c = d/dx, d/dy, d/dz, laplacian of electron e;
+// vj: gradient of the parameterised Jastrow pieces (from k_parm_deriv) or null.
+__global__ __launch_bounds__(64) void k_parm_det(DevSys S, DevWalkers W, const int* csf_d0, const double* csf_w,
+                                                 int ncsf, int nw, const double* vj, double* grad, double* gd) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x, w = blockIdx.x;
+  if (w >= nw) return;
+  const int ne = S.ne, nd = S.ndet, ns = S.nsig, nsp = S.nsp, np = ncsf - 1;
+  double* mdl = lds;          // [4][nd]
+  double* bw = mdl + 4 * nd;  // [nd]
+  double* ed = bw + nd;       // [nd] D_d / (c_0 D_0)
+  double* tot = ed + nd;      // [ne][4] determinant parts of Psi_S
+  for (int i = lane; i < 2 * nd; i += kWave) {
+    mdl[i] = W.ldd[(size_t)w * 2 * nd + i];
+    mdl[2 * nd + i] = W.sgd[(size_t)w * 2 * nd + i];
+  }
+  __syncthreads();
+  double l0, s0;
+  const double den = md_weights(S, mdl, mdl + 2 * nd, bw, l0, s0, lane);
+  for (int d = lane; d < nd; d += kWave)
+    ed[d] = mdl[2 * nd + d] * mdl[3 * nd + d] * s0 * exp((mdl[d] + mdl[nd + d]) - l0);
+  const double* minv = W.invd + (size_t)w * 2 * nd * ns * ns;
+  const double* mog = W.mo + (size_t)w * ne * 5 * nsp;
+  __syncthreads();
+  for (int e = 0; e < ne; e++) {
+    const int s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0);
+    double t[4];
+    md_contract<4>(S, minv, s, k, mog + ((size_t)e * 5 + 1) * nsp, nsp, bw, den, t, lane);
+    if (lane == 0)
+      for (int c = 0; c < 4; c++) tot[4 * e + c] = t[c];
+  }
+  __syncthreads();
+  for (int p = 0; p < np; p++) {
+    const int d0 = csf_d0[p + 1], d1 = csf_d0[p + 2];
+    if (lane == 0) {
+      double g = 0;
+      for (int j = d0; j < d1; j++) g += csf_w[j] * ed[j] / den;
+      grad[(size_t)w * np + p] = g;
+    }
+    for (int e = lane; e < ne; e += kWave) {
+      const int s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0), n = s ? S.ndn : S.nup;
+      double acc[4] = {0, 0, 0, 0};
+      for (int j = d0; j < d1; j++) {
+        const double* row = minv + (((size_t)s * nd + j) * ns + k) * ns;
+        const QW_CONST int* o = cptr(S.occ) + ((size_t)s * nd + j) * ns;
+        double G[4] = {0, 0, 0, 0};
+        for (int q = 0; q < n; q++)
+          for (int c = 0; c < 4; c++) G[c] += mog[((size_t)e * 5 + 1 + c) * nsp + o[q]] * row[q];
+        for (int c = 0; c < 4; c++) acc[c] += csf_w[j] * ed[j] * (G[c] - tot[4 * e + c]) / den;
+      }
+      if (vj) acc[3] += 2 * (acc[0] * vj[((size_t)w * ne + e) * 3] + acc[1] * vj[((size_t)w * ne + e) * 3 + 1] +
+                             acc[2] * vj[((size_t)w * ne + e) * 3 + 2]);
+      double* out = gd + (((size_t)w * np + p) * ne + e) * 4;
+      for (int c = 0; c < 4; c++) out[c] = acc[c];
+    }
+  }
+}
+
+inline size_t parm_det_lds_bytes(int ne, int ndet) { return ((size_t)6 * ndet + 4 * (size_t)ne) * sizeof(double) + 16; }
+
 }  // namespace qw

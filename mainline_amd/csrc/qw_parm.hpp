@@ -28,10 +28,12 @@ struct JParmDev {
 
 // One block (wavefront) per configuration; outputs grad[nw][np], gd[nw][np][ne][4],
 // vg[nw][ne][3].  LDS: positions, determinant gradients, Jastrow gradients (3 x 3 x ne doubles).
-template <typename TM>
+// MD: several determinants (the determinant gradient is their weighted sum); vjo, when set,
+// receives the gradient of the parameterised Jastrow pieces alone ([nw][ne][3]).
+template <typename TM, bool MD = false>
 __global__ __launch_bounds__(64) void k_parm_deriv(DevSys S, DevWalkers W, const JParmDev* P, const JRad<double>* U,
                                                    const int* patoms, int np, int nw, double* grad, double* gd,
-                                                   double* vg) {
+                                                   double* vg, double* vjo) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, w = blockIdx.x;
   if (w >= nw) return;
@@ -46,7 +48,27 @@ __global__ __launch_bounds__(64) void k_parm_deriv(DevSys S, DevWalkers W, const
   // determinant gradient per electron from the inverse and the MO cache: electron by
   // electron, lanes over the MOs
   const TM* mog = mo_cache<TM>(W) + (size_t)w * ne * 5 * nsp;
-  for (int s = 0; s < 2; s++) {
+  if constexpr (MD) {
+    double* mdl = vj + 3 * ne;  // [4][ndet] logs and signs, then [ndet] weights
+    double* bw = mdl + 4 * S.ndet;
+    for (int i = lane; i < 2 * S.ndet; i += kWave) {
+      mdl[i] = W.ldd[(size_t)w * 2 * S.ndet + i];
+      mdl[2 * S.ndet + i] = W.sgd[(size_t)w * 2 * S.ndet + i];
+    }
+    __syncthreads();
+    double l0, s0;
+    const double den = md_weights(S, mdl, mdl + 2 * S.ndet, bw, l0, s0, lane);
+    const double* minv = W.invd + (size_t)w * 2 * S.ndet * S.nsig * S.nsig;
+    __syncthreads();
+    for (int e = 0; e < ne; e++) {
+      const int s = e < S.nup ? 0 : 1, k = e - (s ? S.nup : 0);
+      double t[3];
+      md_contract<3>(S, minv, s, k, (const double*)mog + ((size_t)e * 5 + 1) * nsp, nsp, bw, den, t, lane);
+      if (lane == 0)
+        for (int d = 0; d < 3; d++) gD[3 * e + d] = t[d];
+    }
+  }
+  for (int s = 0; s < 2 && !MD; s++) {
     const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0;
     const TM* ig = inv_store<TM>(W) + ((size_t)w * 2 + s) * nsp * nsp;
     for (int k = 0; k < n; k++) {
@@ -124,6 +146,8 @@ __global__ __launch_bounds__(64) void k_parm_deriv(DevSys S, DevWalkers W, const
     v[0] = vx;
     v[1] = vy;
     v[2] = vz;
+    if (vjo)
+      for (int d = 0; d < 3; d++) vjo[((size_t)w * ne + e) * 3 + d] = vj[3 * e + d];
   }
 }
 

@@ -515,6 +515,9 @@ SysTables read_deck(const std::string& path, bool* unsup) {
     // determinant weights: DETWT, or CSF sections { c w_1 w_2 ... } giving c * w_j to the next
     // determinants in STATES order (src/wavefunction/Slat_wf_data.cpp:69-118)
     T.detwt.clear();
+    T.csf_c.clear();
+    T.csf_w.clear();
+    T.csf_d0.assign(1, 0);
     {
       size_t cp = 0;
       Words cs;
@@ -522,16 +525,27 @@ SysTables read_deck(const std::string& path, bool* unsup) {
       while (section(wf1, cp, cs, "CSF")) {
         if (cs.size() < 2) bad("a CSF needs a coefficient and at least one determinant weight");
         const double c = std::atof(cs[0].c_str());
-        for (size_t j = 1; j < cs.size(); j++) T.detwt.push_back(c * std::atof(cs[j].c_str()));
+        T.csf_c.push_back(c);
+        for (size_t j = 1; j < cs.size(); j++) {
+          T.csf_w.push_back(std::atof(cs[j].c_str()));
+          T.detwt.push_back(c * T.csf_w.back());
+        }
+        T.csf_d0.push_back((int)T.detwt.size());
         ncsf++;
       }
       if (ncsf) {
         if (section0(wf1, dw, "DETWT")) bad("CSF is incompatible with DETWT");
       } else {
         if (!section0(wf1, dw, "DETWT") || dw.empty()) bad("need DETWT");
-        for (auto& x : dw) T.detwt.push_back(std::atof(x.c_str()));
+        for (auto& x : dw) {
+          T.detwt.push_back(std::atof(x.c_str()));
+          T.csf_c.push_back(T.detwt.back());
+          T.csf_w.push_back(1.0);
+          T.csf_d0.push_back((int)T.detwt.size());
+        }
       }
     }
+    T.optimize_det = keyword(wf1, "OPTIMIZE_DET");
     T.ndet = (int)T.detwt.size();
     if (T.ndet > kMaxDet) unsupported("at most " + std::to_string(kMaxDet) + " determinants are supported");
     if (T.detwt[0] == 0.0) bad("the first determinant must have a nonzero weight");

@@ -83,6 +83,12 @@ struct SysTables {
   std::vector<double> detwt;
   std::vector<int> occ[2];
   int nmo[2] = {0, 0};
+  // CSFs over consecutive determinants (DETWT input: one CSF {detwt, 1} per determinant):
+  // coefficient, first determinant ([ncsf+1]) and each determinant's weight in its CSF;
+  // OPTIMIZE_DET makes the coefficients of CSFs 1.. parameters
+  std::vector<double> csf_c, csf_w;
+  std::vector<int> csf_d0;
+  bool optimize_det = false;
   std::vector<JObj> ee;           // electron-electron objects (merged over groups)
   std::vector<JObj> ei;           // electron-ion objects, atom ranges in atom_ei0
   std::vector<int> atom_ei0;      // [na+1]
