@@ -553,6 +553,15 @@ SysTables read_deck(const std::string& path, bool* unsup) {
     if ((int)st.size() != T.ndet * T.ne()) bad("STATES must list one orbital per electron for every determinant");
     if (!section0(wf1, orb, "ORBITALS")) unsupported("only ORBITALS (real MOs) are supported");
     if (!keyword(orb, "CUTOFF_MO")) unsupported("only CUTOFF_MO orbitals are supported");
+    {
+      // basis centers: the atoms (USEATOMS, or USEGLOBAL of a molecule); a centers file
+      // (READ) places basis functions away from the atoms (src/orbitals/Center_set.cpp:51-90)
+      Words cw;
+      if (section0(orb, cw, "CENTERS")) {
+        if (keyword(cw, "READ")) unsupported("CENTERS { READ file } (basis centers other than the atoms) is not supported");
+        if (!keyword(cw, "USEATOMS") && !keyword(cw, "USEGLOBAL")) bad("CENTERS needs USEATOMS, USEGLOBAL or READ");
+      }
+    }
     int nmo_tot = 0;
     if (!ivalue(orb, nmo_tot, "NMO")) bad("need NMO");
     double magnify = 1;

@@ -923,6 +923,10 @@ void read_slater(System& S, const Tok& wf1, const std::string& dir) {
   if ((int)states.size() != S.ndet * S.ne()) fail("STATES count must be ndet x the number of electrons");
   if (!readsection0(wf1, orbs, "ORBITALS")) fail("need ORBITALS");
   if (!haskeyword(orbs, "CUTOFF_MO")) fail("only CUTOFF_MO orbitals are supported");
+  {
+    Tok cw;
+    if (readsection0(orbs, cw, "CENTERS") && haskeyword(cw, "READ")) fail("CENTERS { READ } is not supported");
+  }
   std::string v;
   if (!readvalue(orbs, v, "NMO")) fail("need NMO");
   int nmo_total = atoi(v.c_str());

@@ -33,8 +33,8 @@ def test_missing_deck_is_an_input_error():
 
 def test_determinant_input_checks(tmp_path):
     """STATES must hold ndet x Ne orbitals, a CSF a coefficient and weights, and CSF excludes
-    DETWT (bad input); several determinants with a mixed precision are valid QWalk the
-    engine does not run (unsupported)."""
+    DETWT (bad input); basis centers from a file, and several determinants with a mixed
+    precision, are valid QWalk the engine does not run (unsupported)."""
     g, deck = golden("h_atom")
     d = os.path.dirname(deck)
     base = open(os.path.join(d, "qw.slater")).read()
@@ -43,7 +43,8 @@ def test_determinant_input_checks(tmp_path):
     (tmp_path / "x.wfeval").write_text("INCLUDE qw.sys\nTRIALFUNC { INCLUDE qw.slater }\n")
     for txt, status in [(base.replace("DETWT { 1.0 }", "DETWT { 1.0 0.5 }"), -2),
                         (base.replace("DETWT { 1.0 }", "CSF { 1.0 }"), -2),
-                        (base.replace("DETWT { 1.0 }", "DETWT { 1.0 } CSF { 1.0 1.0 }"), -2)]:
+                        (base.replace("DETWT { 1.0 }", "DETWT { 1.0 } CSF { 1.0 1.0 }"), -2),
+                        (re.sub(r"CENTERS\s*\{[^}]*\}", "CENTERS { READ qw.centers }", base, flags=re.I), -3)]:
         (tmp_path / "qw.slater").write_text(txt)
         with pytest.raises(E.QwError) as ex:
             E.Engine(str(tmp_path / "x.wfeval"))
