@@ -6,9 +6,11 @@ Workload: the seeded 128-electron molecule ae32 (32 Z=4 all-electron atoms, 64+6
 in HBM.  A step is one DMC step of the whole population: 128 single-electron moves per walker,
 the local energy, the weight update (and branching every fifth step).
 
-Prints one JSON line: the headline value is the mixed-precision rate when the mixed path is
-available, otherwise FP64; both are reported, with the roofline of the dominant kernel
-(the electron-move sweep) and a CPU baseline timed on this host.
+Prints one JSON line: the headline value is the fastest mixed-precision rate whose DMC energy
+agrees with FP64's (otherwise FP64); every precision is reported, with the roofline of the
+dominant kernel (the electron-move sweep), a CPU baseline timed on this host, the energy
+agreement (paired with FP64: a short stretch of the 128-electron workload, and equilibrated
+DMC of the reference's N2 deck) and the latencies of a stream of serving jobs.
 """
 import argparse
 import json
@@ -62,6 +64,37 @@ def run_gpu(deck, mol, precision, nw, steps, warmup, tau):
     moves = nw * mol.nelec * steps
     return dict(moves_per_s=moves / dt, seconds=dt, ms_per_step=1e3 * dt / steps, energy=float(res["eblock"][0, 0]),
                 acceptance=float(res["acceptance"]), kernels=kt, tuning=eng.tuning())
+
+
+N2_DECK = os.path.join(ROOT, "tests", "golden", "decks", "n2_pp", "n2.wfeval")
+
+
+def accuracy_leg_n2(precs, nw=4096, nblock=40, nstep=20, tau=0.02, nequil=200):
+    """The same paired comparison on a physical system with a stationary DMC energy: the
+    reference's N2 deck (tests/n2 basis, orbitals and BFD pseudopotential, with a Jastrow),
+    walkers from the reference's random guess, 50 VMC steps, nequil DMC steps, then nblock
+    blocks of nstep steps."""
+    from mainline_amd import engine as E
+    out = {}
+    for p in precs:
+        eng = E.Engine(N2_DECK, p, seed=99)
+        pos = eng.random_walkers(nw, seed=7)
+        eng.set_walkers(pos)
+        eng.run(E.method("VMC", nw, nblock=1, nstep=50, timestep=0.3))
+        eng.run(E.method("DMC", nw, nblock=1, nstep=nequil, timestep=tau))
+        eb = eng.run(E.method("DMC", nw, nblock=nblock, nstep=nstep, timestep=tau))["eblock"][:, 0]
+        out[p] = dict(mean=float(eb.mean()), err=float(eb.std(ddof=1) / np.sqrt(len(eb))), blocks=eb.tolist())
+        log("[accuracy N2 %s] E = %.5f +- %.5f" % (p, out[p]["mean"], out[p]["err"]))
+        eng.close()
+    ref = out.get("fp64")
+    for p, v in out.items():
+        if ref is None or p == "fp64":
+            continue
+        d = np.array(v["blocks"]) - np.array(ref["blocks"])
+        v["paired_delta"] = float(d.mean())
+        v["paired_err"] = float(d.std(ddof=1) / np.sqrt(len(d)))
+        v["agrees_2sigma"] = bool(abs(v["paired_delta"]) <= 2 * v["paired_err"])
+    return out
 
 
 def accuracy_leg(deck, mol, precs, nw, nblock, nstep, tau, nequil):
@@ -264,9 +297,10 @@ def main():
     for p in precs:
         results[p] = run_gpu(deck, mol, p, a.walkers, a.steps, a.warmup, tau)
         log("[%s] %.3e electron-moves/s, %.2f ms/step" % (p, results[p]["moves_per_s"], results[p]["ms_per_step"]))
-    acc = None
+    acc = acc_n2 = None
     if a.accuracy_blocks > 0:
         acc = accuracy_leg(deck, mol, list(results), a.walkers, a.accuracy_blocks, 10, tau, 40)
+        acc_n2 = accuracy_leg_n2(list(results))
     # headline: the fastest mixed precision whose energy agrees with FP64 within two combined
     # error bars (FP64 itself when none does or no mixed mode was run)
     head = "fp64" if "fp64" in results else precs[0]
@@ -317,6 +351,8 @@ def main():
             line[p]["over_fp64"] = results[p]["moves_per_s"] / results["fp64"]["moves_per_s"]
     if acc is not None:
         line["energy_agreement"] = acc
+    if acc_n2 is not None:
+        line["energy_agreement_n2"] = acc_n2
     if serving:
         line["serving"] = serving
         line["p50_job_latency_s"] = serving[head]["p50_s"]
