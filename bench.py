@@ -284,8 +284,8 @@ def main():
         copy_probe(a.copy_probe)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the walker engine runs on a single MI355X in this build; "
-              "multi-GPU sharding is described in DESIGN.md and not run here.", file=sys.stderr)
+        print("bench.py measures one GPU: the walker engine runs on a single MI355X; sharding the "
+              "walkers over several GPUs is not run by this benchmark.", file=sys.stderr)
         sys.exit(2)
     from mainline_amd import synth
     mol = synth.ae32()
