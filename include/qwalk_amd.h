@@ -24,7 +24,8 @@
  *                        src/wavefunction/Jastrow2_wf.cpp:1486-1553,1900-1945,
  *                        src/wavefunction/Wavefunction.cpp:27-113)
  *   qw_method_read       METHOD { VMC | DMC } keyword reading with the reference defaults
- *                        (src/method/Vmc_method.cpp:33-132, src/method/Dmc_method.cpp:33-160)
+ *   qw_method_count/     (src/method/Vmc_method.cpp:33-132, src/method/Dmc_method.cpp:33-160);
+ *   qw_method_read_at    every METHOD section in order (src/program_utils/ooqmc.cpp:148-170)
  *   qw_walkers_set/get   Config_save_point restorePos / savePos for a walker population
  *                        (src/method/Vmc_method.cpp:394-399,492)
  *   qw_run               Vmc_method::runWithVariables / Dmc_method::runWithVariables
@@ -81,6 +82,8 @@ typedef struct qw_method_desc {
   int pp_stochastic;        /* 1 = the reference's default stochastic nonlocal evaluation */
   int tmoves;               /* DMC: 0 = locality approximation only, 1 = TMOVES, 2 = TMOVESSC
                                (Casula T-moves, src/method/Dmc_method.cpp:67-73,850-906) */
+  double start_feedback;    /* DMC: feedback of the first trial energy, START_FEEDBACK
+                               (src/method/Dmc_method.cpp:97-98,771); <= 0 = use feedback */
 } qw_method_desc;
 
 /* Engine for the SYSTEM / PSEUDO / TRIALFUNC sections of a QWalk deck.  INCLUDE and ORBFILE
@@ -124,6 +127,14 @@ qw_status qw_parm_deriv(qw_engine* e, int K, const double* pos, double* gradient
 
 /* Read the first METHOD section of a deck into *out (reference defaults applied). */
 qw_status qw_method_read(const char* deck_path, qw_method_desc* out);
+
+/* A deck may hold several METHOD sections, run one after another on the same walkers
+ * (src/program_utils/ooqmc.cpp:148-170).  *n = their number. */
+qw_status qw_method_count(const char* deck_path, int* n);
+/* The index-th METHOD section (0-based) and its log label (LABEL, default "vmc" / "dmc";
+ * src/method/Vmc_method.cpp:65-66, src/method/Dmc_method.cpp:94-95) into label[label_len]
+ * (may be NULL). */
+qw_status qw_method_read_at(const char* deck_path, int index, qw_method_desc* out, char* label, int label_len);
 
 /* Walker population: pos[nw][Ne][3], weights[nw] (NULL = all 1). */
 qw_status qw_walkers_set(qw_engine* e, int nw, const double* pos, const double* weights);

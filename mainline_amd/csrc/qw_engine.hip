@@ -173,8 +173,17 @@ struct qw_engine {
     }
     return (T*)p;
   }
+  void drop_graph() {
+    if (gx) (void)hipGraphExecDestroy(gx);
+    gx = nullptr;
+    g_np = g_nw = g_tm = g_ppm = 0;
+    g_tau = 0;
+  }
   template <typename T>
   T* walloc(size_t n) {
+    // never inside a recorded interval: the allocation and its clearing memset would be
+    // replayed against memory this call does not own
+    if (capturing) throw DevErr("walker allocation while recording a graph");
     void* p = nullptr;
     {
       std::lock_guard<std::mutex> g(g_devmem_mu);
@@ -204,9 +213,10 @@ struct qw_engine {
     return e;
   }
   // Launch wrapper: records start/stop events around the launch on the engine stream.
+  bool quiet = false;  // launches issue no timing events (the allocation-free entry point)
   template <typename F>
   void launch(const char* name, F f) {
-    if (!timing || capturing) {
+    if (!timing || capturing || quiet) {
       f();
       HIPCHK(hipGetLastError());
       return;
@@ -412,6 +422,8 @@ struct qw_engine {
       return;
     }
     HIPCHK(hipStreamSynchronize(stream));
+    // a recorded DMC interval holds the old walker pointers: it is recorded again on next use
+    drop_graph();
     {
       std::lock_guard<std::mutex> g(g_devmem_mu);
       for (void* p : walker_allocs) (void)hipFree(p);
@@ -638,6 +650,7 @@ struct qw_engine {
   }
   void ensure_tneg() {
     if (tneg && tnegcap >= nw) return;
+    drop_graph();
     tneg = walloc<double>((size_t)nw * S.na * S.ne);
     tnegcap = nw;
   }
@@ -1212,22 +1225,50 @@ extern "C" qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const
   });
 }
 
+namespace {
+void method_to_abi(const MethodDesc& d, qw_method_desc* out) {
+  out->method = d.method;
+  out->nconfig = d.nconfig;
+  out->nblock = d.nblock;
+  out->nstep = d.nstep;
+  out->ndecorr = d.ndecorr;
+  out->timestep = d.timestep;
+  out->feedback_interval = d.feedback_interval;
+  out->feedback = d.feedback;
+  out->max_weight = d.max_weight;
+  out->branch_start_cutoff = d.branch_start_cutoff;
+  out->pp_stochastic = d.pp_stochastic;
+  out->tmoves = d.tmoves;
+  out->start_feedback = d.start_feedback;
+}
+}  // namespace
+
 extern "C" qw_status qw_method_read(const char* deck_path, qw_method_desc* out) {
-  if (!deck_path || !out) return QW_ERR_ARG;
+  return qw_method_read_at(deck_path, 0, out, nullptr, 0);
+}
+
+extern "C" qw_status qw_method_count(const char* deck_path, int* n) {
+  if (!deck_path || !n) return QW_ERR_ARG;
   try {
-    MethodDesc d = read_method(deck_path);
-    out->method = d.method;
-    out->nconfig = d.nconfig;
-    out->nblock = d.nblock;
-    out->nstep = d.nstep;
-    out->ndecorr = d.ndecorr;
-    out->timestep = d.timestep;
-    out->feedback_interval = d.feedback_interval;
-    out->feedback = d.feedback;
-    out->max_weight = d.max_weight;
-    out->branch_start_cutoff = d.branch_start_cutoff;
-    out->pp_stochastic = d.pp_stochastic;
-    out->tmoves = d.tmoves;
+    *n = (int)read_methods(deck_path).size();
+    return QW_OK;
+  } catch (std::exception& x) {
+    g_create_error = x.what();
+    return QW_ERR_INPUT;
+  }
+}
+
+extern "C" qw_status qw_method_read_at(const char* deck_path, int index, qw_method_desc* out, char* label,
+                                       int label_len) {
+  if (!deck_path || !out || index < 0) return QW_ERR_ARG;
+  try {
+    std::vector<MethodDesc> all = read_methods(deck_path);
+    if (index >= (int)all.size()) {
+      g_create_error = "no METHOD section " + std::to_string(index);
+      return QW_ERR_ARG;
+    }
+    method_to_abi(all[index], out);
+    if (label && label_len > 0) snprintf(label, (size_t)label_len, "%s", all[index].label.c_str());
     return QW_OK;
   } catch (std::exception& x) {
     g_create_error = x.what();
@@ -1289,7 +1330,8 @@ extern "C" qw_status qw_run(qw_engine* e, const qw_method_desc* m, double* ebloc
       ctl.max_weight = m->max_weight;
       ctl.tau = tau;
       ctl.teff = tau;
-      ctl.etrial = ea - m->feedback * std::log(tw / nw);
+      // first trial energy with START_FEEDBACK (src/method/Dmc_method.cpp:771,780-787)
+      ctl.etrial = ea - (m->start_feedback > 0 ? m->start_feedback : m->feedback) * std::log(tw / nw);
       HIPCHK(hipMemcpyAsync(e->dctl, &ctl, sizeof(DmcCtl), hipMemcpyHostToDevice, e->stream));
     }
     e->step++;

@@ -802,10 +802,8 @@ SysTables read_deck(const std::string& path, bool* unsup) {
   }
 }
 
-MethodDesc read_method(const std::string& path) {
-  Words all, m;
-  lex(path, all);
-  if (!section0(all, m, "METHOD")) throw std::runtime_error("no METHOD section in " + path);
+// One METHOD section (src/method/Vmc_method.cpp:33-80, src/method/Dmc_method.cpp:33-104).
+MethodDesc method_of(const Words& m) {
   MethodDesc d;
   if (m.empty()) throw std::runtime_error("empty METHOD section");
   if (same(m[0], "VMC")) {
@@ -815,10 +813,12 @@ MethodDesc read_method(const std::string& path) {
     d.nconfig = 1;
     d.ndecorr = 2;
     d.timestep = -1;  // the reference tunes it when absent
+    d.label = "vmc";
     dvalue(m, d.timestep, "TIMESTEP");
     ivalue(m, d.ndecorr, "NDECORR");
   } else if (same(m[0], "DMC")) {
     d.method = 1;
+    d.label = "dmc";
     if (!dvalue(m, d.timestep, "TIMESTEP")) throw std::runtime_error("DMC needs TIMESTEP");
     d.nblock = 100;
     int target = 2048;
@@ -826,8 +826,10 @@ MethodDesc read_method(const std::string& path) {
     d.nconfig = std::max(target, 1);
     d.nstep = std::max(int(1.0 / d.timestep + 0.5), 1);
     dvalue(m, d.max_weight, "MAX_POSS_WEIGHT");
-    ivalue(m, d.feedback_interval, "FEEDBACK_INTERVAL");
+    if (ivalue(m, d.feedback_interval, "FEEDBACK_INTERVAL") && d.feedback_interval < 1)
+      throw std::runtime_error("FEEDBACK_INTERVAL must be greater than or equal to 1");
     dvalue(m, d.feedback, "FEEDBACK");
+    dvalue(m, d.start_feedback, "START_FEEDBACK");
     dvalue(m, d.branch_start_cutoff, "BRANCH_START_CUTOFF");
     // TMOVES takes precedence when both are given (src/method/Dmc_method.cpp:70-73,869)
     d.tmoves = keyword(m, "TMOVES") ? 1 : keyword(m, "TMOVESSC") ? 2 : 0;
@@ -839,7 +841,22 @@ MethodDesc read_method(const std::string& path) {
   ivalue(m, d.nblock, "NBLOCK");
   ivalue(m, d.nstep, "NSTEP");
   ivalue(m, d.nconfig, "NCONFIG");
+  value(m, d.label, "LABEL");
   return d;
 }
+
+// Every METHOD section of a deck, in input order (the reference runs them one after another,
+// src/program_utils/ooqmc.cpp:148-170).
+std::vector<MethodDesc> read_methods(const std::string& path) {
+  Words all, m;
+  lex(path, all);
+  std::vector<MethodDesc> out;
+  size_t pos = 0;
+  while (section(all, pos, m, "METHOD")) out.push_back(method_of(m));
+  if (out.empty()) throw std::runtime_error("no METHOD section in " + path);
+  return out;
+}
+
+MethodDesc read_method(const std::string& path) { return read_methods(path)[0]; }
 
 }  // namespace qw

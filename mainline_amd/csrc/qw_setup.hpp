@@ -113,11 +113,14 @@ struct MethodDesc {
   double branch_start_cutoff = 10.0;
   int pp_stochastic = 1;
   int tmoves = 0;  // 1 TMOVES, 2 TMOVESSC
+  double start_feedback = 1.0;  // DMC: feedback of the first trial energy
+  std::string label = "vmc";    // block-log label (LABEL)
 };
 
 // Throws std::runtime_error with a message on bad or unsupported input.
 // `unsupported` is set when the input is valid QWalk but outside this engine's scope.
 SysTables read_deck(const std::string& path, bool* unsupported);
 MethodDesc read_method(const std::string& path);
+std::vector<MethodDesc> read_methods(const std::string& path);
 
 }  // namespace qw

@@ -11,8 +11,12 @@
 // Walkers come from <deck>.config when it exists (READCONFIG), otherwise from the reference's
 // random guess followed by VMC equilibration (src/method/Generate_sample.cpp:24-71).
 //
+// Every METHOD section runs in input order on the same walkers (src/program_utils/ooqmc.cpp:148-170);
+// a VMC section without TIMESTEP gets a timestep tuned toward acceptance 0.7.
+//
 // usage: qwalk_amd <deck> [--precision fp64|mixed_fp32|mixed_bf16|mixed_bf16x6] [--device N]
 //                         [--seed S] [--walkers N]
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -138,86 +142,169 @@ void write_json(const std::string& path, const std::vector<double>& eb, int nblo
   }
 }
 
+// Mean acceptance of a short VMC segment at the given timestep (the walkers move on).
+bool vmc_acceptance(qw_engine* e, qw_method_desc vmc, double tau, int nstep, double* acc) {
+  vmc.method = 0;
+  vmc.nblock = 1;
+  vmc.nstep = nstep;
+  vmc.timestep = tau;
+  std::vector<double> tmp(QW_BLOCK_FIELDS);
+  double st[4];
+  if (qw_run(e, &vmc, tmp.data(), st) != QW_OK) return false;
+  *acc = st[0];
+  return true;
+}
+
+// VMC without TIMESTEP: the reference nudges the timestep by 0.01 after every move of its
+// warm-up chain, up while the move's acceptance exceeds 0.7 and down otherwise
+// (src/method/Generate_sample.cpp:40-53, src/method/Vmc_method.cpp:364-369).  With a whole
+// population the same target is reached by measuring the acceptance of short segments and
+// solving for the timestep whose mean acceptance is 0.7 (secant steps on log tau).
+bool tune_timestep(qw_engine* e, const qw_method_desc& vmc, int nw, int ne, double* tau_out) {
+  const double target = 0.7;
+  // segments of at least ~2e4 moves, so a segment's acceptance is known to about 0.005
+  const int nstep = std::max(4, std::min(2000, (int)std::ceil(2e4 / ((double)nw * ne))));
+  double t0 = 1.0, a0 = 0;
+  if (!vmc_acceptance(e, vmc, t0, nstep, &a0)) return false;
+  double t1 = a0 > target ? 2.0 : 0.5, a1 = 0;
+  for (int it = 0; it < 12; it++) {
+    if (!vmc_acceptance(e, vmc, t1, nstep, &a1)) return false;
+    if (std::fabs(a1 - target) < 0.01) break;
+    double l0 = std::log(t0), l1 = std::log(t1), l2;
+    if (std::fabs(a1 - a0) > 1e-6)
+      l2 = l1 + (target - a1) * (l1 - l0) / (a1 - a0);
+    else
+      l2 = l1 + (a1 > target ? 0.5 : -0.5);
+    l2 = std::max(l1 - 1.0, std::min(l1 + 1.0, l2));  // at most a factor e per step
+    t0 = t1;
+    a0 = a1;
+    t1 = std::exp(l2);
+  }
+  *tau_out = t1;
+  return true;
+}
+
+bool parse_precision(const std::string& v, int* p) {
+  if (v == "fp64") *p = QW_PREC_FP64;
+  else if (v == "mixed_fp32") *p = QW_PREC_MIXED_FP32;
+  else if (v == "mixed_bf16") *p = QW_PREC_MIXED_BF16;
+  else if (v == "mixed_bf16x6") *p = QW_PREC_MIXED_BF16X6;
+  else return false;
+  return true;
+}
+
+const char* kUsage =
+    "usage: qwalk_amd <deck> [--precision fp64|mixed_fp32|mixed_bf16|mixed_bf16x6] [--device N] [--seed S] "
+    "[--walkers N]\n";
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: qwalk_amd <deck> [--precision P] [--device N] [--seed S] [--walkers N]\n");
+    fprintf(stderr, "%s", kUsage);
     return 2;
   }
   std::string deck = argv[1];
   qw_engine_opts opts{QW_PREC_FP64, 0, 20261202ull, 0};
   int walkers_override = 0;
-  for (int i = 2; i + 1 < argc; i += 2) {
-    std::string k = argv[i], v = argv[i + 1];
+  for (int i = 2; i < argc; i += 2) {
+    std::string k = argv[i];
+    if (i + 1 >= argc) {
+      fprintf(stderr, "qwalk_amd: %s needs a value\n%s", k.c_str(), kUsage);
+      return 2;
+    }
+    std::string v = argv[i + 1];
     if (k == "--precision") {
-      opts.precision = v == "fp64" ? QW_PREC_FP64 : v == "mixed_fp32" ? QW_PREC_MIXED_FP32
-                       : v == "mixed_bf16x6" ? QW_PREC_MIXED_BF16X6 : QW_PREC_MIXED_BF16;
+      if (!parse_precision(v, &opts.precision)) {
+        fprintf(stderr, "qwalk_amd: unknown precision '%s'\n%s", v.c_str(), kUsage);
+        return 2;
+      }
     } else if (k == "--device") {
       opts.device = atoi(v.c_str());
     } else if (k == "--seed") {
       opts.seed = strtoull(v.c_str(), nullptr, 10);
     } else if (k == "--walkers") {
       walkers_override = atoi(v.c_str());
+    } else {
+      fprintf(stderr, "qwalk_amd: unknown option '%s'\n%s", k.c_str(), kUsage);
+      return 2;
     }
   }
-  qw_method_desc m;
-  if (qw_method_read(deck.c_str(), &m) != QW_OK) return fail("METHOD", nullptr);
+  int nmethod = 0;
+  if (qw_method_count(deck.c_str(), &nmethod) != QW_OK) return fail("METHOD", nullptr);
   qw_engine* e = nullptr;
   if (qw_engine_create(deck.c_str(), &opts, &e) != QW_OK) return fail("engine", nullptr);
   int info[6];
   qw_engine_info(e, info);
   const int ne = info[0] + info[1];
-  const bool dmc = m.method == 1;
-  if (walkers_override > 0) m.nconfig = walkers_override;
-  if (m.timestep <= 0) m.timestep = 0.5;  // the reference tunes it when TIMESTEP is absent
   const std::string cfg = deck + ".config";
   std::vector<double> pos, w;
-  bool have = read_configs(cfg, ne, pos, w);
-  int nw = m.nconfig;
-  if (have) {
-    // round-robin reuse of the stored walkers, as read_configurations distributes them
-    std::vector<double> p2((size_t)nw * ne * 3), w2(nw);
-    int nr = (int)w.size();
-    for (int i = 0; i < nw; i++) {
-      for (int k = 0; k < ne * 3; k++) p2[(size_t)i * ne * 3 + k] = pos[(size_t)(i % nr) * ne * 3 + k];
-      w2[i] = dmc ? w[i % nr] : 1.0;
+  int nw = 0;  // walkers resident on the device
+  for (int mi = 0; mi < nmethod; mi++) {
+    qw_method_desc m;
+    char label[64];
+    if (qw_method_read_at(deck.c_str(), mi, &m, label, sizeof label) != QW_OK) return fail("METHOD", nullptr);
+    const bool dmc = m.method == 1;
+    if (walkers_override > 0) m.nconfig = walkers_override;
+    // walkers: the previous method's population (the reference hands them over through the
+    // .config file), else <deck>.config (READCONFIG), else the random guess + VMC equilibration
+    std::vector<double> p2, w2;
+    bool have = false, fresh = false;
+    if (nw > 0) {
+      p2.resize((size_t)nw * ne * 3);
+      w2.resize(nw);
+      if (qw_walkers_get(e, nw, p2.data(), w2.data()) != QW_OK) return fail("walkers", e);
+      have = true;
+    } else {
+      have = read_configs(cfg, ne, p2, w2);
     }
-    pos.swap(p2);
-    w.swap(w2);
-  } else {
-    pos.assign((size_t)nw * ne * 3, 0.0);
-    w.assign(nw, 1.0);
-    if (qw_walkers_random(e, nw, opts.seed, 3.0, pos.data()) != QW_OK) return fail("walkers", e);
+    const int nwn = m.nconfig;
+    pos.assign((size_t)nwn * ne * 3, 0.0);
+    w.assign(nwn, 1.0);
+    if (have) {
+      // round-robin reuse of the stored walkers, as read_configurations distributes them
+      const int nr = (int)w2.size();
+      for (int i = 0; i < nwn; i++) {
+        for (int k = 0; k < ne * 3; k++) pos[(size_t)i * ne * 3 + k] = p2[(size_t)(i % nr) * ne * 3 + k];
+        w[i] = dmc ? w2[i % nr] : 1.0;
+      }
+    } else {
+      if (qw_walkers_random(e, nwn, opts.seed, 3.0, pos.data()) != QW_OK) return fail("walkers", e);
+      fresh = true;
+    }
+    nw = nwn;
+    if (qw_walkers_set(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
+    const bool auto_tau = m.timestep <= 0;
+    if (fresh) {
+      // equilibration from the random guess (VMC, depth-2 split moves; the reference's
+      // warm-up starts from timestep 1 when it tunes it)
+      qw_method_desc eq = m;
+      eq.method = 0;
+      eq.nblock = 1;
+      eq.nstep = 50;
+      eq.ndecorr = 2;
+      eq.timestep = auto_tau ? 1.0 : dmc ? std::max(m.timestep, 0.05) : m.timestep;
+      std::vector<double> tmp(QW_BLOCK_FIELDS);
+      if (qw_run(e, &eq, tmp.data(), nullptr) != QW_OK) return fail("equilibration", e);
+    }
+    if (auto_tau && !tune_timestep(e, m, nw, ne, &m.timestep)) return fail("timestep", e);
+    std::vector<double> eb((size_t)std::max(m.nblock, 1) * QW_BLOCK_FIELDS);
+    double stats[4];
+    if (qw_run(e, &m, eb.data(), stats) != QW_OK) return fail("run", e);
+    write_log(deck + ".log", label, m.timestep, dmc, eb, m.nblock);
+    write_json(deck + ".json", eb, m.nblock);
+    if (qw_walkers_get(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
+    write_configs(cfg, nw, ne, pos, w, dmc);
+    double s = 0, s2 = 0;
+    for (int b = 0; b < m.nblock; b++) s += eb[(size_t)b * QW_BLOCK_FIELDS];
+    s /= std::max(m.nblock, 1);
+    for (int b = 0; b < m.nblock; b++) s2 += std::pow(eb[(size_t)b * QW_BLOCK_FIELDS] - s, 2);
+    double err = m.nblock > 1 ? std::sqrt(s2 / (m.nblock - 1) / m.nblock) : 0.0;
+    printf("%s (%s): %d walkers, %d blocks x %d steps, timestep %g%s\n", dmc ? "DMC" : "VMC", label, nw, m.nblock,
+           m.nstep, m.timestep, auto_tau ? " (tuned)" : "");
+    printf("total_energy %.10f +/- %.10f\n", s, err);
+    printf("acceptance %.4f  electron-moves/s %.4e  wall %.2f s\n", stats[0], stats[3], stats[2]);
   }
-  if (qw_walkers_set(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
-  if (!have) {
-    // equilibration from the random guess (VMC, depth-2 split moves)
-    qw_method_desc eq = m;
-    eq.method = 0;
-    eq.nblock = 1;
-    eq.nstep = 50;
-    eq.ndecorr = 2;
-    eq.timestep = dmc ? std::max(m.timestep, 0.05) : m.timestep;
-    std::vector<double> tmp(QW_BLOCK_FIELDS);
-    if (qw_run(e, &eq, tmp.data(), nullptr) != QW_OK) return fail("equilibration", e);
-  }
-  std::vector<double> eb((size_t)std::max(m.nblock, 1) * QW_BLOCK_FIELDS);
-  double stats[4];
-  if (qw_run(e, &m, eb.data(), stats) != QW_OK) return fail("run", e);
-  write_log(deck + ".log", dmc ? "dmc" : "vmc", m.timestep, dmc, eb, m.nblock);
-  write_json(deck + ".json", eb, m.nblock);
-  if (qw_walkers_get(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
-  write_configs(cfg, nw, ne, pos, w, dmc);
-  double s = 0, s2 = 0;
-  for (int b = 0; b < m.nblock; b++) s += eb[(size_t)b * QW_BLOCK_FIELDS];
-  s /= std::max(m.nblock, 1);
-  for (int b = 0; b < m.nblock; b++) s2 += std::pow(eb[(size_t)b * QW_BLOCK_FIELDS] - s, 2);
-  double err = m.nblock > 1 ? std::sqrt(s2 / (m.nblock - 1) / m.nblock) : 0.0;
-  printf("%s: %d walkers, %d blocks x %d steps, timestep %g\n", dmc ? "DMC" : "VMC", nw, m.nblock, m.nstep,
-         m.timestep);
-  printf("total_energy %.10f +/- %.10f\n", s, err);
-  printf("acceptance %.4f  electron-moves/s %.4e  wall %.2f s\n", stats[0], stats[3], stats[2]);
   qw_engine_destroy(e);
   return 0;
 }

@@ -29,7 +29,7 @@ class MethodDesc(ctypes.Structure):
                 ("nstep", ctypes.c_int), ("ndecorr", ctypes.c_int), ("timestep", ctypes.c_double),
                 ("feedback_interval", ctypes.c_int), ("feedback", ctypes.c_double),
                 ("max_weight", ctypes.c_double), ("branch_start_cutoff", ctypes.c_double),
-                ("pp_stochastic", ctypes.c_int), ("tmoves", ctypes.c_int)]
+                ("pp_stochastic", ctypes.c_int), ("tmoves", ctypes.c_int), ("start_feedback", ctypes.c_double)]
 
 
 class ServerOpts(ctypes.Structure):
@@ -65,6 +65,9 @@ def lib():
         L.qw_eval_fixed.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
                                     _dp, _dp, _dp, _dp, _dp]
         L.qw_method_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(MethodDesc)]
+        L.qw_method_count.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
+        L.qw_method_read_at.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(MethodDesc), ctypes.c_char_p,
+                                        ctypes.c_int]
         L.qw_walkers_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.qw_walkers_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.qw_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(MethodDesc), _dp, _dp]
@@ -91,7 +94,7 @@ def exported_symbols():
     L = lib()
     names = []
     for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
-              "qw_method_read", "qw_walkers_set", "qw_walkers_get", "qw_walkers_random", "qw_run",
+              "qw_method_read", "qw_method_count", "qw_method_read_at", "qw_walkers_set", "qw_walkers_get", "qw_walkers_random", "qw_run",
               "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_parm_count",
               "qw_parm_deriv", "qw_server_create", "qw_server_submit",
               "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
@@ -112,11 +115,29 @@ def read_method(deck):
     return m
 
 
+def read_methods(deck):
+    """Every METHOD section of a deck in input order: [(MethodDesc, label)]."""
+    n = ctypes.c_int()
+    st = lib().qw_method_count(deck.encode(), ctypes.byref(n))
+    if st:
+        raise QwError(st, lib().qw_last_error(None).decode())
+    out = []
+    for i in range(n.value):
+        m = MethodDesc()
+        lab = ctypes.create_string_buffer(64)
+        st = lib().qw_method_read_at(deck.encode(), i, ctypes.byref(m), lab, 64)
+        if st:
+            raise QwError(st, lib().qw_last_error(None).decode())
+        out.append((m, lab.value.decode()))
+    return out
+
+
 def method(kind, nconfig, nblock, nstep, timestep, ndecorr=2, feedback_interval=5, feedback=1.0,
-           max_weight=7.0, branch_start_cutoff=10.0, pp_stochastic=1, tmoves=0):
+           max_weight=7.0, branch_start_cutoff=10.0, pp_stochastic=1, tmoves=0, start_feedback=1.0):
     """tmoves (DMC): 0 = off, 1 = TMOVES, 2 = TMOVESSC (Casula T-moves)."""
     return MethodDesc(1 if kind.upper() == "DMC" else 0, nconfig, nblock, nstep, ndecorr, timestep,
-                      feedback_interval, feedback, max_weight, branch_start_cutoff, pp_stochastic, tmoves)
+                      feedback_interval, feedback, max_weight, branch_start_cutoff, pp_stochastic, tmoves,
+                      start_feedback)
 
 
 class Engine:

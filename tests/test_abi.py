@@ -81,3 +81,32 @@ def test_csf_fixture_is_the_same_expansion():
     b, _ = golden("n2_csf")
     for k in ["logpsi", "sign", "grad", "lap", "kinetic", "vloc", "vnonloc"]:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_every_method_section_in_order(tmp_path):
+    """Several METHOD sections run one after another (src/program_utils/ooqmc.cpp:148-170):
+    each is read with its own defaults, LABEL (default vmc / dmc) and START_FEEDBACK
+    (default 1, src/method/Dmc_method.cpp:94-98)."""
+    p = tmp_path / "two.run"
+    p.write_text("METHOD { VMC NSTEP 7 }\nMETHOD { DMC TIMESTEP 0.02 LABEL mydmc START_FEEDBACK 0.5 }\n"
+                 "METHOD { DMC TIMESTEP 0.01 }\n")
+    ms = E.read_methods(str(p))
+    assert [(m.method, lab) for m, lab in ms] == [(0, "vmc"), (1, "mydmc"), (1, "dmc")]
+    assert ms[0][0].nstep == 7 and ms[0][0].timestep < 0  # no TIMESTEP: tuned by the driver
+    assert (ms[1][0].timestep, ms[1][0].start_feedback, ms[2][0].start_feedback) == (0.02, 0.5, 1.0)
+    assert E.read_method(str(p)).nstep == 7
+    p.write_text("METHOD { DMC TIMESTEP 0.02 FEEDBACK_INTERVAL 0 }\n")
+    with pytest.raises(E.QwError):
+        E.read_methods(str(p))
+
+
+def test_driver_rejects_unknown_precision(tmp_path):
+    """A misspelt --precision is an error, not a silent choice of another mode."""
+    import subprocess
+    driver = os.path.join(ROOT, "mainline_amd", "qwalk_amd")
+    if not os.path.exists(driver):
+        pytest.skip("driver not built")
+    out = subprocess.run([driver, str(tmp_path / "none.run"), "--precision", "fp32"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 2
+    assert "unknown precision" in out.stderr

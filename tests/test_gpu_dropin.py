@@ -120,3 +120,32 @@ def test_n2_ci_vmc_through_driver_and_gosling(tmp_path):
         v, e = props[key]["value"][0], props[key]["error"][0]
         print("gosling", key, v, "+/-", e, flush=True)
         assert abs(v - ref) <= 3 * e + (2e-3 if key == "total_energy" else 2e-2)
+
+
+@pytest.mark.skipif(not os.path.exists(GOSLING), reason="reference gosling not built (oracle/Makefile.ref)")
+def test_vmc_then_dmc_deck_through_driver_and_gosling(tmp_path):
+    """A deck with a VMC section (no TIMESTEP: tuned toward acceptance 0.7) followed by a DMC
+    section with its own LABEL: both run in order on the same walkers, and the reference's
+    gosling reblocks each label of the one log (HF -0.4998098113 and exact -0.5, 3 sigma)."""
+    d = os.path.join(GOLDEN, "decks", "h_atom")
+    for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb"]:
+        shutil.copy(os.path.join(d, f), tmp_path)
+    (tmp_path / "qw.run").write_text("method { VMC nconfig 256 nstep 100 nblock 20 }\n"
+                                     "method { DMC timestep 0.005 nblock 60 label hdmc }\n"
+                                     "include qw.sys\ntrialfunc { include qw.slater }\n")
+    print("drop-in: VMC then DMC", flush=True)
+    out = subprocess.run([DRIVER, "qw.run"], cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    assert "(tuned)" in out.stdout
+    acc = float(re.search(r"acceptance ([0-9.]+)", out.stdout).group(1))
+    assert abs(acc - 0.7) < 0.05
+    for label, ref in [("vmc", -0.4998098113), ("hdmc", -0.5)]:
+        js = subprocess.run([GOSLING, "-json", "-label", label, "qw.run.log"], cwd=tmp_path, capture_output=True,
+                            text=True, timeout=60)
+        assert js.returncode == 0, js.stderr
+        props = json.loads(js.stdout)["properties"]
+        v, e = props["total_energy"]["value"][0], props["total_energy"]["error"][0]
+        print("gosling", label, v, "+/-", e, flush=True)
+        assert e < 0.1
+        assert abs(v - ref) / e <= 3.0

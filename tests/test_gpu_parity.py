@@ -183,3 +183,32 @@ def test_block_and_row_inversions_are_bitwise_equal(prec, monkeypatch):
     _say("inversions", prec, out["rows"][0][:3], out["blocks"][0][:3])
     for a, b in zip(out["rows"], out["blocks"]):
         assert np.array_equal(a, b)
+
+
+def test_recorded_dmc_interval_follows_walker_reallocation():
+    """A DMC interval recorded into a graph holds the walker buffers' addresses.  A fixed-
+    configuration evaluation with more configurations than walkers reallocates those buffers;
+    the next DMC run on the original population must record the interval again and take
+    exactly the moves of an engine that never reallocated."""
+    deck, mol = synth_deck("h4")
+    nw = 64
+    pos = synth.random_configs(mol, nw, seed=31)
+    more = synth.random_configs(mol, 2 * nw, seed=32)
+    m = E.method("DMC", nw, nblock=1, nstep=6, timestep=0.2)
+    out = []
+    for realloc in (False, True):
+        _say("graph after reallocation:", realloc)
+        eng = E.Engine(deck, "fp64", seed=515)
+        assert eng.tuning()["graphs"] == 1
+        eng.set_walkers(pos)
+        eng.run(m)
+        if realloc:
+            eng.eval(more)
+        eng.set_walkers(pos)
+        r = eng.run(m)
+        p, w = eng.get_walkers(nw)
+        out.append((p, w, r["eblock"]))
+        eng.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
