@@ -65,7 +65,8 @@ typedef struct qw_engine_opts {
   int device;               /* HIP device ordinal */
   unsigned long long seed;  /* key of the counter-based random streams */
   int refresh_interval;     /* mixed precision: FP64 LU recompute every this many moves
-                               (0 = once per sweep of a spin channel) */
+                               (0 = the default, 32; a value >= the electrons of a spin =
+                               once per sweep of that spin channel) */
 } qw_engine_opts;
 
 typedef struct qw_method_desc {
@@ -139,6 +140,15 @@ qw_status qw_method_read_at(const char* deck_path, int index, qw_method_desc* ou
 /* Walker population: pos[nw][Ne][3], weights[nw] (NULL = all 1). */
 qw_status qw_walkers_set(qw_engine* e, int nw, const double* pos, const double* weights);
 qw_status qw_walkers_get(qw_engine* e, int nw, double* pos, double* weights);
+/* The resident walkers' determinant state and last local energy, as the method loop holds
+ * them (after Sherman-Morrison updates since the last recompute; the reference keeps the same
+ * quantities in each walker's Slat_wf storage and Properties_point,
+ * src/wavefunction/Slat_wf.h:293-401, src/properties/Properties_point.h:30-90):
+ *   logdet[nw][2] ln|det| per spin, sign[nw][2], elast[nw][4] = kinetic, Coulomb, nonlocal,
+ *   total of the last local-energy evaluation, dgl[nw][Ne][4] = grad_e ln|D| (3) and
+ *   lap_e D / D of the determinants, from the inverse the last sweep of each spin ended with
+ *   (src/wavefunction/Slat_wf.h:1021-1080).  Any pointer may be NULL. */
+qw_status qw_walkers_state(qw_engine* e, int nw, double* logdet, double* sign, double* elast, double* dgl);
 
 /* Fields of one block record written by qw_run (the quantities of a QWalk block log,
  * src/properties/Properties.cpp:578-641, src/properties/Properties_block.cpp:73-138):

@@ -21,6 +21,9 @@ namespace qw {
 
 constexpr int kBigThreads = 256;
 constexpr int kBigMaxSpin = 256;
+// Electrons of a spin up to which the inverse recompute is the hand-written k_inv_grid<16>
+// (above: the device solver library's batched LU).
+constexpr int kGridMaxSpin = 128;
 
 // Spline table element type that goes with an inverse / MO cache element type.
 template <typename TI>
@@ -77,10 +80,12 @@ struct BigPtrs {
   const void* spline;  // double or float table, matching TI
 };
 
+// The sweep of walker w by a 256-thread block (t = thread, lds0 = the block's LDS).
 template <typename TI, typename R, int DEPTH>
-__global__ __launch_bounds__(256) void k_sweep_big(DevSys S, DevWalkers W, BigPtrs<TI> P, SweepArgs A, int cap) {
-  extern __shared__ double lds[];
-  const int t = threadIdx.x, wv = t >> 6, lane = t & 63, w = blockIdx.x;
+__device__ __forceinline__ void sweep_big_body(const DevSys& S, const DevWalkers& W, const BigPtrs<TI>& P,
+                                               const SweepArgs& A, int cap, int w, int t, char* lds0) {
+  double* lds = (double*)lds0;
+  const int wv = t >> 6, lane = t & 63;
   if (w >= W.nw) return;
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
   double* pos = lds;
@@ -255,6 +260,12 @@ __global__ __launch_bounds__(256) void k_sweep_big(DevSys S, DevWalkers W, BigPt
   }
 }
 
+template <typename TI, typename R, int DEPTH>
+__global__ __launch_bounds__(256) void k_sweep_big(DevSys S, DevWalkers W, BigPtrs<TI> P, SweepArgs A, int cap) {
+  extern __shared__ double lds[];
+  sweep_big_body<TI, R, DEPTH>(S, W, P, A, cap, (int)blockIdx.x, (int)threadIdx.x, (char*)lds);
+}
+
 // Basis + LCAO of electron e of walker w (one 256-thread block per electron): wavefront 0
 // evaluates the active functions chunk by chunk, every thread accumulates its MO.
 template <typename T>
@@ -298,6 +309,234 @@ __global__ __launch_bounds__(256) void k_mo_rows_big(DevSys S, const double* wpo
     T* row = mo_out + ((size_t)w * S.ne + e) * 5 * nsp;
     for (int c = 0; c < 5; c++) row[(size_t)c * nsp + t] = acc[c];
   }
+}
+
+// ---------------------------------------------------------------- k_inv_grid
+// The pivoted Gauss-Jordan inversion of k_inv_blk for up to 8G electrons of a spin
+// (src/program_utils/MatrixAlgebrac.cpp:108-224,316-421: the pivot sequence of the reference's
+// Crout ludcmp), with the matrix spread over a G x G grid of threads, one walker-spin per block:
+// thread (bi, bj) = (t / G, t % G) holds rows 8bi..8bi+7 and columns 8bj..8bj+7 as one
+// register vector per column, as a lane of k_inv_blk does.  A step: the threads of column
+// block J propose their best unpivoted row (largest vv[i]*|a[i][j]|, last row on ties), each
+// wavefront keeps its highest-lane winner, the wavefronts' winners meet in LDS (the highest row
+// wins ties), then the pivot row and the multipliers go through LDS and every thread updates
+// its block with one FMA per element.  Every element sees the operations of k_inv_blk in the
+// same order, so the two agree bit for bit on matrices both handle; G = 16 (256 threads)
+// inverts up to 128 electrons of a spin in registers.
+template <int G>
+struct InvGridLds {
+  double prowL[2][8 * G];
+  double gL[2][8 * G];
+  double pvs[8 * G];
+  int pl[8 * G];
+  int qinv[8 * G];
+  double wbest[2][G * G / 64], wraw[2][G * G / 64];
+  int wrow[2][G * G / 64];
+  double redd[G * G / 64];
+  int redi[G * G / 64];
+};
+// The inversion of walker w's spin s by a G*G-thread block (t = thread).
+template <int G, typename TIN, typename TI>
+__device__ __forceinline__ void k_inv_grid_body(const DevSys& S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                                double* sgn_out, int w, int s, int t, InvGridLds<G>& Ls) {
+  static_assert(G == 8 || G == 16, "a row block's G threads share one 16-lane DPP row");
+  constexpr int NT = G * G, NWV = NT / 64, N8 = 8 * G;
+  typedef double dv8 __attribute__((ext_vector_type(8)));
+  auto& prowL = Ls.prowL;
+  auto& gL = Ls.gL;
+  double* pvs = Ls.pvs;
+  int* pl = Ls.pl;
+  int* qinv = Ls.qinv;
+  auto& wbest = Ls.wbest;
+  auto& wraw = Ls.wraw;
+  auto& wrow = Ls.wrow;
+  double* redd = Ls.redd;
+  int* redi = Ls.redi;
+  const int lane = t & 63, wv = t >> 6, bi = t / G, bj = t % G;
+  const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
+  TI* out = inv_out + ((size_t)w * 2 + s) * nsp * nsp;
+  if (n == 0) {
+    if (t == 0) {
+      logdet_out[2 * w + s] = 0;
+      sgn_out[2 * w + s] = 1;
+    }
+    return;
+  }
+  dv8 a[8];
+  double vv[8];
+  bool zero_row = false;
+  int pivm = 0;  // bit r: row 8bi + r already a pivot (rows >= n count as such)
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const int row = 8 * bi + r;
+    const TIN* rp = mo + ((size_t)w * S.ne + off + (row < n ? row : 0)) * 5 * nsp;
+    double mx = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int col = 8 * bj + c;
+      const double v = (row < n && col < n) ? (double)rp[col] : (row == col ? 1.0 : 0.0);
+      a[c][r] = v;
+      mx = fmax(mx, fabs(v));
+    }
+    // row maximum over the G threads of the row block (a contiguous group of G lanes)
+    mx = fmax(mx, dpp_d<kDppXor1>(mx));
+    mx = fmax(mx, dpp_d<kDppXor2>(mx));
+    mx = fmax(mx, dpp_d<kDppHalfMirror>(mx));
+    if constexpr (G == 16) mx = fmax(mx, dpp_d<kDppMirror>(mx));
+    vv[r] = mx == 0.0 ? 0.0 : 1.0 / mx;
+    zero_row |= row < n && mx == 0.0;
+    if (row >= n) pivm |= 1 << r;
+  }
+  if (lane == 0) redi[wv] = wave_any(zero_row) ? 1 : 0;
+  __syncthreads();
+  bool singular = false;
+#pragma unroll
+  for (int k = 0; k < NWV; k++) singular |= redi[k] != 0;
+  bool done = singular;
+  for (int J = 0; J < G && !done; J++) {
+    const bool mine = bj == J;  // this thread holds column 8J + c of its rows
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int j = 8 * J + c;
+      if (done || j >= n) {
+        done = true;
+        break;
+      }
+      double best = -1.0, braw = 0.0;
+      int br = 0;
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const double x = a[c][r];
+        const double cd = (mine && !((pivm >> r) & 1)) ? vv[r] * fabs(x) : -1.0;
+        if (cd >= best) {
+          best = cd;
+          br = r;
+          braw = x;
+        }
+      }
+      // this wavefront's winner (highest lane with the maximum), then the wavefronts'
+      const int L = wave_argmax_last(best, lane);
+      const int Lc = L < 0 ? 0 : L;
+      const double bb = wave_bcast(best, Lc), bw = wave_bcast(braw, Lc);
+      const int brr = __builtin_amdgcn_readlane(br, Lc);
+      if (lane == 0) {
+        wbest[j & 1][wv] = L < 0 ? __builtin_nan("") : bb;
+        wraw[j & 1][wv] = L < 0 ? 0.0 : bw;
+        wrow[j & 1][wv] = L < 0 ? -1 : 8 * ((64 * wv + L) / G) + brr;
+      }
+      __syncthreads();
+      double gb = -2.0;
+      int gw = -1;
+#pragma unroll
+      for (int k = 0; k < NWV; k++) {
+        const double b = wbest[j & 1][k];
+        if (b >= gb) {
+          gb = b;
+          gw = k;
+        }
+      }
+      const double pv = gw < 0 ? 0.0 : wraw[j & 1][gw];
+      if (gw < 0 || wrow[j & 1][gw] < 0 || !(gb >= 0.0) || pv == 0.0) {
+        singular = done = true;
+        break;
+      }
+      const int p = wrow[j & 1][gw];
+      const int pb = p >> 3, pr = p & 7;
+      if (t == 0) {
+        pl[j] = p;
+        pvs[j] = pv;
+      }
+      double* prow = prowL[j & 1];
+      double* gw8 = gL[j & 1];
+      if (bi == pb) {
+#pragma unroll
+        for (int c2 = 0; c2 < 8; c2++) prow[8 * bj + c2] = a[c2][pr];
+      }
+      const double rpv = 1.0 / pv;
+      if (mine) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) gw8[8 * bi + r] = (8 * bi + r == p) ? 0.0 : a[c][r] * rpv;
+      }
+      __syncthreads();
+      double pr8[8], g8[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        pr8[k] = prow[8 * bj + k];
+        g8[k] = gw8[8 * bi + k];
+      }
+#pragma unroll
+      for (int c2 = 0; c2 < 8; c2++)
+#pragma unroll
+        for (int r = 0; r < 8; r++) a[c2][r] = fma(-g8[r], pr8[c2], a[c2][r]);
+      if (mine) {
+#pragma unroll
+        for (int r = 0; r < 8; r++) a[c][r] = (8 * bi + r == p) ? 1.0 : -g8[r];
+      }
+      if (bi == pb) pivm |= 1 << pr;
+    }
+  }
+  if (singular) {
+    if (t == 0) {
+      logdet_out[2 * w + s] = -1e201;
+      sgn_out[2 * w + s] = 1;
+    }
+    for (int k = t; k < n * nsp; k += NT) out[k] = (TI)0;
+    return;
+  }
+  __syncthreads();
+  for (int j = t; j < n; j += NT) qinv[pl[j]] = j;
+  // log|det| (wavefront sums added in wavefront order), negative pivots and the inversion
+  // count of the pivot sequence
+  double lg = 0;
+  int neg = 0, inv_cnt = 0;
+  for (int j = t; j < n; j += NT) {
+    lg += log(fabs(pvs[j]));
+    neg += pvs[j] < 0;
+    const int pj = pl[j];
+    for (int m = j + 1; m < n; m++) inv_cnt += pl[m] < pj;
+  }
+  lg = wsum(lg);
+  const int par = wave_isum(neg + inv_cnt);
+  if (lane == 0) {
+    redd[wv] = lg;
+    redi[wv] = par;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double lt = redd[0];
+    int pt = redi[0];
+    for (int k = 1; k < NWV; k++) {
+      lt += redd[k];
+      pt += redi[k];
+    }
+    logdet_out[2 * w + s] = lt;
+    sgn_out[2 * w + s] = (pt & 1) ? -1.0 : 1.0;
+  }
+  // row 8bi+r was the pivot of step q: it holds row q of the inverse of the row-permuted
+  // matrix, scaled by 1/pivot_q; its column m goes to out[p_m][q]
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const int row = 8 * bi + r;
+    if (row < n) {
+      const int q = qinv[row];
+      const double sc = 1.0 / pvs[q];
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int m = 8 * bj + c;
+        if (m < n) out[(size_t)pl[m] * nsp + q] = (TI)(a[c][r] * sc);
+      }
+    }
+  }
+}
+
+template <int G, typename TIN, typename TI>
+__global__ __launch_bounds__(G * G) void k_inv_grid(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                                  double* sgn_out, int nw, int only_spin) {
+  __shared__ InvGridLds<G> L;
+  const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+  const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
+  if (w >= nw) return;
+  k_inv_grid_body<G>(S, mo, inv_out, logdet_out, sgn_out, w, s, (int)threadIdx.x, L);
 }
 
 // Column-major Slater matrices of spin s for the batched solver: work[w][c][r] = phi_c(r_r)

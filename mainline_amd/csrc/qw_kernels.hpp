@@ -127,16 +127,21 @@ __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) 
 // log|det| = sum log|pivot|; the sign comes from the pivots and the permutation parity; an
 // exactly zero pivot flags the matrix singular as the reference does (log|det| = -1e201,
 // inverse zero).
-template <int NS, typename TIN, typename TI>
-__global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
-                                            double* sgn_out, int nw, int only_spin) {
-  __shared__ double prow2[2][NS];
-  __shared__ double pvs[NS];
-  __shared__ int pl[NS];
-  const int lane = threadIdx.x;
-  const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
-  const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
-  if (w >= nw) return;
+// LDS of one wavefront's inversion (k_inv_rows_body).
+template <int NS>
+struct InvRowsLds {
+  double prow2[2][NS];
+  double pvs[NS];
+  int pl[NS];
+};
+// The inversion of walker w's spin s by one wavefront (WPB wavefronts per block, each on its
+// own matrix with its own LDS L; wavefront-level barriers when WPB > 1).
+template <int NS, int WPB, typename TIN, typename TI>
+__device__ inline void k_inv_rows_body(const DevSys& S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                       double* sgn_out, int w, int s, int lane, InvRowsLds<NS>& L) {
+  double(&prow2)[2][NS] = L.prow2;
+  double* pvs = L.pvs;
+  int* pl = L.pl;
   const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
   TI* out = inv_out + ((size_t)w * 2 + s) * nsp * nsp;
   if (n == 0) {
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
 #pragma unroll
           for (int k = 0; k < VW; k++) prow[b * VW + k] = a[b][k];
       }
-      __syncthreads();
+      walker_sync<WPB>();
       const double rpv = 1.0 / pv;
       const double g = me ? 0.0 : aj * rpv;
 #pragma unroll
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
     for (int k = lane; k < n * nsp; k += kWave) out[k] = (TI)0;
     return;
   }
-  __syncthreads();
+  walker_sync<WPB>();
   // log|det|, negative pivots and the inversion count of the pivot sequence
   double lg = 0;
   int neg = 0, inv_cnt = 0;
@@ -245,6 +250,16 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
   }
 }
 
+template <int NS, typename TIN, typename TI>
+__global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                            double* sgn_out, int nw, int only_spin) {
+  __shared__ InvRowsLds<NS> L;
+  const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+  const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
+  if (w >= nw) return;
+  k_inv_rows_body<NS, 1>(S, mo, inv_out, logdet_out, sgn_out, w, s, (int)threadIdx.x, L);
+}
+
 // ---------------------------------------------------------------- k_inv_blk
 // The same inversion for 33..64 electrons of a spin with the matrix spread over the wave as
 // 8x8 blocks: lane (bi, bj) = (lane / 8, lane % 8) holds rows 8bi..8bi+7 and columns
@@ -255,19 +270,23 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
 // pivots, multipliers a[i][j] * (1/pivot), one FMA per step, column j replaced by -multiplier
 // or 1), so both kernels give bitwise-equal results.  Rows and columns >= n hold the identity
 // and are never pivots before step n.
-template <typename TIN, typename TI>
-__global__ __launch_bounds__(64) void k_inv_blk(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
-                                                double* sgn_out, int nw, int only_spin) {
+struct InvBlkLds {
+  double prowL[2][64];
+  double gL[2][64];
+  double pvs[64];
+  int pl[64];
+  int qinv[64];
+};
+template <int WPB, typename TIN, typename TI>
+__device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                      double* sgn_out, int w, int s, int lane, InvBlkLds& L) {
   typedef double dv8 __attribute__((ext_vector_type(8)));
-  __shared__ double prowL[2][64];
-  __shared__ double gL[2][64];
-  __shared__ double pvs[64];
-  __shared__ int pl[64];
-  __shared__ int qinv[64];
-  const int lane = threadIdx.x, bi = lane >> 3, bj = lane & 7;
-  const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
-  const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
-  if (w >= nw) return;
+  double(&prowL)[2][64] = L.prowL;
+  double(&gL)[2][64] = L.gL;
+  double* pvs = L.pvs;
+  int* pl = L.pl;
+  int* qinv = L.qinv;
+  const int bi = lane >> 3, bj = lane & 7;
   const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
   TI* out = inv_out + ((size_t)w * 2 + s) * nsp * nsp;
   if (n == 0) {
@@ -348,7 +367,7 @@ __global__ __launch_bounds__(64) void k_inv_blk(DevSys S, const TIN* mo, TI* inv
 #pragma unroll
         for (int r = 0; r < 8; r++) gw[8 * bi + r] = (8 * bi + r == p) ? 0.0 : a[c][r] * rpv;
       }
-      __syncthreads();
+      walker_sync<WPB>();
       double pr8[8], g8[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
@@ -374,7 +393,7 @@ __global__ __launch_bounds__(64) void k_inv_blk(DevSys S, const TIN* mo, TI* inv
     for (int k = lane; k < n * nsp; k += kWave) out[k] = (TI)0;
     return;
   }
-  __syncthreads();
+  walker_sync<WPB>();
   for (int j = lane; j < n; j += kWave) qinv[pl[j]] = j;
   // log|det|, negative pivots and the inversion count of the pivot sequence
   double lg = 0;
@@ -391,7 +410,7 @@ __global__ __launch_bounds__(64) void k_inv_blk(DevSys S, const TIN* mo, TI* inv
     logdet_out[2 * w + s] = lg;
     sgn_out[2 * w + s] = (par & 1) ? -1.0 : 1.0;
   }
-  __syncthreads();
+  walker_sync<WPB>();
   // row 8bi+r was the pivot of step q: it holds row q of the inverse of the row-permuted
   // matrix, scaled by 1/pivot_q; its column m goes to out[p_m][q]
 #pragma unroll
@@ -407,6 +426,16 @@ __global__ __launch_bounds__(64) void k_inv_blk(DevSys S, const TIN* mo, TI* inv
       }
     }
   }
+}
+
+template <typename TIN, typename TI>
+__global__ __launch_bounds__(64) void k_inv_blk(DevSys S, const TIN* mo, TI* inv_out, double* logdet_out,
+                                                double* sgn_out, int nw, int only_spin) {
+  __shared__ InvBlkLds L;
+  const int w = only_spin < 0 ? (int)(blockIdx.x >> 1) : (int)blockIdx.x;
+  const int s = only_spin < 0 ? (int)(blockIdx.x & 1) : only_spin;
+  if (w >= nw) return;
+  k_inv_blk_body<1>(S, mo, inv_out, logdet_out, sgn_out, w, s, (int)threadIdx.x, L);
 }
 
 // ---------------------------------------------------------------- k_sweep
@@ -486,15 +515,20 @@ __host__ __device__ inline size_t sweep_lds_bytes(int ne, int na, int nsh, int n
 
 // U: coefficient loads in flight in the LCAO (chosen by the engine's online tuner; every U
 // gives the same arithmetic in the same order).
-template <int NS, int DEPTH, int U>
-__global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepArgs A) {
-  extern __shared__ double lds[];
-  const int lane = threadIdx.x, w = blockIdx.x;
+// The sweep of walker w by wavefront `wid` of a block of WPB wavefronts, each with a whole
+// region of sweep_lds_bytes of its own (lds0: the block's LDS) and wavefront-level barriers
+// when WPB > 1: the same arithmetic as a one-walker block.
+template <int NS, int DEPTH, int U, int WPB>
+__device__ __forceinline__ void sweep_fp64_body(const DevSys& S, const DevWalkers& W, const SweepArgs& A, int w,
+                                                int lane, int wid, char* lds0) {
   if (w >= W.nw) return;
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
-  double* pos = lds;
+  char* base = lds0;
+  if constexpr (WPB > 1)
+    base += (size_t)wid * ((sweep_lds_bytes(ne, S.na, S.nsh, S.ntab, NS, kCapF64, 8, DEPTH) + 15) & ~(size_t)15);
+  double* pos = (double*)base;
   SysLds V;
-  char* q = sys_lds_carve(S, (char*)lds + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
+  char* q = sys_lds_carve(S, base + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
   double* rnd = (double*)q;
   double* chi = rnd + sweep_rand_bytes(NS, DEPTH) / sizeof(double);
   int* roff = (int*)(chi + 5 * kCapF64);
@@ -520,7 +554,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
     m2 = mog[(e0 * 5 + 2) * nsp + lane];
     m3 = mog[(e0 * 5 + 3) * nsp + lane];
   }
-  __syncthreads();
+  walker_sync<WPB>();
   for (int k = A.k0; k < A.k1; k++) {
     const int e = off + k;
     double r0[3] = {pos[3 * e], pos[3 * e + 1], pos[3 * e + 2]};
@@ -528,7 +562,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
 #pragma unroll
       for (int j = 0; j < NS; j++) rowb[j] = inv[j];
     }
-    __syncthreads();
+    walker_sync<WPB>();
     const double rk = lane < n ? rowb[lane] : 0.0;
     // drift at the old position: determinant part from the cached MO row, Jastrow part fresh
     double d0[3], gj0[3];
@@ -547,7 +581,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
       double dr[3], rn[3], dn[3];
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (rq[d] * st + dr[d]);
-      mo_lap_point<double, double, U>(S, V, S.spline, S.C[s], n, rn, chi, roff, kCapF64, phi, lane);
+      mo_lap_point<double, double, U, WPB>(S, V, S.spline, S.C[s], n, rn, chi, roff, kCapF64, phi, lane);
       double t0 = wsum(rk * phi[0]);
       double t1 = wsum(rk * phi[1]), t2 = wsum(rk * phi[2]), t3 = wsum(rk * phi[3]);
       double gj[3];
@@ -592,7 +626,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
     if (accepted) {
       // Sherman-Morrison on the transposed inverse for the replaced row k
       if (lane < NS) phib[lane] = lane < n ? phi[0] : 0.0;
-      __syncthreads();
+      walker_sync<WPB>();
       if (lane < n) {
         // four independent partial sums: the dot product is not one serial FMA chain
         double tq[4] = {0, 0, 0, 0};
@@ -626,7 +660,7 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
       m2 = mog[(e1 * 5 + 2) * nsp + lane];
       m3 = mog[(e1 * 5 + 3) * nsp + lane];
     }
-    __syncthreads();
+    walker_sync<WPB>();
   }
   double* igw = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
   if (lane < n) {
@@ -642,6 +676,12 @@ __global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepAr
     W.sgn[2 * w + s] = sgn;
     W.acc[w] += nacc;
   }
+}
+
+template <int NS, int DEPTH, int U>
+__global__ __launch_bounds__(64, 2) void k_sweep(DevSys S, DevWalkers W, SweepArgs A) {
+  extern __shared__ double lds[];
+  sweep_fp64_body<NS, DEPTH, U, 1>(S, W, A, (int)blockIdx.x, (int)threadIdx.x, 0, (char*)lds);
 }
 
 // ---------------------------------------------------------------- k_local
@@ -784,10 +824,10 @@ __device__ inline void md_contract(const DevSys& S, const double* invw, int s, i
 
 // MD: several determinants (FP64): determinant parts and quadrature ratios are weighted sums
 // over the determinants (md_contract), from their inverses in W.invd.
+// The local energy of walker w by one 64-thread block (lds: the block's LDS).
 template <typename TM, bool MD = false>
-__global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs A) {
-  extern __shared__ double lds[];
-  const int lane = threadIdx.x, w = blockIdx.x;
+__device__ __forceinline__ void local_body(const DevSys& S, const DevWalkers& W, const LocalArgs& A, int w, int lane,
+                                           double* lds) {
   if (w >= W.nw) return;
   const int ne = S.ne, nsp = S.nsp;
   double* pos = lds;
@@ -1059,6 +1099,12 @@ __global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs 
       bs[14] += wt * wt;
     }
   }
+}
+
+template <typename TM, bool MD = false>
+__global__ __launch_bounds__(64) void k_local(DevSys S, DevWalkers W, LocalArgs A) {
+  extern __shared__ double lds[];
+  local_body<TM, MD>(S, W, A, (int)blockIdx.x, (int)threadIdx.x, lds);
 }
 
 // ---------------------------------------------------------------- k_tmove
@@ -1357,24 +1403,30 @@ inline size_t tmove_lds_bytes(int ne, int chunk_nf, int ndet = 0, int nsig = 0, 
 // fixed-population pair comb: sort by weight (ties by walker index), pair the p-th largest
 // weight >= 1.8 with the p-th smallest, keep one of the two with probability
 // w_big / (w_big + w_small), both slots get the mean weight.
-__global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsigned long long* keys, int* vals,
-                                                 int* pair_src, int* pair_dst, int nmoves, int do_branch,
-                                                 unsigned long long seed, unsigned event_host,
-                                                 const unsigned* eventp) {
+struct BranchLds {
+  double red[1024];
+  long long redi[1024];
+  int npair;
+};
+__device__ __forceinline__ void branch_body(const DevWalkers& W, DmcCtl* ctl, unsigned long long* keys, int* vals,
+                                            int* pair_src, int* pair_dst, int nmoves, int do_branch,
+                                            unsigned long long seed, unsigned event_host, const unsigned* eventp,
+                                            int t, BranchLds& L) {
   const unsigned event = eventp ? *eventp : event_host;
-  __shared__ double red[1024];
-  __shared__ long long redi[1024];
-  const int t = threadIdx.x, nw = W.nw;
+  double* red = L.red;
+  long long* redi = L.redi;
+  int& npair = L.npair;
+  const int nw = W.nw;
   double sw = 0;
   long long sa = 0;
-  for (int i = t; i < nw; i += blockDim.x) {
+  for (int i = t; i < nw; i += 1024) {
     sw += W.weight[i];
     sa += W.acc[i];
   }
   red[t] = sw;
   redi[t] = sa;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = 512; o > 0; o >>= 1) {
     if (t < o) {
       red[t] += red[t + o];
       redi[t] += redi[t + o];
@@ -1389,11 +1441,11 @@ __global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsi
     C.npairs = 0;
   }
   __syncthreads();
-  for (int i = t; i < nw; i += blockDim.x) W.acc[i] = 0;
+  for (int i = t; i < nw; i += 1024) W.acc[i] = 0;
   if (!do_branch) return;
   int P = 1;
   while (P < nw) P <<= 1;
-  for (int i = t; i < P; i += blockDim.x) {
+  for (int i = t; i < P; i += 1024) {
     if (i < nw) {
       keys[i] = (unsigned long long)__double_as_longlong(W.weight[i]);
       vals[i] = i;
@@ -1405,7 +1457,7 @@ __global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsi
   __syncthreads();
   for (int k = 2; k <= P; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = t; i < P; i += blockDim.x) {
+      for (int i = t; i < P; i += 1024) {
         int l = i ^ j;
         if (l > i) {
           bool up = (i & k) == 0;
@@ -1424,7 +1476,6 @@ __global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsi
     }
   }
   // pairs: p-th from the top with p-th from the bottom while the top weight >= 1.8
-  __shared__ int npair;
   if (t == 0) {
     int np = 0;
     for (int i = nw - 1; i > 1 && 2 * np + 1 < nw; i--, np++)
@@ -1433,7 +1484,7 @@ __global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsi
     C.npairs = np;
   }
   __syncthreads();
-  for (int p = t; p < npair; p += blockDim.x) {
+  for (int p = t; p < npair; p += 1024) {
     int big = vals[nw - 1 - p], sm = vals[p];
     double wb = W.weight[big], ws = W.weight[sm];
     double w1 = wb / (wb + ws);
@@ -1451,27 +1502,34 @@ __global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsi
   }
 }
 
+__global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsigned long long* keys, int* vals,
+                                                 int* pair_src, int* pair_dst, int nmoves, int do_branch,
+                                                 unsigned long long seed, unsigned event_host,
+                                                 const unsigned* eventp) {
+  __shared__ BranchLds L;
+  branch_body(W, ctl, keys, vals, pair_src, pair_dst, nmoves, do_branch, seed, event_host, eventp, (int)threadIdx.x, L);
+}
+
 // Copies the state of walker pair_src[p] into walker pair_dst[p] for p < ctl->npairs.
-__global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcCtl* ctl, const int* pair_src,
-                                              const int* pair_dst) {
-  const int p = blockIdx.x;
+__device__ __forceinline__ void copy_body(const DevSys& S, const DevWalkers& W, const DmcCtl* ctl,
+                                          const int* pair_src, const int* pair_dst, int p, int t) {
   if (p >= ctl->npairs) return;
-  const int src = pair_src[p], dst = pair_dst[p], t = threadIdx.x;
+  const int src = pair_src[p], dst = pair_dst[p];
   const int ne = S.ne, nsp = S.nsp;
   const size_t npos = (size_t)ne * 3, ninv = (size_t)2 * nsp * nsp, nmo = (size_t)ne * 5 * nsp;
-  for (size_t i = t; i < npos; i += blockDim.x) W.pos[dst * npos + i] = W.pos[src * npos + i];
+  for (size_t i = t; i < npos; i += 256) W.pos[dst * npos + i] = W.pos[src * npos + i];
   if (W.inv)
-    for (size_t i = t; i < ninv; i += blockDim.x) W.inv[dst * ninv + i] = W.inv[src * ninv + i];
+    for (size_t i = t; i < ninv; i += 256) W.inv[dst * ninv + i] = W.inv[src * ninv + i];
   if (W.invf)
-    for (size_t i = t; i < ninv; i += blockDim.x) W.invf[dst * ninv + i] = W.invf[src * ninv + i];
+    for (size_t i = t; i < ninv; i += 256) W.invf[dst * ninv + i] = W.invf[src * ninv + i];
   if (W.mo)
-    for (size_t i = t; i < nmo; i += blockDim.x) W.mo[dst * nmo + i] = W.mo[src * nmo + i];
+    for (size_t i = t; i < nmo; i += 256) W.mo[dst * nmo + i] = W.mo[src * nmo + i];
   if (W.mof)
-    for (size_t i = t; i < nmo; i += blockDim.x) W.mof[dst * nmo + i] = W.mof[src * nmo + i];
+    for (size_t i = t; i < nmo; i += 256) W.mof[dst * nmo + i] = W.mof[src * nmo + i];
   if (W.invd) {
     const size_t nd = (size_t)2 * S.ndet * S.nsig * S.nsig, nl = (size_t)2 * S.ndet;
-    for (size_t i = t; i < nd; i += blockDim.x) W.invd[dst * nd + i] = W.invd[src * nd + i];
-    for (size_t i = t; i < nl; i += blockDim.x) {
+    for (size_t i = t; i < nd; i += 256) W.invd[dst * nd + i] = W.invd[src * nd + i];
+    for (size_t i = t; i < nl; i += 256) {
       W.ldd[dst * nl + i] = W.ldd[src * nl + i];
       W.sgd[dst * nl + i] = W.sgd[src * nl + i];
     }
@@ -1482,6 +1540,11 @@ __global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcC
   }
   if (t < 4) W.elast[4 * (size_t)dst + t] = W.elast[4 * (size_t)src + t];
   if (t == 0) W.eold[dst] = W.eold[src];
+}
+
+__global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcCtl* ctl, const int* pair_src,
+                                              const int* pair_dst) {
+  copy_body(S, W, ctl, pair_src, pair_dst, (int)blockIdx.x, (int)threadIdx.x);
 }
 
 // Block sums over walkers: out[0..kBsum-1] = sums of the bsum columns, out[kBsum] = current

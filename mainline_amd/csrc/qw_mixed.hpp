@@ -314,20 +314,24 @@ __host__ __device__ inline size_t sweep_mx_lds_bytes(int ne, int na, int nsh, in
 // block first stages the spin's MO coefficients in LDS, so the LCAO of every move reads LDS
 // instead of L2; the walkers then run independently (wavefront-level barriers only).  The
 // arithmetic is the same for every WPB and U.
-template <int NS, int DEPTH, int MODE, int U, int WPB>
-__global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W,
-                                                                          SweepArgs A) {
-  static_assert(WPB == 1 || MODE == LCAO_FP32, "LDS-staged coefficients are FP32");
-  extern __shared__ double ldsd[];
-  const int lane = (int)threadIdx.x & (kWave - 1), wid = WPB == 1 ? 0 : (int)(threadIdx.x / kWave);
-  const int w = (int)blockIdx.x * WPB + wid;
+// The sweep of walker w by wavefront `wid` of a block of WPB wavefronts (lds0: the block's LDS).
+// STAGE: the block shares system tables and the LDS-staged coefficients (the layout above);
+// otherwise each wavefront has a whole region of its own (sweep_mx_lds_bytes with one walker)
+// and runs exactly as a one-walker block does, with wavefront-level barriers.
+template <int NS, int DEPTH, int MODE, int U, int WPB, bool STAGE>
+__device__ __forceinline__ void sweep_mx_body(const DevSys& S, const MixedTables& M, const DevWalkers& W,
+                                              const SweepArgs& A, int w, int lane, int wid, char* lds0) {
+  static_assert(WPB == 1 || MODE == LCAO_FP32, "several walkers per block: FP32 LCAO only");
+  static_assert(!STAGE || WPB > 1, "staged coefficients are shared by several walkers");
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
-  char* base = (char*)ldsd;
+  char* base = lds0;
   float* Cl = (float*)base;
-  if constexpr (WPB > 1) base += lds_round16((size_t)S.nb * nsp * 4);
+  if constexpr (STAGE) base += lds_round16((size_t)S.nb * nsp * 4);
+  if constexpr (!STAGE && WPB > 1)
+    base += (size_t)wid * sweep_mx_lds_bytes(ne, S.na, S.nsh, S.ntab, NS, MODE, M.ldchi, DEPTH);
   SysLds V;
   base = sys_lds_carve(S, base, V);
-  if constexpr (WPB > 1) {
+  if constexpr (STAGE) {
     // stage the coefficients (16-byte chunks while aligned) and the system tables
     const int tid = (int)threadIdx.x, nt = 64 * WPB, nc = S.nb * n;
     const float4* src4 = reinterpret_cast<const float4*>(M.Cf[s]);
@@ -343,11 +347,11 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys 
   if (w >= W.nw) return;
   double* pos = (double*)base;
   char* q = base + lds_round16((size_t)3 * ne * sizeof(double));
-  if constexpr (WPB > 1) {
+  if constexpr (STAGE) {
     V.actl = (int*)q;
     q += lds_round16((size_t)S.nsh * 4);
   }
-  constexpr int cap = WPB > 1 ? kCapCL : kCapF32;
+  constexpr int cap = STAGE ? kCapCL : kCapF32;
   double* rnd = (double*)q;
   q += sweep_rand_bytes(NS, DEPTH);
   float* chi = (float*)q;
@@ -405,10 +409,10 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys 
       double dr[3], rn[3], dn[3];
       lim_drift(d0, t, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (rq[d] * st + dr[d]);
-      if constexpr (MODE == LCAO_FP32 && WPB > 1)
+      if constexpr (MODE == LCAO_FP32 && STAGE)
         mo_lap_point<float, float, U, WPB>(S, V, M.spline, (const float*)Cl, n, rn, chi, flist, cap, phi, lane);
       else if constexpr (MODE == LCAO_FP32)
-        mo_lap_point<float, float, U>(S, V, M.spline, M.Cf[s], n, rn, chi, flist, cap, phi, lane);
+        mo_lap_point<float, float, U, WPB>(S, V, M.spline, M.Cf[s], n, rn, chi, flist, cap, phi, lane);
       else
         lcao_lap_mx<MODE>(S, M, s, rn, chi, phis, flist, phi, lane);
       float t0 = wsumf(rk * phi[0]);
@@ -509,6 +513,15 @@ __global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys 
     W.sgn[2 * w + s] = sgn;
     W.acc[w] += nacc;
   }
+}
+
+template <int NS, int DEPTH, int MODE, int U, int WPB>
+__global__ __launch_bounds__(64 * WPB, WPB == 1 ? 2 : 1) void k_sweep_mx(DevSys S, MixedTables M, DevWalkers W,
+                                                                          SweepArgs A) {
+  extern __shared__ double ldsd[];
+  const int lane = (int)threadIdx.x & (kWave - 1), wid = WPB == 1 ? 0 : (int)(threadIdx.x / kWave);
+  sweep_mx_body<NS, DEPTH, MODE, U, WPB, (WPB > 1)>(S, M, W, A, (int)blockIdx.x * WPB + wid, lane, wid,
+                                                    (char*)ldsd);
 }
 
 }  // namespace qw

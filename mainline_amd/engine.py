@@ -70,6 +70,7 @@ def lib():
                                         ctypes.c_int]
         L.qw_walkers_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.qw_walkers_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.qw_walkers_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.qw_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(MethodDesc), _dp, _dp]
         L.qw_walkers_random.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_double, _dp]
         L.qw_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp,
@@ -94,7 +95,7 @@ def exported_symbols():
     L = lib()
     names = []
     for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
-              "qw_method_read", "qw_method_count", "qw_method_read_at", "qw_walkers_set", "qw_walkers_get", "qw_walkers_random", "qw_run",
+              "qw_method_read", "qw_method_count", "qw_method_read_at", "qw_walkers_set", "qw_walkers_get", "qw_walkers_state", "qw_walkers_random", "qw_run",
               "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_parm_count",
               "qw_parm_deriv", "qw_server_create", "qw_server_submit",
               "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
@@ -212,6 +213,15 @@ class Engine:
         w = np.zeros(nw)
         self._check(lib().qw_walkers_get(self.h, nw, _p(pos), _p(w)))
         return pos, w
+
+    def walker_state(self, nw):
+        """logdet[nw][2], sign[nw][2] and the last local-energy parts elast[nw][4] (kinetic,
+        Coulomb, nonlocal, total) of the resident walkers, and dgl[nw][Ne][4], the determinant
+        gradient and laplacian ratios the last sweep of each spin ended with."""
+        ld, sg, el = np.zeros((nw, 2)), np.zeros((nw, 2)), np.zeros((nw, 4))
+        dgl = np.zeros((nw, self.ne, 4))
+        self._check(lib().qw_walkers_state(self.h, nw, _p(ld), _p(sg), _p(el), _p(dgl)))
+        return dict(logdet=ld, sign=sg, elast=el, dgl=dgl)
 
     def random_walkers(self, nw, seed=1, inirange=3.0):
         pos = np.zeros((nw, self.ne, 3))

@@ -79,3 +79,56 @@ def test_bf16x3_dmc_first_block_tracks_fp64():
     got = _dmc_blocks(deck, mol, "mixed_bf16", nblock=1)
     _say("dmc first block fp64", ref, "bf16x3", got)
     assert abs(got[0] - ref[0]) / abs(ref[0]) < 1e-3
+
+
+def _det_parts(orc, mol, pos):
+    """FP64 log|det| per spin and the determinant gradient / laplacian ratios per electron
+    from the oracle's MO values: dgl[e][c] = sum_j d_c phi_j(r_e) inv(A)[j][e]."""
+    K = pos.shape[0]
+    ld = np.zeros((K, 2))
+    dgl = np.zeros((K, mol.nelec, 4))
+    for s, (e0, n) in enumerate([(0, mol.nup), (mol.nup, mol.ndn)]):
+        for k in range(K):
+            mo = orc.mo(s, pos[k, e0:e0 + n])  # [e][component][orbital]
+            A = mo[:, 0, :]
+            ld[k, s] = np.linalg.slogdet(A)[1]
+            Ainv = np.linalg.inv(A)
+            for c in range(4):
+                dgl[k, e0:e0 + n, c] = np.einsum("ej,je->e", mo[:, c + 1, :], Ainv)
+    return ld, dgl
+
+
+# After one sweep of a spin channel with FP32 Sherman-Morrison updates, per FP64 recompute
+# interval: the determinant gradient and laplacian ratios the sweep ends with (max error over
+# the largest magnitude per configuration), the local energy the method computed from them
+# (relative), and log|det| after the FP64 recompute of the FP32 MO values (absolute).  Every 32
+# moves (the default) the errors stay near those of a fresh evaluation; once per sweep (64 or
+# 80 updates) the gradient error of the worst-conditioned configuration grows several-fold
+# (measured 4.6e-3 on ae32), which is why the default interval is 32.
+DRIFT_TOL = {32: dict(grad=1e-3, lap=1e-2, kin=2e-3, etot=2e-3, logdet=1e-3),
+             1 << 20: dict(grad=1e-2, lap=2e-2, kin=2e-3, etot=2e-3, logdet=1e-3)}
+
+
+@pytest.mark.parametrize("name,refresh", [("ae32", 1 << 20), ("ae32", 32), ("ae40", 32), ("ae40", 1 << 20)])
+def test_mixed_state_after_a_sweep_of_fp32_updates(name, refresh):
+    deck, mol = synth_deck(name)
+    K = 16
+    _say("drift", name, "refresh", refresh)
+    eng = E.Engine(deck, "mixed_fp32", seed=4242, refresh_interval=refresh)
+    eng.set_walkers(synth.random_configs(mol, K, seed=57))
+    eng.run(E.method("VMC", K, nblock=1, nstep=1, timestep=0.3, ndecorr=1))
+    pos, _ = eng.get_walkers(K)
+    st = eng.walker_state(K)
+    orc = O.Oracle(deck)
+    ref = orc.eval(pos)
+    ld, dgl = _det_parts(orc, mol, pos)
+    err = dict(
+        grad=max(np.max(np.abs(st["dgl"][k, :, :3] - dgl[k, :, :3])) / np.max(np.abs(dgl[k, :, :3])) for k in range(K)),
+        lap=max(np.max(np.abs(st["dgl"][k, :, 3] - dgl[k, :, 3])) / np.max(np.abs(dgl[k, :, 3])) for k in range(K)),
+        kin=np.max(np.abs(st["elast"][:, 0] - ref["eparts"][:, 0]) / np.abs(ref["eparts"][:, 0])),
+        etot=np.max(np.abs(st["elast"][:, 3] - ref["eparts"].sum(axis=1)) / np.abs(ref["eparts"].sum(axis=1))),
+        logdet=np.max(np.abs(st["logdet"] - ld)),
+    )
+    _say("drift", name, "refresh", refresh, {k: "%.2e" % v for k, v in err.items()})
+    for k, v in err.items():
+        assert v < DRIFT_TOL[refresh][k], (k, v)
