@@ -106,6 +106,25 @@ qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const double* ro
                         const double* pp_rand, double* logpsi, double* sign, double* grad,
                         double* lap, double* eparts);
 
+/* Allocation-free qw_eval_fixed.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code recorded into a HIP graph, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no device allocator (hipMalloc, hipFree or an
+ * allocating library routine) and does not synchronise: every pointer is a device pointer on
+ * the engine's device, with the layouts of qw_eval_fixed (pos[K][Ne][3], rot[9] or NULL,
+ * pp_rand[K][natoms][Ne] when pp_mode = 1; outputs logpsi[K], sign[K], grad[K][Ne][3],
+ * lap[K][Ne], eparts[K][3]), and the work is queued on `stream` (a hipStream_t; NULL = the
+ * engine's own stream).  The per-configuration state (inverses, MO caches, energies) lives in
+ * a fixed device array of QW_SCRATCH_BYTES per GPU, cleared at the start of each call;
+ * K above what fits (qw_eval_fixed_static_max) is refused with QW_ERR_ARG.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (two streams, two host threads) race.
+ * Systems with more than 128 electrons of one spin are refused (QW_ERR_UNSUPPORTED). */
+#define QW_SCRATCH_BYTES (32u << 20)
+qw_status qw_eval_fixed_static(qw_engine* e, int K, const double* pos, const double* rot, int pp_mode,
+                               const double* pp_rand, double* logpsi, double* sign, double* grad, double* lap,
+                               double* eparts, void* stream);
+/* Largest K qw_eval_fixed_static accepts for this engine (0 when it accepts none). */
+qw_status qw_eval_fixed_static_max(const qw_engine* e, int* kmax);
+
 /* Linear parameters in the reference's order: with OPTIMIZE_DET and several determinants the
  * coefficients of CSFs 2.. first (DETWT input: each determinant is a CSF), then the Jastrow's
  * (group by group: one-body coefficient sections in input order, then two-body coefficients,

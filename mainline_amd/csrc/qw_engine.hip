@@ -7,6 +7,38 @@ namespace {
 thread_local std::string g_create_error;
 }  // namespace
 
+// Per-configuration state of the allocation-free evaluation (qw_eval_fixed_static): a
+// file-scope device array, one copy on every GPU the library runs on, held until the process
+// ends.  Host code never passes the array's name to a kernel or a copy: it uses the device
+// address that hipGetSymbolAddress returns for the current device (scratch_address).
+constexpr size_t kScratchBytes = (size_t)QW_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;  // device ordinal -> address of that device's scratch
+
+// Address of `scratch` on device `dev`, which must be the current device (looked up once per
+// device).
+void* scratch_address(int dev) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* p = nullptr;
+  const hipError_t r = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (r != hipSuccess || !p)
+    throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(r));
+  g_scratch_ptr[dev] = p;
+  return p;
+}
+
+// Clears n16 16-byte words (a fill kernel rather than a memset node, so the call can be
+// recorded into a graph).
+__global__ __launch_bounds__(256) void k_fill_zero(uint4* p, size_t n16) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+}  // namespace
 
 #include "qw_mixed_host.hpp"
 
@@ -94,6 +126,7 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
     e->ns = pick_ns(std::max(e->T.nup, e->T.ndn));
     e->build_device_system();
     if (e->precision != QW_PREC_FP64) mixed_setup(*e);
+    e->scratch = scratch_address(e->device);
   } catch (std::exception& x) {
     g_create_error = x.what();
     return QW_ERR_DEVICE;
@@ -403,6 +436,81 @@ extern "C" qw_status qw_eval_fixed(qw_engine* e, int K, const double* pos, const
     HIPCHK(hipMemcpyAsync(lap, d_la, (size_t)K * ne * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipMemcpyAsync(eparts, d_ep, (size_t)K * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    return QW_OK;
+  });
+}
+
+extern "C" qw_status qw_eval_fixed_static_max(const qw_engine* e, int* kmax) {
+  if (!e || !kmax) return QW_ERR_ARG;
+  DevWalkers V;
+  const size_t per = e->carve_walkers(nullptr, 1, V);
+  *kmax = (std::max(e->T.nup, e->T.ndn) > kGridMaxSpin || per > kScratchBytes) ? 0 : (int)std::min<size_t>(kScratchBytes / per, 1 << 30);
+  // the per-array 256-byte rounding makes K configurations need at most K x (one's bytes)
+  while (*kmax > 0 && e->carve_walkers(nullptr, *kmax, V) > kScratchBytes) (*kmax)--;
+  return QW_OK;
+}
+
+extern "C" qw_status qw_eval_fixed_static(qw_engine* e, int K, const double* pos, const double* rot, int pp_mode,
+                                          const double* pp_rand, double* logpsi, double* sign, double* grad,
+                                          double* lap, double* eparts, void* stream) {
+  if (!e || K <= 0 || !pos || !logpsi || !sign || !grad || !lap || !eparts) return QW_ERR_ARG;
+  if (pp_mode == 1 && !pp_rand && e->S.has_pp) return QW_ERR_ARG;
+  if (std::max(e->T.nup, e->T.ndn) > kGridMaxSpin) {
+    e->err = "qw_eval_fixed_static: more than 128 electrons of one spin need the solver library's workspace";
+    return QW_ERR_UNSUPPORTED;
+  }
+  DevWalkers V;
+  const size_t need = e->carve_walkers(nullptr, K, V);
+  if (need > kScratchBytes) {
+    int kmax = 0;
+    (void)qw_eval_fixed_static_max(e, &kmax);
+    e->err = "qw_eval_fixed_static: " + std::to_string(K) + " configurations need " + std::to_string(need) +
+             " bytes of scratch; at most " + std::to_string(kmax) + " fit in " + std::to_string(kScratchBytes);
+    return QW_ERR_ARG;
+  }
+  return guarded(e, [&] {
+    // the engine's walker view, population, stream and timing are swapped for this call and
+    // restored on every exit path
+    struct Swap {
+      qw_engine* e;
+      DevWalkers W;
+      int nw;
+      hipStream_t st;
+      bool quiet;
+      ~Swap() {
+        e->W = W;
+        e->nw = nw;
+        e->stream = st;
+        e->quiet = quiet;
+      }
+    } sw{e, e->W, e->nw, e->stream, e->quiet};
+    char* base = (char*)e->scratch;
+    e->carve_walkers(base, K, V);
+    if (stream) e->stream = (hipStream_t)stream;
+    e->quiet = true;
+    e->W = V;
+    e->nw = K;
+    e->launch("k_fill", [&] {
+      hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<size_t>((need / 16 + 255) / 256, 4096)), dim3(256), 0,
+                         e->stream, (uint4*)base, need / 16);
+    });
+    const size_t ne = e->S.ne;
+    HIPCHK(hipMemcpyAsync(V.pos, pos, (size_t)K * ne * 3 * sizeof(double), hipMemcpyDefault, e->stream));
+    e->refresh_all();
+    LocalArgs A{};
+    A.mode = 0;
+    A.pp_mode = pp_mode;
+    A.rot = rot;
+    A.pp_rand = pp_mode == 1 ? pp_rand : nullptr;
+    A.seed = e->seed;
+    A.step = e->step;
+    A.out_logpsi = logpsi;
+    A.out_sign = sign;
+    A.out_grad = grad;
+    A.out_lap = lap;
+    A.out_eparts = eparts;
+    A.ctl = e->dctl;
+    e->run_local(A);
     return QW_OK;
   });
 }

@@ -139,6 +139,7 @@ struct qw_engine {
   double* tneg = nullptr;
   int tnegcap = 0;
   int tm_mode = 0;  // 0 off, 1 TMOVES, 2 TMOVESSC (for the current run)
+  void* scratch = nullptr;  // this device's copy of the allocation-free evaluation's array
 
   ~qw_engine() {
     (void)hipSetDevice(device);
@@ -410,6 +411,43 @@ struct qw_engine {
     }
     if (md) b += N * 2 * S.ndet * (S.nsig * S.nsig + 2) * 8;
     return b;
+  }
+
+  // Walker arrays of K configurations carved from `base` (nullptr: only count), each 256-byte
+  // aligned, in the layout ensure_walkers allocates; returns the bytes used.  The branching
+  // buffers and T-move sums are not carved (fixed-configuration evaluation uses neither).
+  size_t carve_walkers(char* base, int K, DevWalkers& V) const {
+    size_t at = 0;
+    auto take = [&](size_t bytes) -> void* {
+      void* p = base ? base + at : nullptr;
+      at += (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+      return p;
+    };
+    const size_t ne = S.ne, nsp = S.nsp, n = (size_t)K;
+    V = DevWalkers{};
+    V.nw = K;
+    V.pos = (double*)take(n * ne * 3 * 8);
+    if (precision == QW_PREC_FP64) {
+      V.inv = (double*)take(n * 2 * nsp * nsp * 8);
+      V.mo = (double*)take(n * ne * 5 * nsp * 8);
+    } else {
+      V.invf = (float*)take(n * 2 * nsp * nsp * 4);
+      V.mof = (float*)take(n * ne * 5 * nsp * 4);
+    }
+    V.logdet = (double*)take(n * 2 * 8);
+    V.sgn = (double*)take(n * 2 * 8);
+    V.weight = (double*)take(n * 8);
+    V.eold = (double*)take(n * 8);
+    V.elast = (double*)take(n * 4 * 8);
+    V.acc = (int*)take(n * 4);
+    V.bsum = (double*)take(n * kBsum * 8);
+    V.dgl = (double*)take(n * ne * 4 * 8);
+    if (md) {
+      V.invd = (double*)take(n * 2 * S.ndet * S.nsig * S.nsig * 8);
+      V.ldd = (double*)take(n * 2 * S.ndet * 8);
+      V.sgd = (double*)take(n * 2 * S.ndet * 8);
+    }
+    return at;
   }
 
   void ensure_walkers(int n) {

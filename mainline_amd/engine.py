@@ -64,6 +64,9 @@ def lib():
         L.qw_engine_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.qw_eval_fixed.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp,
                                     _dp, _dp, _dp, _dp, _dp]
+        L.qw_eval_fixed_static.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int] + \
+            [ctypes.c_void_p] * 7
+        L.qw_eval_fixed_static_max.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.qw_method_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(MethodDesc)]
         L.qw_method_count.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.qw_method_read_at.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(MethodDesc), ctypes.c_char_p,
@@ -95,6 +98,7 @@ def exported_symbols():
     L = lib()
     names = []
     for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
+              "qw_eval_fixed_static", "qw_eval_fixed_static_max",
               "qw_method_read", "qw_method_count", "qw_method_read_at", "qw_walkers_set", "qw_walkers_get", "qw_walkers_state", "qw_walkers_random", "qw_run",
               "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_parm_count",
               "qw_parm_deriv", "qw_server_create", "qw_server_submit",
@@ -178,6 +182,16 @@ class Engine:
         self._check(lib().qw_eval_fixed(self.h, K, _p(pos), _p(r), pp_mode, _p(pr), _p(out["logpsi"]),
                                         _p(out["sign"]), _p(out["grad"]), _p(out["lap"]), _p(out["eparts"])))
         return out
+
+    def eval_static(self, K, pos, rot, pp_mode, pp_rand, outs, stream=None):
+        """qw_eval_fixed_static on device buffers: pos, rot, pp_rand and the five outputs
+        (logpsi, sign, grad, lap, eparts) are device addresses (int) or None."""
+        self._check(lib().qw_eval_fixed_static(self.h, K, pos, rot, pp_mode, pp_rand, *outs, stream))
+
+    def eval_static_max(self):
+        k = ctypes.c_int()
+        self._check(lib().qw_eval_fixed_static_max(self.h, ctypes.byref(k)))
+        return k.value
 
     def parms(self):
         """Values of the linear Jastrow parameters (getParmDeriv order)."""
