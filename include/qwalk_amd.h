@@ -204,6 +204,34 @@ qw_status qw_engine_walker_bytes(const qw_engine* e, int nw, unsigned long long*
  * tile width of the inverse. */
 qw_status qw_engine_tuning(const qw_engine* e, int* out4);
 
+/* ---------------------------------------------------------------- batches of systems
+ * Several engines (systems of different sizes on one device, all fp64 or all mixed_fp32 with
+ * one recompute interval) run DMC together: each step is one sweep launch per spin channel
+ * (per FP64 recompute interval on the mixed path, each followed by one recompute launch)
+ * covering every system, one local-energy launch, and per feedback interval one branching
+ * and one copy launch, recorded once as a HIP graph and replayed.  The blocks of a launch read
+ * their system from a slot table; the LCAO coefficient loads in flight are tuned online per
+ * size bucket (systems with the same register tile).  Each engine keeps its own walkers,
+ * random streams, DMC control and block records, and its results in a batch are bitwise those
+ * of qw_run on that engine alone.  Not batched: several determinants, T-moves, and on the
+ * mixed path systems above 128 electrons of a spin (QW_ERR_UNSUPPORTED).  No reference
+ * counterpart: QWalk runs one system per process (src/program_utils/ooqmc.cpp:148-170; the
+ * per-system loop is src/method/Dmc_method.cpp:328-567). */
+typedef struct qw_batch qw_batch;
+qw_status qw_batch_create(int n, qw_engine* const* engines, qw_batch** out);
+void qw_batch_destroy(qw_batch* b);
+/* Message of the last failure (b = NULL: of qw_batch_create). */
+const char* qw_batch_error(const qw_batch* b);
+/* DMC (m->method = 1) on every engine's resident walkers (qw_walkers_set first).
+ *   eblock[n][nblock][QW_BLOCK_FIELDS] block records in engine order;
+ *   stats[4 + n] = electron moves, seconds, electron-moves/s, tuner finished (0/1), then each
+ *   engine's acceptance (tau_eff / tau). */
+qw_status qw_batch_run(qw_batch* b, const qw_method_desc* m, double* eblock, double* stats);
+/* Device time per launch kind of the last run: avg_ms4[k], count4[k] for k = sweep, inverse
+ * recompute, local energy, branching (+ copies); tuning[2i], tuning[2i+1] = register tile of
+ * size bucket i and its LCAO loads in flight (up to 5 buckets; may be NULL). */
+qw_status qw_batch_kernel_times(const qw_batch* b, double* avg_ms4, int* count4, int* tuning);
+
 /* ---------------------------------------------------------------- serving loop
  * A request loop over the engine: VMC/DMC jobs (decks of different sizes) are submitted at
  * any time, admitted in arrival order while the sum of their device bytes fits the HBM budget

@@ -553,7 +553,7 @@ __global__ __launch_bounds__(256) void k_lu_gather(DevSys S, const TIN* mo, doub
 
 // log|det| and sign from the LU factors (diagonal of U, 1-based pivots); singular matrices
 // (solver info > 0) get the reference's -1e201.
-__global__ __launch_bounds__(256) void k_lu_logdet(const double* work, const int* ipiv, const int* info, double* logdet,
+static __global__ __launch_bounds__(256) void k_lu_logdet(const double* work, const int* ipiv, const int* info, double* logdet,
                                                    double* sgn, int nw, int s, int n) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nw) return;

@@ -94,7 +94,7 @@ __device__ inline double lcao_val(const DevSys& S, int s, const double p[3], dou
 }
 
 // ---------------------------------------------------------------- k_mo_rows
-__global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) {
+static __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x;
   const long blk = blockIdx.x;
@@ -475,7 +475,7 @@ __device__ inline void sweep_randoms(const SweepArgs& A, unsigned w, int off, do
 __host__ __device__ inline size_t sweep_rand_bytes(int ns, int depth) { return (size_t)ns * depth * 4 * sizeof(double); }
 
 // Advances the device step counter (one thread).
-__global__ void k_step_advance(unsigned* stepp) {
+static __global__ void k_step_advance(unsigned* stepp) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *stepp += 1;
 }
 
@@ -1502,7 +1502,7 @@ __device__ __forceinline__ void branch_body(const DevWalkers& W, DmcCtl* ctl, un
   }
 }
 
-__global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsigned long long* keys, int* vals,
+static __global__ __launch_bounds__(1024) void k_branch(DevWalkers W, DmcCtl* ctl, unsigned long long* keys, int* vals,
                                                  int* pair_src, int* pair_dst, int nmoves, int do_branch,
                                                  unsigned long long seed, unsigned event_host,
                                                  const unsigned* eventp) {
@@ -1542,14 +1542,14 @@ __device__ __forceinline__ void copy_body(const DevSys& S, const DevWalkers& W, 
   if (t == 0) W.eold[dst] = W.eold[src];
 }
 
-__global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcCtl* ctl, const int* pair_src,
+static __global__ __launch_bounds__(256) void k_copy(DevSys S, DevWalkers W, const DmcCtl* ctl, const int* pair_src,
                                               const int* pair_dst) {
   copy_body(S, W, ctl, pair_src, pair_dst, (int)blockIdx.x, (int)threadIdx.x);
 }
 
 // Block sums over walkers: out[0..kBsum-1] = sums of the bsum columns, out[kBsum] = current
 // weight sum.
-__global__ __launch_bounds__(256) void k_reduce(DevWalkers W, double* out, int reset) {
+static __global__ __launch_bounds__(256) void k_reduce(DevWalkers W, double* out, int reset) {
   constexpr int NC = kBsum + 1;
   __shared__ double red[NC][256];
   const int t = threadIdx.x;

@@ -21,7 +21,7 @@
 namespace qw {
 
 // Gauss-Jordan on [A | I] in LDS, lanes over rows; A(r, c) = phi_{occ_d(c)}(r_r) of spin s.
-__global__ __launch_bounds__(64) void k_inv_md(DevSys S, DevWalkers W, int nw) {
+static __global__ __launch_bounds__(64) void k_inv_md(DevSys S, DevWalkers W, int nw) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, nd = S.ndet, ns = S.nsig;
   const int blk = (int)blockIdx.x, d = blk % nd, s = (blk / nd) & 1, w = blk / (2 * nd);
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(64) void k_sweep_md(DevSys S, DevWalkers W, SweepAr
 //                 (+ 2 gd(p, e, 0:3) . vj(e) on the laplacian when the Jastrow has parameters),
 // D_j = D_up,j D_dn,j, Psi_S = sum_d c_d D_d, d_c = d/dx, d/dy, d/dz, laplacian of electron e;
 // vj: gradient of the parameterised Jastrow pieces (from k_parm_deriv) or null.
-__global__ __launch_bounds__(64) void k_parm_det(DevSys S, DevWalkers W, const int* csf_d0, const double* csf_w,
+static __global__ __launch_bounds__(64) void k_parm_det(DevSys S, DevWalkers W, const int* csf_d0, const double* csf_w,
                                                  int ncsf, int nw, const double* vj, double* grad, double* gd) {
   extern __shared__ double lds[];
   const int lane = threadIdx.x, w = blockIdx.x;

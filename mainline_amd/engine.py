@@ -82,6 +82,13 @@ def lib():
         L.qw_engine_tuning.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.qw_parm_count.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _dp]
         L.qw_parm_deriv.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.qw_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p)]
+        L.qw_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.qw_batch_error.restype = ctypes.c_char_p
+        L.qw_batch_error.argtypes = [ctypes.c_void_p]
+        L.qw_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(MethodDesc), _dp, _dp]
+        L.qw_batch_kernel_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int)]
         L.qw_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(ctypes.c_void_p)]
         L.qw_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(JobDesc),
                                        ctypes.POINTER(ctypes.c_longlong)]
@@ -101,7 +108,8 @@ def exported_symbols():
               "qw_eval_fixed_static", "qw_eval_fixed_static_max",
               "qw_method_read", "qw_method_count", "qw_method_read_at", "qw_walkers_set", "qw_walkers_get", "qw_walkers_state", "qw_walkers_random", "qw_run",
               "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_parm_count",
-              "qw_parm_deriv", "qw_server_create", "qw_server_submit",
+              "qw_parm_deriv", "qw_batch_create", "qw_batch_destroy", "qw_batch_error", "qw_batch_run",
+              "qw_batch_kernel_times", "qw_server_create", "qw_server_submit",
               "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
         if hasattr(L, n):
             names.append(n)
@@ -266,6 +274,50 @@ class Engine:
         self._check(lib().qw_kernel_times(self.h, buf, 4096, _p(avg), cnt, 64, ctypes.byref(nk)))
         names = buf.value.decode().split(",") if nk.value else []
         return {n: (avg[i], cnt[i]) for i, n in enumerate(names)}
+
+
+class Batch:
+    """Several engines (one device, one precision) running DMC in shared launches
+    (qw_batch_*); each engine's results equal its own qw_run bit for bit."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        h = ctypes.c_void_p()
+        st = lib().qw_batch_create(len(self.engines), arr, ctypes.byref(h))
+        if st:
+            raise QwError(st, lib().qw_batch_error(None).decode())
+        self.h = h
+
+    def run(self, m):
+        """eblock[engine][block][QW_BLOCK_FIELDS], and the run's moves, seconds,
+        electron-moves/s, tuner state and per-engine acceptance."""
+        n = len(self.engines)
+        eb = np.zeros((n, max(m.nblock, 1), BLOCK_FIELDS))
+        stats = np.zeros(4 + n)
+        st = lib().qw_batch_run(self.h, ctypes.byref(m), _p(eb), _p(stats))
+        if st:
+            raise QwError(st, lib().qw_batch_error(self.h).decode())
+        return dict(eblock=eb[:, :m.nblock], moves=stats[0], seconds=stats[1], moves_per_s=stats[2],
+                    tuned=bool(stats[3]), acceptance=stats[4:])
+
+    def kernel_times(self):
+        avg = np.zeros(4)
+        cnt = (ctypes.c_int * 4)()
+        tun = (ctypes.c_int * 10)()
+        st = lib().qw_batch_kernel_times(self.h, _p(avg), cnt, tun)
+        if st:
+            raise QwError(st, "batch kernel times")
+        names = ["sweep", "recompute", "local", "branch"]
+        buckets = {tun[2 * i]: tun[2 * i + 1] for i in range(5) if tun[2 * i]}
+        return {names[k]: (avg[k], cnt[k]) for k in range(4)}, buckets
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qw_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
 
 
 class Server:

@@ -301,6 +301,19 @@ def random_configs(mol, k, seed=7, spread=1.2):
     return out
 
 
+def nonsingular_configs(mol, k, logpsi, seed=7, spread=1.2, tries=20):
+    """random_configs with every configuration whose Slater determinant is exactly zero (an
+    electron beyond every basis function's cutoff: ln|Psi| = -1e201) drawn again from the next
+    seed; logpsi(pos) -> ln|Psi| per configuration."""
+    out = random_configs(mol, k, seed=seed, spread=spread)
+    for t in range(1, tries + 1):
+        bad = np.where(~(logpsi(out) > -1e200))[0]
+        if bad.size == 0:
+            return out
+        out[bad] = random_configs(mol, bad.size, seed=seed + 1000 * t, spread=spread)
+    raise RuntimeError("no nonsingular configurations for %s" % mol.name)
+
+
 def write_configs(path, cfg):
     cfg = np.asarray(cfg)
     with open(path, "w") as f:

@@ -1,0 +1,104 @@
+"""Batches of systems (qw_batch_*): the eight molecules of the batching workload (8 to 192
+electrons, the 192-electron one on the block-per-walker path) run DMC in shared launches.
+Each molecule's block energies, walkers and weights in the batch equal its own engine run
+alone bit for bit (fp64 and mixed_fp32), and in FP64 the molecules of up to 64 electrons per
+spin follow the oracle's trajectory."""
+import sys
+
+import numpy as np
+import pytest
+
+from mainline_amd import engine as E
+from mainline_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+_DECKS = {}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _molecules(tmp_path_factory):
+    if not _DECKS:
+        d = tmp_path_factory.mktemp("batch")
+        for m, nw in synth.batch_molecules():
+            _DECKS[m.name] = (m.write(str(d / m.name)), m, nw)
+    return _DECKS
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_fp32"])
+def test_batch_equals_each_engine_alone(prec, tmp_path_factory):
+    mols = _molecules(tmp_path_factory)
+    m = E.method("DMC", 0, nblock=2, nstep=6, timestep=0.01)
+    solo, batched, starts = [], [], []
+    for name, (deck, mol, nw) in mols.items():
+        nw = max(16, nw // 16)  # a sixteenth of the workload's walkers keeps the test short
+        a = E.Engine(deck, prec, seed=1701)
+        pos = synth.nonsingular_configs(mol, nw, lambda p: a.eval(p)["logpsi"], seed=61)
+        starts.append((name, mol, nw, pos))
+        a.set_walkers(pos)
+        b = E.Engine(deck, prec, seed=1701)
+        b.set_walkers(pos)
+        solo.append(a)
+        batched.append(b)
+    _say("batch", prec, [(n, mol.nelec, nw) for n, mol, nw, _ in starts])
+    ref = [e.run(m)["eblock"] for e in solo]
+    B = E.Batch(batched)
+    res = B.run(m)
+    times, buckets = B.kernel_times()
+    _say("batch kernels", times, "buckets", buckets)
+    for k, (name, mol, nw, _) in enumerate(starts):
+        pa, wa = solo[k].get_walkers(nw)
+        pb, wb = batched[k].get_walkers(nw)
+        assert np.array_equal(pa, pb), (name, np.max(np.abs(pa - pb)))
+        assert np.array_equal(wa, wb), name
+        assert np.array_equal(ref[k], res["eblock"][k]), (name, ref[k][:, 0], res["eblock"][k][:, 0])
+    # one sweep launch per spin channel and step (per recompute interval of 32 moves on the
+    # mixed path: the 192-electron molecule has 96 electrons of a spin)
+    nseg = 1 if prec == "fp64" else 3
+    assert times["sweep"][1] == 2 * nseg * m.nstep * m.nblock
+    # a second run continues every engine (DMC state kept) and still equals the engines alone
+    ref2 = [e.run(m)["eblock"] for e in solo]
+    res2 = B.run(m)
+    for k in range(len(starts)):
+        assert np.array_equal(ref2[k], res2["eblock"][k]), starts[k][0]
+    B.close()
+
+
+def test_batch_fp64_follows_the_oracle(tmp_path_factory):
+    mols = _molecules(tmp_path_factory)
+    names = [n for n, (_, mol, _) in mols.items() if mol.nelec <= 64]
+    m = E.method("DMC", 0, nblock=1, nstep=3, timestep=0.02)
+    engs, starts = [], []
+    for name in names:
+        deck, mol, _ = mols[name]
+        nw = 16
+        e = E.Engine(deck, "fp64", seed=4242)
+        pos = synth.nonsingular_configs(mol, nw, lambda p: e.eval(p)["logpsi"], seed=62)
+        e.set_walkers(pos)
+        engs.append(e)
+        starts.append((name, deck, mol, nw, pos))
+    res = E.Batch(engs).run(m)
+    for k, (name, deck, mol, nw, pos) in enumerate(starts):
+        _say("batch vs oracle", name, mol.nelec)
+        gp, gw = engs[k].get_walkers(nw)
+        o = O.Oracle(deck).run(pos, "DMC", nblock=1, nstep=3, timestep=0.02, seed=4242, step0=0)
+        assert np.max(np.abs(gp - o["pos"])) < 1e-8, name
+        assert np.max(np.abs(gw - o["weights"])) < 1e-8, name
+        assert np.max(np.abs(res["eblock"][k][:, :4] - o["eblock"][:, :4])) < 1e-7 * max(1, abs(o["eblock"][0, 0]))
+
+
+def test_batch_refuses_what_it_does_not_run(tmp_path_factory):
+    mols = _molecules(tmp_path_factory)
+    deck, mol, _ = next(iter(mols.values()))
+    a = E.Engine(deck, "fp64")
+    b = E.Engine(deck, "mixed_fp32")
+    with pytest.raises(E.QwError) as ex:
+        E.Batch([a, b])
+    assert ex.value.status == -1
+    with pytest.raises(E.QwError):
+        E.Batch([a, a])
