@@ -73,18 +73,23 @@ N2_DECK = os.path.join(ROOT, "tests", "golden", "decks", "n2_pp", "n2.wfeval")
 
 
 def accuracy_leg_n2(precs, nw=4096, nblock=40, nstep=20, tau=0.02, nequil=200, refresh=32):
-    """The same paired comparison on a physical system with a stationary DMC energy: the
-    reference's N2 deck (tests/n2 basis, orbitals and BFD pseudopotential, with a Jastrow),
-    walkers from the reference's random guess, 50 VMC steps, nequil DMC steps, then nblock
-    blocks of nstep steps."""
+    """The paired comparison on a physical system with a stationary DMC energy: the reference's
+    N2 deck (tests/n2 basis, orbitals and BFD pseudopotential, with a Jastrow).  One FP64
+    equilibration (walkers from the reference's random guess, 50 VMC steps, nequil DMC steps)
+    gives a common population (positions and weights); every precision, FP64 included, then
+    runs nblock blocks of nstep steps from it with the same random streams, so the block-paired
+    differences measure what the precision changes, not the equilibration."""
     from mainline_amd import engine as E
+    eq = E.Engine(N2_DECK, "fp64", seed=98)
+    eq.set_walkers(eq.random_walkers(nw, seed=7))
+    eq.run(E.method("VMC", nw, nblock=1, nstep=50, timestep=0.3))
+    eq.run(E.method("DMC", nw, nblock=1, nstep=nequil, timestep=tau))
+    pos, wts = eq.get_walkers(nw)
+    eq.close()
     out = {}
     for p in precs:
         eng = E.Engine(N2_DECK, p, seed=99, refresh_interval=0 if p == "fp64" else refresh)
-        pos = eng.random_walkers(nw, seed=7)
-        eng.set_walkers(pos)
-        eng.run(E.method("VMC", nw, nblock=1, nstep=50, timestep=0.3))
-        eng.run(E.method("DMC", nw, nblock=1, nstep=nequil, timestep=tau))
+        eng.set_walkers(pos, wts)
         eb = eng.run(E.method("DMC", nw, nblock=nblock, nstep=nstep, timestep=tau))["eblock"][:, 0]
         out[p] = dict(mean=float(eb.mean()), err=float(eb.std(ddof=1) / np.sqrt(len(eb))), blocks=eb.tolist())
         log("[accuracy N2 %s] E = %.5f +- %.5f" % (p, out[p]["mean"], out[p]["err"]))
@@ -97,6 +102,7 @@ def accuracy_leg_n2(precs, nw=4096, nblock=40, nstep=20, tau=0.02, nequil=200, r
         v["paired_delta"] = float(d.mean())
         v["paired_err"] = float(d.std(ddof=1) / np.sqrt(len(d)))
         v["agrees_2sigma"] = bool(abs(v["paired_delta"]) <= 2 * v["paired_err"])
+        v["within_fp64_error_bar"] = bool(abs(v["paired_delta"]) <= ref["err"])
     return out
 
 
@@ -181,6 +187,44 @@ def fixed_config_errors(deck, mol, precs, K=10000, refresh=32):
     return out
 
 
+def batching_leg(precision, steps, tmp, warmup=90):
+    """The batching workload: eight molecules of 8..192 electrons (Z=4 atoms, 13 functions per
+    atom, seeded walker counts in 512..4096) as DMC (timestep 0.01) in shared launches: one
+    sweep launch per spin channel and 32-move recompute interval covering all eight, one
+    recompute, local-energy, branching and copy launch, replayed as a HIP graph, LCAO loads in
+    flight tuned per size bucket during the warm-up.  Beside it the same engines run one after
+    another with their own launches."""
+    from mainline_amd import engine as E
+    from mainline_amd import synth
+    engs, sizes = [], []
+    for mol, nw in synth.batch_molecules():
+        deck = mol.write(os.path.join(tmp, mol.name))
+        e = E.Engine(deck, precision, seed=77)
+        e.set_walkers(synth.nonsingular_configs(mol, nw, lambda p: e.eval(p)["logpsi"], seed=3))
+        engs.append(e)
+        sizes.append((mol.nelec, nw))
+    B = E.Batch(engs)
+    log("[batching %s] warm-up %d steps (tuning)" % (precision, warmup))
+    B.run(E.method("DMC", 0, nblock=1, nstep=warmup, timestep=0.01))
+    r = B.run(E.method("DMC", 0, nblock=1, nstep=steps, timestep=0.01))
+    _, buckets = B.kernel_times()
+    moves_solo, secs_solo = 0.0, 0.0
+    per = []
+    for e, (ne, nw) in zip(engs, sizes):
+        rr = e.run(E.method("DMC", 0, nblock=1, nstep=steps, timestep=0.01))
+        moves_solo += rr["moves"]
+        secs_solo += rr["seconds"]
+        per.append(dict(electrons=ne, walkers=nw, alone_moves_per_s=rr["moves_per_s"]))
+    B.close()
+    res = dict(precision=precision, molecules=per, steps=steps, moves_per_s=r["moves_per_s"],
+               ms_per_step=1e3 * r["seconds"] / steps, tuned=r["tuned"], lcao_loads_per_bucket=buckets,
+               one_after_another_moves_per_s=moves_solo / secs_solo,
+               over_one_after_another=r["moves_per_s"] / (moves_solo / secs_solo))
+    log("[batching %s] %.3e electron-moves/s in shared launches, %.3e one after another" % (
+        precision, res["moves_per_s"], res["one_after_another_moves_per_s"]))
+    return res
+
+
 def serving_leg(precision, njobs, rate, seed, tmp):
     """Poisson stream of VMC/DMC requests (mean `rate` jobs/s) over molecules of 8..512
     electrons (log-uniform, up to 1664 basis functions), 1024 walkers and 1000..20000
@@ -193,7 +237,7 @@ def serving_leg(precision, njobs, rate, seed, tmp):
     for ne in sizes:
         m = synth.z4_molecule(ne // 4, 20261300 + ne, name="srv%d" % ne)
         decks[ne] = (m.write(tmp), m)
-    srv = E.Server(max_active=8, hbm_budget=128 << 30)
+    srv = E.Server(max_active=32, hbm_budget=128 << 30)
     t_arr = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
     jobs = []
     t0 = time.perf_counter()
@@ -330,6 +374,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-blocks", type=int, default=10, help="0 disables the energy-agreement leg")
     ap.add_argument("--serving-jobs", type=int, default=100, help="0 disables the serving leg")
+    ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batching leg (0 disables it)")
     ap.add_argument("--refresh", type=int, default=32,
                     help="mixed precision: FP64 inverse recompute every this many moves")
     ap.add_argument("--error-configs", type=int, default=10000,
@@ -399,6 +444,9 @@ def main():
             "VALU (%s, non-packed)" % ("FP64" if head == "fp64" else "FP32"),
             "kernel": kname, "avg_ms": avg_ms, "flops_per_move": f_em, "moves_per_launch": moves_per_launch,
             "launch_choice": R.get("tuning", {}).get("sweep")}
+    batching = None
+    if a.batch_steps > 0:
+        batching = batching_leg("mixed_fp32", a.batch_steps, tmp)
     serving = {}
     if a.serving_jobs > 0:
         for p in dict.fromkeys(["fp64", head]):
@@ -434,6 +482,8 @@ def main():
         line["energy_agreement_n2"] = acc_n2
     if errs is not None:
         line["errors_vs_fp64"] = errs
+    if batching is not None:
+        line["batching"] = batching
     if serving:
         line["serving"] = serving
         line["p50_job_latency_s"] = serving[head]["p50_s"]

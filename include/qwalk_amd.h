@@ -227,6 +227,21 @@ const char* qw_batch_error(const qw_batch* b);
  *   stats[4 + n] = electron moves, seconds, electron-moves/s, tuner finished (0/1), then each
  *   engine's acceptance (tau_eff / tau). */
 qw_status qw_batch_run(qw_batch* b, const qw_method_desc* m, double* eblock, double* stats);
+/* Continuous batching (the serving loop's launches): qw_batch_open makes an empty batch;
+ * engines (walkers set) join it at any time with the runs they are to do (e.g. a VMC
+ * equilibration, then their method; VMC and DMC runs mix in one batch), and every
+ * qw_batch_step advances every member by one step (DMC: one step of its feedback interval;
+ * VMC: one step of ndecorr sweeps) in launches shared by all members of a method: per spin
+ * channel and recompute interval one sweep launch and one recompute launch, then one local-
+ * energy launch, and one branching launch for the DMC members whose feedback interval ends.
+ * A member's results are bitwise those of qw_run on its engine alone, whichever members it
+ * shares the launches with.  finished[*nfinished] (up to 64) = members whose runs ended in
+ * this step; qw_batch_leave then gives the last run's eblock[nblock][QW_BLOCK_FIELDS] and
+ * stats[4] (as qw_run) and removes the member (the caller then owns the engine again). */
+qw_status qw_batch_open(int device, int precision, int refresh_interval, qw_batch** out);
+qw_status qw_batch_join(qw_batch* b, qw_engine* e, int nruns, const qw_method_desc* runs, int* member);
+qw_status qw_batch_step(qw_batch* b, int* finished, int* nfinished);
+qw_status qw_batch_leave(qw_batch* b, int member, double* eblock, double* stats);
 /* Device time per launch kind of the last run: avg_ms4[k], count4[k] for k = sweep, inverse
  * recompute, local energy, branching (+ copies); tuning[2i], tuning[2i+1] = register tile of
  * size bucket i and its LCAO loads in flight (up to 5 buckets; may be NULL). */

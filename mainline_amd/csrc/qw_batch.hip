@@ -30,6 +30,11 @@ struct BatchSlot {
   int u;        // LCAO coefficient loads in flight (per size bucket, tuned)
   int inv_rows; // 33..64 electrons of a spin: row-per-lane inversion instead of 8x8 blocks
   int cap;      // basis tile of the block-per-walker sweep
+  // the slot's method: DMC (depth 1, node-crossing moves rejected, weights and branching)
+  // or VMC (depth 2 with delayed rejection, ndecorr sweeps per step)
+  int depth, ndecorr, lmode, pp_mode;
+  int branch_np;  // branching after this step: the feedback interval's steps (0 = none)
+  double tau;
 };
 
 constexpr int kBatchWpb = 4;  // walkers (wavefronts) per 256-thread block for <= 64 per spin
@@ -45,12 +50,13 @@ __device__ inline int batch_blocks(const QW_CONST BatchSlot& B) {
   return B.ns ? (B.W.nw + kBatchWpb - 1) / kBatchWpb : B.W.nw;
 }
 
-// The slot of block `blk` among the slots with moves k0.. of spin s, and the slot's first
-// block (*b0).  Uniform over the block; nslot is small.
-__device__ inline int batch_find(const QW_CONST BatchSlot* sl, int nslot, int s, int k0, int blk, int* b0) {
+// The slot of block `blk` among the slots of method depth `depth` with a sweep `sw` and moves
+// k0.. of spin s, and the slot's first block (*b0).  Uniform over the block; nslot is small.
+__device__ inline int batch_find(const QW_CONST BatchSlot* sl, int nslot, int s, int k0, int depth, int sw, int blk,
+                                 int* b0) {
   int acc = 0;
   for (int i = 0; i < nslot; i++) {
-    if (sl[i].n[s] <= k0) continue;
+    if (sl[i].n[s] <= k0 || sl[i].depth != depth || sw >= sl[i].ndecorr) continue;
     const int nb = batch_blocks(sl[i]);
     if (blk < acc + nb) {
       *b0 = acc;
@@ -61,33 +67,34 @@ __device__ inline int batch_find(const QW_CONST BatchSlot* sl, int nslot, int s,
   return -1;
 }
 
-// Moves k0 .. min(n_s, k0 + R) of spin s of every slot with more than k0 electrons of that
-// spin.  PREC: 0 FP64, 1 mixed (FP32 LCAO and inverse).
-template <int PREC>
+// Moves k0 .. min(n_s, k0 + R) of spin s, in sweep `sw` of the step, of every slot of method
+// depth DEPTH (1 DMC, 2 VMC) with more than k0 electrons of that spin.  PREC: 0 FP64, 1 mixed
+// (FP32 LCAO and inverse).
+template <int PREC, int DEPTH>
 __global__ __launch_bounds__(256, 2) void k_sweep_batch(const BatchSlot* __restrict__ slots, int nslot, int s, int k0,
-                                                        int R, double tau) {
+                                                        int R, int sw) {
   extern __shared__ double ldsb[];
   const QW_CONST BatchSlot* sl = cptr(slots);
   int b0 = 0;
-  const int i = batch_find(sl, nslot, s, k0, (int)blockIdx.x, &b0);
+  const int i = batch_find(sl, nslot, s, k0, DEPTH, sw, (int)blockIdx.x, &b0);
   if (i < 0) return;
   const QW_CONST BatchSlot& B = sl[i];
   const int n = B.n[s];
-  SweepArgs A{s, k0, min(n, k0 + R), tau, B.seed, 0u, 0u, 1, 0, B.stepp};
-  A.dets = A.k1 == n;
+  SweepArgs A{s, k0, min(n, k0 + R), B.tau, B.seed, 0u, (unsigned)sw, DEPTH == 1 ? 1 : 0, 0, B.stepp};
+  A.dets = A.k1 == n && sw == B.ndecorr - 1;
   const int t = (int)threadIdx.x, blk = (int)blockIdx.x - b0;
   char* lds = (char*)ldsb;
   if (B.ns == 0) {
     if constexpr (PREC == 0)
-      sweep_big_body<double, double, 1>(gref(B.S), gref(B.W), gref(B.Pd), A, B.cap, blk, t, lds);
+      sweep_big_body<double, double, DEPTH>(gref(B.S), gref(B.W), gref(B.Pd), A, B.cap, blk, t, lds);
     else
-      sweep_big_body<float, float, 1>(gref(B.S), gref(B.W), gref(B.Pf), A, B.cap, blk, t, lds);
+      sweep_big_body<float, float, DEPTH>(gref(B.S), gref(B.W), gref(B.Pf), A, B.cap, blk, t, lds);
     return;
   }
   const int lane = t & 63, wid = t >> 6, w = blk * kBatchWpb + wid;
 #define QW_MX(NS, U) \
-  sweep_mx_body<NS, 1, LCAO_FP32, U, kBatchWpb, false>(gref(B.S), gref(B.M), gref(B.W), A, w, lane, wid, lds)
-#define QW_FP(NS, U) sweep_fp64_body<NS, 1, U, kBatchWpb>(gref(B.S), gref(B.W), A, w, lane, wid, lds)
+  sweep_mx_body<NS, DEPTH, LCAO_FP32, U, kBatchWpb, false>(gref(B.S), gref(B.M), gref(B.W), A, w, lane, wid, lds)
+#define QW_FP(NS, U) sweep_fp64_body<NS, DEPTH, U, kBatchWpb>(gref(B.S), gref(B.W), A, w, lane, wid, lds)
 #define QW_CASES(X, U0, U1, U2)                 \
   switch (B.ns * 100 + B.u) {                   \
     case 400 + U0: X(4, U0); break;             \
@@ -128,11 +135,12 @@ union InvBatchLds {
   InvBlkLds blk[kBatchWpb];
   InvGridLds<16> grid;
 };
-__global__ __launch_bounds__(256) void k_inv_batch(const BatchSlot* __restrict__ slots, int nslot, int s, int k0) {
+__global__ __launch_bounds__(256) void k_inv_batch(const BatchSlot* __restrict__ slots, int nslot, int s, int k0,
+                                                   int depth, int sw) {
   __shared__ InvBatchLds L;
   const QW_CONST BatchSlot* sl = cptr(slots);
   int b0 = 0;
-  const int i = batch_find(sl, nslot, s, k0, (int)blockIdx.x, &b0);
+  const int i = batch_find(sl, nslot, s, k0, depth, sw, (int)blockIdx.x, &b0);
   if (i < 0) return;
   const QW_CONST BatchSlot& B = sl[i];
   const int t = (int)threadIdx.x, blk = (int)blockIdx.x - b0;
@@ -160,17 +168,17 @@ __global__ __launch_bounds__(256) void k_inv_batch(const BatchSlot* __restrict__
 
 // Local energy and DMC weight of every walker of every slot (one 64-thread block each).
 template <int PREC>
-__global__ __launch_bounds__(64) void k_local_batch(const BatchSlot* __restrict__ slots, int nslot, int pp_mode) {
+__global__ __launch_bounds__(64) void k_local_batch(const BatchSlot* __restrict__ slots, int nslot) {
   extern __shared__ double ldsl[];
   const QW_CONST BatchSlot* sl = cptr(slots);
   int i = 0, b0 = 0;
   while (i + 1 < nslot && (int)blockIdx.x >= b0 + sl[i].W.nw) b0 += sl[i++].W.nw;
   const QW_CONST BatchSlot& B = sl[i];
   LocalArgs A{};
-  A.mode = 2;
+  A.mode = B.lmode;
   A.use_dgl = 1;
   A.stepp = B.stepp;
-  A.pp_mode = pp_mode;
+  A.pp_mode = B.pp_mode;
   A.seed = B.seed;
   A.ctl = B.ctl;
   if constexpr (PREC == 0)
@@ -179,20 +187,24 @@ __global__ __launch_bounds__(64) void k_local_batch(const BatchSlot* __restrict_
     local_body<float>(gref(B.S), gref(B.W), A, (int)blockIdx.x - b0, (int)threadIdx.x, ldsl);
 }
 
-// Trial energy, effective timestep and the pair comb of each slot (one block each).
+// Trial energy, effective timestep and the pair comb of each branching slot (one block each):
+// every slot over np steps, or with np = 0 the slots whose branch_np is set (over that many).
 __global__ __launch_bounds__(1024) void k_branch_batch(const BatchSlot* __restrict__ slots, int nslot, int np) {
   __shared__ BranchLds L;
   const QW_CONST BatchSlot& B = cptr(slots)[blockIdx.x];
-  branch_body(gref(B.W), B.ctl, B.keys, B.vals, B.psrc, B.pdst, B.W.nw * B.S.ne * np, 1, B.seed, 0u, B.stepp,
+  const int m = np ? np : B.branch_np;
+  if (m <= 0) return;
+  branch_body(gref(B.W), B.ctl, B.keys, B.vals, B.psrc, B.pdst, B.W.nw * B.S.ne * m, 1, B.seed, 0u, B.stepp,
               (int)threadIdx.x, L);
 }
 
-// The branching copies of every slot (one block per pair slot of each).
-__global__ __launch_bounds__(256) void k_copy_batch(const BatchSlot* __restrict__ slots, int nslot) {
+// The branching copies of the slots that branched (one block per pair slot of each).
+__global__ __launch_bounds__(256) void k_copy_batch(const BatchSlot* __restrict__ slots, int nslot, int np) {
   const QW_CONST BatchSlot* sl = cptr(slots);
   int i = 0, b0 = 0;
   while (i + 1 < nslot && (int)blockIdx.x >= b0 + sl[i].W.nw / 2 + 1) b0 += sl[i++].W.nw / 2 + 1;
   const QW_CONST BatchSlot& B = sl[i];
+  if (!np && B.branch_np <= 0) return;
   copy_body(gref(B.S), gref(B.W), B.ctl, B.psrc, B.pdst, (int)blockIdx.x - b0, (int)threadIdx.x);
 }
 
@@ -205,24 +217,44 @@ __global__ void k_step_batch(const BatchSlot* __restrict__ slots, int nslot) {
 }  // namespace qw
 
 // ====================================================================== the batch
+// A member: one engine with the runs it still has to do (an optional VMC equilibration, then
+// its method), advanced one step per batch step in step mode, or a whole DMC run at a time in
+// qw_batch_run.
+struct BatchMember {
+  qw_engine* e = nullptr;
+  int id = 0;
+  std::vector<qw_method_desc> runs;
+  int run = 0;            // the run being done
+  bool begun = false;     // run_begin done for it
+  qw_engine::RunState R;
+  int b = 0, st = 0;      // block, step within the block
+  int ip = 0, np = 0;     // DMC: steps done in the feedback interval, its length
+  std::vector<double> eblock;  // block records of the current run
+  double stats[4] = {0, 0, 0, 0};
+  bool done = false;
+  const qw_method_desc& m() const { return runs[run]; }
+  bool dmc() const { return m().method == 1; }
+};
+
 struct qw_batch {
   std::string err;
-  std::vector<qw_engine*> eng;
   int device = 0, precision = 0, refresh = kRefreshDefault;
   hipStream_t stream = nullptr;
-  std::vector<BatchSlot> slots;
+  std::vector<BatchMember> mem;  // in join order
+  int next_id = 1;
+  std::vector<BatchSlot> slots;  // largest systems first
   BatchSlot* d_slots = nullptr;
+  int slot_cap = 0;
   hipGraphExec_t gx = nullptr;
   int g_np = 0;
-  double g_tau = 0;
-  int g_ppm = -1;
   bool graphs = true;
-  // online tuning per size bucket (slots of one register tile): LCAO loads in flight, and for
-  // 33..64 electrons of a spin the inversion layout; every choice gives the same results
-  std::vector<int> buckets;            // register tiles present, in order
-  int tune_bucket = 0, tune_cand = 0;  // position of the tuning walk
+  // online tuning per size bucket (slots of one register tile): LCAO loads in flight; every
+  // choice gives the same results
+  std::vector<int> buckets;  // register tiles seen, sorted
+  std::vector<int> bucket_u;
+  int tune_bucket = 0, tune_cand = 0;
   bool tuned = false;
-  std::vector<double> tune_best;
+  std::vector<std::pair<int, double>> tune_hist;
   std::vector<hipEvent_t> evpool;
   struct Timed {
     hipEvent_t a, b;
@@ -232,6 +264,8 @@ struct qw_batch {
   double ktime[4] = {0, 0, 0, 0};
   int kcount[4] = {0, 0, 0, 0};
   size_t lds_sweep = 0, lds_local = 0;
+  bool capturing = false;
+  bool step_mode = false;  // made by qw_batch_open: members join and leave between steps
 
   ~qw_batch() {
     (void)hipSetDevice(device);
@@ -245,7 +279,9 @@ struct qw_batch {
     }
     if (stream) (void)hipStreamDestroy(stream);
   }
-  std::vector<int> ucands() const { return precision == QW_PREC_FP64 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8, 16}; }
+  std::vector<int> ucands() const {
+    return precision == QW_PREC_FP64 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8, 16};
+  }
   hipEvent_t ev() {
     if (evpool.empty()) {
       hipEvent_t e;
@@ -256,7 +292,6 @@ struct qw_batch {
     evpool.pop_back();
     return e;
   }
-  bool capturing = false;
   template <typename F>
   void launch(int kind, F f) {
     if (capturing) {
@@ -284,17 +319,46 @@ struct qw_batch {
     timed.clear();
   }
 
-  // slot table from the engines' current walkers (walker counts change with the engines)
-  void build_slots() {
+  // Whether an engine can be a member (same device, precision and recompute interval).
+  std::string admissible(const qw_engine* e) const {
+    if (e->device != device || e->precision != precision || e->refresh != refresh)
+      return "the engines of a batch share one device, precision and recompute interval";
+    if (precision != QW_PREC_FP64 && precision != QW_PREC_MIXED_FP32) return "batches run in fp64 or mixed_fp32";
+    if (e->md) return "several determinants are not batched";
+    if (e->big && std::max(e->T.nup, e->T.ndn) > kGridMaxSpin && precision != QW_PREC_FP64)
+      return "mixed-precision batches hold systems of up to 128 electrons of a spin";
+    for (const auto& m : mem)
+      if (m.e == e) return "an engine appears twice in the batch";
+    return std::string();
+  }
+  void add_bucket(const qw_engine* e) {
+    const int key = e->big ? 0 : e->ns;
+    if (!key || std::find(buckets.begin(), buckets.end(), key) != buckets.end()) return;
+    buckets.push_back(key);
+    std::sort(buckets.begin(), buckets.end());
+    bucket_u.assign(buckets.size(), precision == QW_PREC_FP64 ? 4 : 8);
+    tuned = false;  // a new size bucket is tuned from the start
+    tune_bucket = tune_cand = 0;
+    tune_hist.clear();
+  }
+  int bucket_of(int ns) const {
+    return (int)(std::find(buckets.begin(), buckets.end(), ns) - buckets.begin());
+  }
+
+  // Slot table of the members (largest systems first: their blocks, the longest-running, are
+  // dispatched first and the small systems' blocks fill in behind them).  branch: DMC
+  // members branch after this step (step mode).
+  void build_slots(bool step_mode) {
     slots.clear();
     lds_sweep = lds_local = 0;
-    // largest systems first: their blocks, the longest-running, are dispatched first and the
-    // small systems' blocks fill in behind them (longest-processing-time-first order)
-    std::vector<size_t> order(eng.size());
+    std::vector<size_t> order(mem.size());
     for (size_t k = 0; k < order.size(); k++) order[k] = k;
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return eng[a]->S.ne > eng[b]->S.ne; });
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return mem[a].e->S.ne > mem[b].e->S.ne; });
     for (size_t k : order) {
-      qw_engine& e = *eng[k];
+      const BatchMember& mb = mem[k];
+      if (mb.done) continue;
+      qw_engine& e = *mb.e;
+      const qw_method_desc& m = mb.m();
       BatchSlot B{};
       B.S = e.S;
       B.M = e.MT;
@@ -312,12 +376,20 @@ struct qw_batch {
       B.n[1] = e.T.ndn;
       B.ns = e.big ? 0 : e.ns;
       B.cap = precision == QW_PREC_FP64 ? kCapF64 : kCapF32;
-      B.u = precision == QW_PREC_FP64 ? 4 : 8;
+      const int bi = bucket_of(B.ns);
+      B.u = (B.ns && bi < (int)buckets.size()) ? bucket_u[bi] : (precision == QW_PREC_FP64 ? 4 : 8);
       B.inv_rows = e.inv_rows ? 1 : 0;
+      B.depth = m.method == 1 ? 1 : 2;
+      B.ndecorr = m.method == 1 ? 1 : std::max(1, m.ndecorr);
+      B.lmode = m.method == 1 ? 2 : 1;
+      B.pp_mode = m.pp_stochastic ? 1 : 0;
+      B.tau = m.timestep;
+      B.branch_np = (step_mode && m.method == 1 && mb.ip + 1 == mb.np) ? mb.np : 0;
       if (B.ns) {
-        const size_t per = precision == QW_PREC_FP64
-                               ? ((sweep_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, kCapF64, 8, 1) + 15) & ~(size_t)15)
-                               : sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, LCAO_FP32, e.MT.ldchi, 1);
+        const size_t per =
+            precision == QW_PREC_FP64
+                ? ((sweep_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, kCapF64, 8, B.depth) + 15) & ~(size_t)15)
+                : sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, LCAO_FP32, e.MT.ldchi, B.depth);
         lds_sweep = std::max(lds_sweep, (size_t)kBatchWpb * per);
       } else {
         lds_sweep = std::max(lds_sweep, sweep_big_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.S.nsp, B.cap,
@@ -326,80 +398,103 @@ struct qw_batch {
       lds_local = std::max(lds_local, e.lds_local());
       slots.push_back(B);
     }
-    // the tuned choices of each bucket
-    for (auto& B : slots) {
-      const int bi = (int)(std::find(buckets.begin(), buckets.end(), B.ns) - buckets.begin());
-      if (bi < (int)buckets.size() && B.ns) B.u = bucket_u[bi];
-    }
-    if (!d_slots) {
+    if ((int)slots.size() > slot_cap) {
       std::lock_guard<std::mutex> g(g_devmem_mu);
-      HIPCHK(hipMalloc(&d_slots, std::max<size_t>(eng.size(), 1) * sizeof(BatchSlot)));
+      if (d_slots) HIPCHK(hipFree(d_slots));
+      d_slots = nullptr;
+      slot_cap = std::max<int>(64, (int)slots.size());
+      HIPCHK(hipMalloc(&d_slots, (size_t)slot_cap * sizeof(BatchSlot)));
     }
-    HIPCHK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, stream));
+    if (!slots.empty())
+      HIPCHK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, stream));
   }
-  std::vector<int> bucket_u;
 
   int nslot() const { return (int)slots.size(); }
-  int max_n(int s) const {
+  int max_n(int s, int depth, int sw) const {
     int m = 0;
-    for (auto& B : slots) m = std::max(m, B.n[s]);
+    for (auto& B : slots)
+      if (B.depth == depth && sw < B.ndecorr) m = std::max(m, B.n[s]);
     return m;
   }
-  unsigned sweep_grid(int s, int k0) const {
+  unsigned sweep_grid(int s, int k0, int depth, int sw) const {
     unsigned g = 0;
     for (auto& B : slots)
-      if (B.n[s] > k0) g += B.ns ? (unsigned)((B.W.nw + kBatchWpb - 1) / kBatchWpb) : (unsigned)B.W.nw;
+      if (B.n[s] > k0 && B.depth == depth && sw < B.ndecorr)
+        g += B.ns ? (unsigned)((B.W.nw + kBatchWpb - 1) / kBatchWpb) : (unsigned)B.W.nw;
     return g;
   }
-  void sweep(int s, int k0, int R, double tau) {
-    const unsigned g = sweep_grid(s, k0);
+  template <int DEPTH>
+  void sweep(int s, int k0, int R, int sw) {
+    const unsigned g = sweep_grid(s, k0, DEPTH, sw);
     if (!g) return;
     launch(0, [&] {
       if (precision == QW_PREC_FP64)
-        hipLaunchKernelGGL(k_sweep_batch<0>, dim3(g), dim3(256), lds_sweep, stream, d_slots, nslot(), s, k0, R, tau);
+        hipLaunchKernelGGL((k_sweep_batch<0, DEPTH>), dim3(g), dim3(256), lds_sweep, stream, d_slots, nslot(), s, k0,
+                           R, sw);
       else
-        hipLaunchKernelGGL(k_sweep_batch<1>, dim3(g), dim3(256), lds_sweep, stream, d_slots, nslot(), s, k0, R, tau);
+        hipLaunchKernelGGL((k_sweep_batch<1, DEPTH>), dim3(g), dim3(256), lds_sweep, stream, d_slots, nslot(), s, k0,
+                           R, sw);
     });
   }
-  void recompute(int s, int k0) {
-    const unsigned g = sweep_grid(s, k0);
+  void recompute(int s, int k0, int depth, int sw) {
+    const unsigned g = sweep_grid(s, k0, depth, sw);
     if (!g) return;
-    launch(1, [&] { hipLaunchKernelGGL(k_inv_batch, dim3(g), dim3(256), 0, stream, d_slots, nslot(), s, k0); });
+    launch(1, [&] {
+      hipLaunchKernelGGL(k_inv_batch, dim3(g), dim3(256), 0, stream, d_slots, nslot(), s, k0, depth, sw);
+    });
   }
-  // one DMC feedback interval of every slot: the launches of qw_engine::dmc_interval
-  void interval(int np, double tau, int pp_mode) {
-    unsigned nwt = 0, ncp = 0;
-    for (auto& B : slots) {
-      nwt += (unsigned)B.W.nw;
-      ncp += (unsigned)(B.W.nw / 2 + 1);
-    }
-    for (int p = 0; p < np; p++) {
+  // the moves of one step of every slot of method depth DEPTH: its sweeps of both spin
+  // channels, on the mixed path in recompute intervals each followed by the recompute
+  template <int DEPTH>
+  void moves() {
+    int nsw = 0;
+    for (auto& B : slots)
+      if (B.depth == DEPTH) nsw = std::max(nsw, B.ndecorr);
+    for (int sw = 0; sw < nsw; sw++)
       for (int s = 0; s < 2; s++) {
-        const int nmax = max_n(s);
+        const int nmax = max_n(s, DEPTH, sw);
+        if (nmax == 0) continue;
         if (precision == QW_PREC_FP64) {
-          sweep(s, 0, nmax, tau);
+          sweep<DEPTH>(s, 0, nmax, sw);
         } else {
           for (int k0 = 0; k0 < nmax; k0 += refresh) {
-            sweep(s, k0, refresh, tau);
-            recompute(s, k0);
+            sweep<DEPTH>(s, k0, refresh, sw);
+            recompute(s, k0, DEPTH, sw);
           }
         }
       }
-      launch(2, [&] {
-        if (precision == QW_PREC_FP64)
-          hipLaunchKernelGGL(k_local_batch<0>, dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot(), pp_mode);
-        else
-          hipLaunchKernelGGL(k_local_batch<1>, dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot(), pp_mode);
-        hipLaunchKernelGGL(k_step_batch, dim3(1), dim3(64), 0, stream, d_slots, nslot());
-      });
-    }
-    launch(3, [&] {
-      hipLaunchKernelGGL(k_branch_batch, dim3(nslot()), dim3(1024), 0, stream, d_slots, nslot(), np);
-      hipLaunchKernelGGL(k_copy_batch, dim3(ncp), dim3(256), 0, stream, d_slots, nslot());
+  }
+  void local_and_step() {
+    unsigned nwt = 0;
+    for (auto& B : slots) nwt += (unsigned)B.W.nw;
+    launch(2, [&] {
+      if (precision == QW_PREC_FP64)
+        hipLaunchKernelGGL(k_local_batch<0>, dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
+      else
+        hipLaunchKernelGGL(k_local_batch<1>, dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
+      hipLaunchKernelGGL(k_step_batch, dim3(1), dim3(64), 0, stream, d_slots, nslot());
     });
   }
-  void interval_graph(int np, double tau, int pp_mode) {
-    if (!gx || g_np != np || g_tau != tau || g_ppm != pp_mode) {
+  // branching: every slot over np steps, or (np = 0) the slots whose branch_np is set
+  void branch(int np) {
+    unsigned ncp = 0;
+    for (auto& B : slots) ncp += (unsigned)(B.W.nw / 2 + 1);
+    launch(3, [&] {
+      hipLaunchKernelGGL(k_branch_batch, dim3(nslot()), dim3(1024), 0, stream, d_slots, nslot(), np);
+      hipLaunchKernelGGL(k_copy_batch, dim3(ncp), dim3(256), 0, stream, d_slots, nslot(), np);
+    });
+  }
+  // one DMC feedback interval of every slot (qw_batch_run): the launches of
+  // qw_engine::dmc_interval
+  void interval(int np) {
+    for (int p = 0; p < np; p++) {
+      moves<1>();
+      local_and_step();
+    }
+    branch(np);
+  }
+  void interval_graph(int np) {
+    if (!gx || g_np != np) {
       if (gx) {
         HIPCHK(hipGraphExecDestroy(gx));
         gx = nullptr;
@@ -409,7 +504,7 @@ struct qw_batch {
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       capturing = true;
       try {
-        interval(np, tau, pp_mode);
+        interval(np);
       } catch (...) {
         capturing = false;
         (void)hipStreamEndCapture(stream, &g);
@@ -421,8 +516,6 @@ struct qw_batch {
       HIPCHK(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
       HIPCHK(hipGraphDestroy(g));
       g_np = np;
-      g_tau = tau;
-      g_ppm = pp_mode;
     }
     HIPCHK(hipGraphLaunch(gx, stream));
   }
@@ -431,49 +524,34 @@ struct qw_batch {
     gx = nullptr;
   }
 
-  // Online tuning, one size bucket at a time: each candidate of the bucket runs one full DMC
-  // step's first sweep launch (spin 0, moves 0..), timed with events; the fastest is kept.
-  // Runs eagerly during the first intervals of the first run.
-  int tune_pending() const { return !tuned; }
-  void tune_step_choose() {
+  // Online tuning, one size bucket at a time: a candidate is set for the bucket, one step (or
+  // interval) runs with every launch timed, and the sum of its sweep launches is the
+  // candidate's time; after every candidate of a bucket the fastest is kept.
+  void tune_choose() {
     if (tuned) return;
-    const auto c = ucands();
     if (tune_bucket >= (int)buckets.size()) {
       tuned = true;
       return;
     }
-    bucket_u[tune_bucket] = c[tune_cand];
+    bucket_u[tune_bucket] = ucands()[tune_cand];
     for (auto& B : slots)
-      if (B.ns == buckets[tune_bucket]) B.u = c[tune_cand];
-    HIPCHK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, stream));
+      if (B.ns == buckets[tune_bucket]) B.u = bucket_u[tune_bucket];
+    if (!slots.empty())
+      HIPCHK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, stream));
   }
-  // after an interval of np steps run eagerly while tuning: the sweep launches of spin 0 at
-  // moves 0.. are the first sweep launch of each step
-  void tune_step_record(int np) {
+  // nsteps: steps the timed launches cover (the candidate's time is per step)
+  void tune_record(int nsteps) {
     if (tuned) return;
-    drain_tuning(np);
-  }
-  std::vector<std::pair<int, double>> tune_hist;  // (candidate index, ms) of the bucket
-  void drain_tuning(int np) {
     HIPCHK(hipStreamSynchronize(stream));
-    // the first sweep launch of each step is every (2 * segments_0 + ...)th; find them by order
-    const int seg0 = precision == QW_PREC_FP64 ? 1 : (max_n(0) + refresh - 1) / refresh;
-    const int seg1 = precision == QW_PREC_FP64 ? 1 : (max_n(1) + refresh - 1) / refresh;
-    const int per_step = seg0 + seg1;
-    int k = 0;
-    double best = 1e300;
-    for (auto& t : timed)
-      if (t.kind == 0) {
-        if (k % per_step == 0) {
-          float ms = 0;
-          HIPCHK(hipEventElapsedTime(&ms, t.a, t.b));
-          best = std::min(best, (double)ms);
-        }
-        k++;
+    double t = 0;
+    for (auto& x : timed)
+      if (x.kind == 0) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, x.a, x.b));
+        t += ms;
       }
-    (void)np;
     drain();
-    tune_hist.push_back({tune_cand, best});
+    tune_hist.push_back({tune_cand, t / std::max(1, nsteps)});
     const auto c = ucands();
     if (++tune_cand >= (int)c.size()) {
       int bi = 0;
@@ -482,11 +560,80 @@ struct qw_batch {
       bucket_u[tune_bucket] = c[tune_hist[bi].first];
       for (auto& B : slots)
         if (B.ns == buckets[tune_bucket]) B.u = bucket_u[tune_bucket];
-      HIPCHK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, stream));
+      if (!slots.empty())
+        HIPCHK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * sizeof(BatchSlot), hipMemcpyHostToDevice, stream));
       tune_hist.clear();
       tune_cand = 0;
       if (++tune_bucket >= (int)buckets.size()) tuned = true;
     }
+  }
+
+  // ---- step mode: every member advances one step (DMC: one step of its feedback interval;
+  // VMC: one step of ndecorr sweeps), in shared launches per method
+  void step_once(std::vector<int>& finished) {
+    // engine-side work of the step's start, on each engine's stream: the run's start, the
+    // all-electron recompute at a VMC block's start, the device step counter
+    for (auto& mb : mem) {
+      if (mb.done) continue;
+      qw_engine& e = *mb.e;
+      if (!mb.begun) {
+        e.run_begin(mb.m(), mb.R);
+        if (e.tm_mode) throw Unsupported("T-moves are not batched");
+        mb.begun = true;
+        mb.b = mb.st = mb.ip = 0;
+        mb.eblock.assign((size_t)std::max(mb.m().nblock, 1) * QW_BLOCK_FIELDS, 0.0);
+      }
+      if (!mb.dmc() && mb.st == 0) e.refresh_all();
+      if (mb.dmc() && mb.ip == 0) mb.np = std::min(mb.R.fbi, mb.m().nstep - mb.st);
+      if (!mb.dmc() || mb.ip == 0) e.set_device_step();
+    }
+    for (auto& mb : mem)
+      if (!mb.done) HIPCHK(hipStreamSynchronize(mb.e->stream));
+    build_slots(true);
+    if (slots.empty()) return;
+    tune_choose();
+    bool any_dmc = false, any_vmc = false, any_branch = false;
+    for (auto& B : slots) {
+      (B.depth == 1 ? any_dmc : any_vmc) = true;
+      any_branch |= B.branch_np > 0;
+    }
+    if (any_dmc) moves<1>();
+    if (any_vmc) moves<2>();
+    local_and_step();
+    if (any_branch) branch(0);
+    tune_record(1);
+    HIPCHK(hipStreamSynchronize(stream));
+    drain();
+    // bookkeeping of the members, and their block ends on their own streams
+    for (auto& mb : mem) {
+      if (mb.done) continue;
+      qw_engine& e = *mb.e;
+      const qw_method_desc& m = mb.m();
+      mb.st++;
+      if (mb.dmc()) {
+        if (++mb.ip == mb.np) {
+          e.run_advance(mb.R, mb.np);
+          mb.ip = 0;
+        }
+      } else {
+        mb.R.moves += (long long)e.nw * e.S.ne * (std::max(1, m.ndecorr) - 1);
+        e.run_advance(mb.R, 1);
+      }
+      if (mb.st == m.nstep) {
+        e.run_block_end(m, mb.R, mb.b, mb.eblock.data());
+        mb.b++;
+        mb.st = mb.ip = 0;
+        if (mb.b == m.nblock) {
+          e.run_finish(m, mb.R, mb.stats);
+          mb.begun = false;
+          if (++mb.run == (int)mb.runs.size()) {
+            mb.done = true;
+            finished.push_back(mb.id);
+          }
+        }
+      }
+    }
+    for (auto& mb : mem) HIPCHK(hipStreamSynchronize(mb.e->stream));
   }
 };
 
@@ -511,55 +658,14 @@ qw_status bguard(qw_batch* b, F f) {
   }
 }
 thread_local std::string g_batch_error;
-}  // namespace
 
-extern "C" qw_status qw_batch_create(int n, qw_engine* const* engines, qw_batch** out) {
-  if (!out || n <= 0 || !engines) return QW_ERR_ARG;
-  *out = nullptr;
-  if (n > 64) {  // k_step_batch advances the step counters with one 64-thread block
-    g_batch_error = "at most 64 engines per batch";
-    return QW_ERR_ARG;
-  }
+qw_status batch_new(int device, int precision, int refresh, qw_batch** out) {
   std::unique_ptr<qw_batch> b(new qw_batch);
-  for (int i = 0; i < n; i++) {
-    qw_engine* e = engines[i];
-    if (!e) return QW_ERR_ARG;
-    for (int j = 0; j < i; j++)
-      if (engines[j] == e) {
-        g_batch_error = "an engine appears twice in the batch";
-        return QW_ERR_ARG;
-      }
-    if (i == 0) {
-      b->device = e->device;
-      b->precision = e->precision;
-      b->refresh = e->refresh;
-    }
-    if (e->device != b->device || e->precision != b->precision || e->refresh != b->refresh) {
-      g_batch_error = "the engines of a batch share one device, precision and recompute interval";
-      return QW_ERR_ARG;
-    }
-    if (b->precision != QW_PREC_FP64 && b->precision != QW_PREC_MIXED_FP32) {
-      g_batch_error = "batches run in fp64 or mixed_fp32";
-      return QW_ERR_UNSUPPORTED;
-    }
-    if (e->md) {
-      g_batch_error = "several determinants are not batched";
-      return QW_ERR_UNSUPPORTED;
-    }
-    if (e->big && std::max(e->T.nup, e->T.ndn) > kGridMaxSpin && b->precision != QW_PREC_FP64) {
-      g_batch_error = "mixed-precision batches hold systems of up to 128 electrons of a spin";
-      return QW_ERR_UNSUPPORTED;
-    }
-    b->eng.push_back(e);
-  }
+  b->device = device;
+  b->precision = precision;
+  b->refresh = refresh > 0 ? refresh : kRefreshDefault;
   qw_status st = bguard(b.get(), [&] {
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    for (qw_engine* e : b->eng) {
-      const int key = e->big ? 0 : e->ns;
-      if (key && std::find(b->buckets.begin(), b->buckets.end(), key) == b->buckets.end()) b->buckets.push_back(key);
-    }
-    std::sort(b->buckets.begin(), b->buckets.end());
-    b->bucket_u.assign(b->buckets.size(), b->precision == QW_PREC_FP64 ? 4 : 8);
     if (const char* t = std::getenv("QW_TUNE")) b->tuned = t[0] == '0';
     if (const char* g = std::getenv("QW_GRAPHS")) b->graphs = g[0] != '0';
     return QW_OK;
@@ -570,6 +676,120 @@ extern "C" qw_status qw_batch_create(int n, qw_engine* const* engines, qw_batch*
   }
   *out = b.release();
   return QW_OK;
+}
+}  // namespace
+
+extern "C" qw_status qw_batch_create(int n, qw_engine* const* engines, qw_batch** out) {
+  if (!out || n <= 0 || !engines) return QW_ERR_ARG;
+  *out = nullptr;
+  if (n > 64) {  // k_step_batch advances the step counters with one 64-thread block
+    g_batch_error = "at most 64 engines per batch";
+    return QW_ERR_ARG;
+  }
+  for (int i = 0; i < n; i++)
+    if (!engines[i]) return QW_ERR_ARG;
+  qw_batch* b = nullptr;
+  qw_status st = batch_new(engines[0]->device, engines[0]->precision, engines[0]->refresh, &b);
+  if (st != QW_OK) return st;
+  for (int i = 0; i < n; i++) {
+    const std::string why = b->admissible(engines[i]);
+    if (!why.empty()) {
+      g_batch_error = why;
+      const bool prec_ok = b->precision == QW_PREC_FP64 || b->precision == QW_PREC_MIXED_FP32;
+      delete b;
+      return prec_ok ? QW_ERR_ARG : QW_ERR_UNSUPPORTED;
+    }
+    BatchMember mb;
+    mb.e = engines[i];
+    mb.id = b->next_id++;
+    b->mem.push_back(mb);
+    b->add_bucket(engines[i]);
+  }
+  *out = b;
+  return QW_OK;
+}
+
+extern "C" qw_status qw_batch_open(int device, int precision, int refresh_interval, qw_batch** out) {
+  if (!out) return QW_ERR_ARG;
+  *out = nullptr;
+  if (precision != QW_PREC_FP64 && precision != QW_PREC_MIXED_FP32) {
+    g_batch_error = "batches run in fp64 or mixed_fp32";
+    return QW_ERR_UNSUPPORTED;
+  }
+  qw_status st = batch_new(device, precision, refresh_interval, out);
+  if (st == QW_OK) (*out)->step_mode = true;
+  return st;
+}
+
+extern "C" qw_status qw_batch_join(qw_batch* b, qw_engine* e, int nruns, const qw_method_desc* runs, int* member) {
+  if (!b || !e || nruns <= 0 || !runs || !member) return QW_ERR_ARG;
+  if (!b->step_mode) {
+    b->err = "engines join a batch made by qw_batch_open";
+    return QW_ERR_ARG;
+  }
+  const std::string why = b->admissible(e);
+  if (!why.empty()) {
+    b->err = why;
+    return e->device == b->device && e->precision == b->precision ? QW_ERR_UNSUPPORTED : QW_ERR_ARG;
+  }
+  if (e->nw <= 0) {
+    b->err = "an engine joins a batch with its walkers set";
+    return QW_ERR_ARG;
+  }
+  int live = 0;
+  for (auto& m : b->mem) live += !m.done;
+  if (live >= 64) {
+    b->err = "at most 64 engines per batch";
+    return QW_ERR_ARG;
+  }
+  for (int r = 0; r < nruns; r++) {
+    if (runs[r].timestep <= 0 || runs[r].nblock < 1 || runs[r].nstep < 1) {
+      b->err = "every run needs timestep > 0, nblock >= 1, nstep >= 1";
+      return QW_ERR_ARG;
+    }
+    if (runs[r].method == 1 && runs[r].tmoves && e->S.has_pp) {
+      b->err = "T-moves are not batched";
+      return QW_ERR_UNSUPPORTED;
+    }
+  }
+  BatchMember mb;
+  mb.e = e;
+  mb.id = b->next_id++;
+  mb.runs.assign(runs, runs + nruns);
+  b->mem.push_back(mb);
+  b->add_bucket(e);
+  *member = mb.id;
+  return QW_OK;
+}
+
+extern "C" qw_status qw_batch_step(qw_batch* b, int* finished, int* nfinished) {
+  if (!b || !b->step_mode) return QW_ERR_ARG;
+  return bguard(b, [&] {
+    std::vector<int> fin;
+    b->step_once(fin);
+    if (nfinished) *nfinished = (int)fin.size();
+    if (finished)
+      for (size_t i = 0; i < fin.size(); i++) finished[i] = fin[i];
+    return QW_OK;
+  });
+}
+
+extern "C" qw_status qw_batch_leave(qw_batch* b, int member, double* eblock, double* stats) {
+  if (!b) return QW_ERR_ARG;
+  for (size_t i = 0; i < b->mem.size(); i++) {
+    BatchMember& mb = b->mem[i];
+    if (mb.id != member) continue;
+    if (!mb.done) {
+      b->err = "a member leaves the batch when its runs are done";
+      return QW_ERR_ARG;
+    }
+    if (eblock) std::copy(mb.eblock.begin(), mb.eblock.end(), eblock);
+    if (stats) std::copy(mb.stats, mb.stats + 4, stats);
+    b->mem.erase(b->mem.begin() + i);
+    return QW_OK;
+  }
+  b->err = "no such member";
+  return QW_ERR_ARG;
 }
 
 extern "C" void qw_batch_destroy(qw_batch* b) { delete b; }
@@ -582,17 +802,23 @@ extern "C" qw_status qw_batch_run(qw_batch* b, const qw_method_desc* m, double* 
     b->err = "qw_batch_run runs DMC with timestep > 0, nblock >= 0, nstep >= 1";
     return QW_ERR_ARG;
   }
-  for (qw_engine* e : b->eng)
-    if (e->nw <= 0) {
+  if (b->step_mode) {
+    b->err = "qw_batch_run runs a batch made by qw_batch_create";
+    return QW_ERR_ARG;
+  }
+  for (auto& mb : b->mem)
+    if (mb.e->nw <= 0) {
       b->err = "every engine of the batch needs walkers (qw_walkers_set)";
       return QW_ERR_ARG;
     }
   return bguard(b, [&] {
-    const int ne_sys = (int)b->eng.size();
-    std::vector<qw_engine::RunState> R(ne_sys);
-    for (int i = 0; i < ne_sys; i++) {
-      qw_engine& e = *b->eng[i];
-      e.run_begin(*m, R[i]);
+    const int nsys = (int)b->mem.size();
+    for (auto& mb : b->mem) {
+      mb.runs.assign(1, *m);
+      mb.run = 0;
+      mb.done = false;
+      qw_engine& e = *mb.e;
+      e.run_begin(*m, mb.R);
       if (e.tm_mode) throw Unsupported("T-moves are not batched");
       e.set_device_step();
       HIPCHK(hipStreamSynchronize(e.stream));
@@ -600,46 +826,39 @@ extern "C" qw_status qw_batch_run(qw_batch* b, const qw_method_desc* m, double* 
     // walker buffers and counts may have changed since the last run: new slot table, and the
     // recorded interval (grid sizes and slot contents) is recorded again
     b->drop_graph();
-    b->build_slots();
+    b->build_slots(false);
     for (int k = 0; k < 4; k++) b->ktime[k] = 0, b->kcount[k] = 0;
-    const int pp_mode = m->pp_stochastic ? 1 : 0;
     const int fbi = std::max(1, m->feedback_interval);
     auto t0 = std::chrono::steady_clock::now();
-    long long moves = 0;
-    long long per_step = 0;
-    for (qw_engine* e : b->eng) per_step += (long long)e->nw * e->S.ne;
+    long long moves = 0, per_step = 0;
+    for (auto& mb : b->mem) per_step += (long long)mb.e->nw * mb.e->S.ne;
     for (int blk = 0; blk < m->nblock; blk++) {
       for (int st = 0; st < m->nstep;) {
         const int np = std::min(fbi, m->nstep - st);
         if (!b->tuned) {
-          b->tune_step_choose();
-          if (!b->tuned) {
-            b->interval(np, m->timestep, pp_mode);
-            b->tune_step_record(np);
-          } else if (b->graphs) {
-            b->interval_graph(np, m->timestep, pp_mode);
-          } else {
-            b->interval(np, m->timestep, pp_mode);
-          }
+          b->tune_choose();
+          b->interval(np);
+          b->tune_record(np);
         } else if (b->graphs) {
-          b->interval_graph(np, m->timestep, pp_mode);
+          b->interval_graph(np);
         } else {
-          b->interval(np, m->timestep, pp_mode);
+          b->interval(np);
         }
-        for (int i = 0; i < ne_sys; i++) b->eng[i]->run_advance(R[i], np);
+        for (auto& mb : b->mem) mb.e->run_advance(mb.R, np);
         moves += per_step * np;
         st += np;
       }
       HIPCHK(hipStreamSynchronize(b->stream));
-      for (int i = 0; i < ne_sys; i++)
-        b->eng[i]->run_block_end(*m, R[i], blk, eblock ? eblock + (size_t)i * m->nblock * QW_BLOCK_FIELDS : nullptr);
-      for (qw_engine* e : b->eng) HIPCHK(hipStreamSynchronize(e->stream));
+      for (int i = 0; i < nsys; i++)
+        b->mem[i].e->run_block_end(*m, b->mem[i].R, blk,
+                                   eblock ? eblock + (size_t)i * m->nblock * QW_BLOCK_FIELDS : nullptr);
+      for (auto& mb : b->mem) HIPCHK(hipStreamSynchronize(mb.e->stream));
     }
     b->drain();
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    for (int i = 0; i < ne_sys; i++) {
+    for (int i = 0; i < nsys; i++) {
       double st4[4];
-      b->eng[i]->run_finish(*m, R[i], st4);
+      b->mem[i].e->run_finish(*m, b->mem[i].R, st4);
       if (stats) stats[4 + i] = st4[0];
     }
     if (stats) {
@@ -659,7 +878,7 @@ extern "C" qw_status qw_batch_kernel_times(const qw_batch* b, double* avg_ms4, i
     if (count4) count4[k] = b->kcount[k];
   }
   if (tuning)
-    for (size_t i = 0; i < b->buckets.size(); i++) {
+    for (size_t i = 0; i < b->buckets.size() && i < 5; i++) {
       tuning[2 * i] = b->buckets[i];
       tuning[2 * i + 1] = b->bucket_u[i];
     }

@@ -892,6 +892,7 @@ struct qw_engine {
   };
   void run_begin(const qw_method_desc& m, RunState& R) {
     reset_times();
+    R = RunState{};  // a run starts afresh (block averages, move count)
     R.dmc = m.method == 1;
     R.t0 = std::chrono::steady_clock::now();
     R.A = LocalArgs{};

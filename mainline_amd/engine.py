@@ -87,6 +87,11 @@ def lib():
         L.qw_batch_error.restype = ctypes.c_char_p
         L.qw_batch_error.argtypes = [ctypes.c_void_p]
         L.qw_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(MethodDesc), _dp, _dp]
+        L.qw_batch_open.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.qw_batch_join.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(MethodDesc),
+                                    ctypes.POINTER(ctypes.c_int)]
+        L.qw_batch_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.qw_batch_leave.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.qw_batch_kernel_times.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int)]
         L.qw_server_create.argtypes = [ctypes.POINTER(ServerOpts), ctypes.POINTER(ctypes.c_void_p)]
@@ -109,6 +114,7 @@ def exported_symbols():
               "qw_method_read", "qw_method_count", "qw_method_read_at", "qw_walkers_set", "qw_walkers_get", "qw_walkers_state", "qw_walkers_random", "qw_run",
               "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_parm_count",
               "qw_parm_deriv", "qw_batch_create", "qw_batch_destroy", "qw_batch_error", "qw_batch_run",
+              "qw_batch_open", "qw_batch_join", "qw_batch_step", "qw_batch_leave",
               "qw_batch_kernel_times", "qw_server_create", "qw_server_submit",
               "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
         if hasattr(L, n):
@@ -280,14 +286,49 @@ class Batch:
     """Several engines (one device, one precision) running DMC in shared launches
     (qw_batch_*); each engine's results equal its own qw_run bit for bit."""
 
-    def __init__(self, engines):
-        self.engines = list(engines)
-        arr = (ctypes.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+    def __init__(self, engines=None, precision=None, device=0, refresh_interval=0):
+        """Batch(engines): a fixed set for run(); Batch(precision=...): an empty batch that
+        engines join() and leave between step() calls."""
         h = ctypes.c_void_p()
-        st = lib().qw_batch_create(len(self.engines), arr, ctypes.byref(h))
+        if engines is not None:
+            self.engines = list(engines)
+            arr = (ctypes.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+            st = lib().qw_batch_create(len(self.engines), arr, ctypes.byref(h))
+        else:
+            self.engines = []
+            st = lib().qw_batch_open(device, PREC[precision], refresh_interval, ctypes.byref(h))
         if st:
             raise QwError(st, lib().qw_batch_error(None).decode())
         self.h = h
+        self.nblock = {}
+
+    def join(self, engine, runs):
+        """The engine (walkers set) joins with its runs (MethodDesc list); returns its id."""
+        arr = (MethodDesc * len(runs))(*runs)
+        mid = ctypes.c_int()
+        st = lib().qw_batch_join(self.h, engine.h, len(runs), arr, ctypes.byref(mid))
+        if st:
+            raise QwError(st, lib().qw_batch_error(self.h).decode())
+        self.nblock[mid.value] = runs[-1].nblock
+        self.engines.append(engine)
+        return mid.value
+
+    def step(self):
+        """One step of every member; returns the ids of members whose runs ended."""
+        fin = (ctypes.c_int * 64)()
+        nf = ctypes.c_int()
+        st = lib().qw_batch_step(self.h, fin, ctypes.byref(nf))
+        if st:
+            raise QwError(st, lib().qw_batch_error(self.h).decode())
+        return [fin[i] for i in range(nf.value)]
+
+    def leave(self, mid):
+        eb = np.zeros((self.nblock[mid], BLOCK_FIELDS))
+        stats = np.zeros(4)
+        st = lib().qw_batch_leave(self.h, mid, _p(eb), _p(stats))
+        if st:
+            raise QwError(st, lib().qw_batch_error(self.h).decode())
+        return dict(eblock=eb, acceptance=stats[0], moves=stats[1], seconds=stats[2], moves_per_s=stats[3])
 
     def run(self, m):
         """eblock[engine][block][QW_BLOCK_FIELDS], and the run's moves, seconds,

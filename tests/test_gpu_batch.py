@@ -102,3 +102,44 @@ def test_batch_refuses_what_it_does_not_run(tmp_path_factory):
     assert ex.value.status == -1
     with pytest.raises(E.QwError):
         E.Batch([a, a])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_fp32"])
+def test_step_mode_members_join_and_leave(prec, tmp_path_factory):
+    """Continuous batching: engines join an open batch at different steps with a VMC
+    equilibration and then VMC or DMC, share each step's launches, leave when done, and each
+    one's last run equals qw_run of the same runs on an engine alone, bit for bit."""
+    mols = _molecules(tmp_path_factory)
+    picks = [n for n, (_, mol, _) in mols.items() if mol.nelec in (16, 64, 192)]
+    eq = E.method("VMC", 0, nblock=1, nstep=2, timestep=0.3, ndecorr=1)
+    plans = [[eq, E.method("DMC", 0, nblock=2, nstep=3, timestep=0.01)],
+             [eq, E.method("VMC", 0, nblock=2, nstep=2, timestep=0.3, ndecorr=2)],
+             [E.method("DMC", 0, nblock=1, nstep=7, timestep=0.01)]]
+    alone, inbatch = [], []
+    for k, name in enumerate(picks):
+        deck, mol, nw = mols[name]
+        nw = max(16, nw // 16)
+        a = E.Engine(deck, prec, seed=90 + k)
+        pos = synth.nonsingular_configs(mol, nw, lambda p: a.eval(p)["logpsi"], seed=63)
+        a.set_walkers(pos)
+        for m in plans[k]:
+            r = a.run(m)
+        alone.append(r["eblock"])
+        b = E.Engine(deck, prec, seed=90 + k)
+        b.set_walkers(pos)
+        inbatch.append(b)
+    B = E.Batch(precision=prec)
+    ids = {B.join(inbatch[0], plans[0]): 0}
+    out, steps = {}, 0
+    while len(out) < len(picks):
+        if steps == 2:
+            ids[B.join(inbatch[1], plans[1])] = 1
+        if steps == 3:
+            ids[B.join(inbatch[2], plans[2])] = 2
+        for mid in B.step():
+            out[ids[mid]] = B.leave(mid)["eblock"]
+        steps += 1
+        assert steps < 100
+    _say("step mode", prec, "steps", steps)
+    for k in range(len(picks)):
+        assert np.array_equal(out[k], alone[k]), (picks[k], out[k][:, 0], alone[k][:, 0])

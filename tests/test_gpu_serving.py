@@ -74,3 +74,41 @@ def ctypes_bytes(eng, nw):
     v = ctypes.c_ulonglong()
     assert E.lib().qw_engine_walker_bytes(eng.h, nw, ctypes.byref(v)) == 0
     return v.value
+
+
+def test_serving_workload_sizes_alone_and_in_the_mix(tmp_path):
+    """Jobs at the serving workload's sizes (8 to 512 electrons, 1024 walkers, VMC and DMC,
+    fp64 and mixed_fp32): the batched jobs share launches step by step (the 192-electron and
+    512-electron ones on the block-per-walker path; the 512-electron mixed-precision one, which
+    no batch holds, on its own engine), and every job's block records equal the same job run
+    alone, bit for bit."""
+    from mainline_amd import synth
+    decks = {}
+    for ne in (8, 48, 96, 192, 512):
+        m = synth.z4_molecule(ne // 4, 20261300 + ne, name="srv%d" % ne)
+        decks[ne] = m.write(str(tmp_path / m.name))
+    jobs = [
+        (8, E.method("VMC", 1024, nblock=2, nstep=4, timestep=0.3, ndecorr=1), "fp64", 31),
+        (48, E.method("DMC", 1024, nblock=2, nstep=3, timestep=0.01), "mixed_fp32", 32),
+        (96, E.method("VMC", 1024, nblock=1, nstep=3, timestep=0.3, ndecorr=1), "mixed_fp32", 33),
+        (192, E.method("DMC", 1024, nblock=1, nstep=2, timestep=0.01), "mixed_fp32", 34),
+        (512, E.method("DMC", 1024, nblock=1, nstep=2, timestep=0.01), "fp64", 35),
+        (512, E.method("VMC", 1024, nblock=1, nstep=1, timestep=0.3, ndecorr=1), "mixed_fp32", 36),
+        (48, E.method("DMC", 1024, nblock=2, nstep=6, timestep=0.01), "fp64", 37),
+    ]
+    alone = []
+    for ne, m, prec, seed in jobs:
+        s = E.Server(max_active=1)
+        jid = s.submit(decks[ne], m, 1024, precision=prec, seed=seed, equil_steps=2)
+        eb, st = s.wait(jid)
+        s.close()
+        alone.append(eb)
+        print("alone", ne, prec, eb[:, 0], "latency %.3f s" % st[3], flush=True)
+        assert np.all(np.isfinite(eb[:, 0]))
+    s = E.Server(max_active=16, hbm_budget=96 << 30)
+    ids = [s.submit(decks[ne], m, 1024, precision=prec, seed=seed, equil_steps=2) for ne, m, prec, seed in jobs]
+    res = [s.wait(j) for j in ids]
+    s.close()
+    for (ne, m, prec, seed), a, (eb, st) in zip(jobs, alone, res):
+        print("mix", ne, prec, eb[:, 0], "latency %.3f s" % st[3], flush=True)
+        assert np.array_equal(a, eb), (ne, prec)
