@@ -35,6 +35,9 @@ struct BatchSlot {
   int depth, ndecorr, lmode, pp_mode;
   int branch_np;  // branching after this step: the feedback interval's steps (0 = none)
   double tau;
+  // block-per-... layout: walkers (wavefronts) per 256-thread block, the LDS bytes of each,
+  // rows per lane of the one-wavefront sweep above 64 electrons of a spin (2 or 4)
+  int wpb, lds_w, nr;
 };
 
 constexpr int kBatchWpb = 4;  // walkers (wavefronts) per 256-thread block for <= 64 per spin
@@ -46,18 +49,22 @@ __device__ inline const T& gref(const QW_CONST T& r) {
   return *(const T*)&r;
 }
 
-__device__ inline int batch_blocks(const QW_CONST BatchSlot& B) {
+// Blocks of a slot in a sweep launch (wpb walkers each), and in a recompute launch (a
+// wavefront per walker up to 64 electrons of a spin, a whole block per walker above).
+__device__ inline int batch_blocks(const QW_CONST BatchSlot& B) { return (B.W.nw + B.wpb - 1) / B.wpb; }
+__device__ inline int batch_blocks_inv(const QW_CONST BatchSlot& B) {
   return B.ns ? (B.W.nw + kBatchWpb - 1) / kBatchWpb : B.W.nw;
 }
 
 // The slot of block `blk` among the slots of method depth `depth` with a sweep `sw` and moves
 // k0.. of spin s, and the slot's first block (*b0).  Uniform over the block; nslot is small.
+template <bool INV = false>
 __device__ inline int batch_find(const QW_CONST BatchSlot* sl, int nslot, int s, int k0, int depth, int sw, int blk,
                                  int* b0) {
   int acc = 0;
   for (int i = 0; i < nslot; i++) {
     if (sl[i].n[s] <= k0 || sl[i].depth != depth || sw >= sl[i].ndecorr) continue;
-    const int nb = batch_blocks(sl[i]);
+    const int nb = INV ? batch_blocks_inv(sl[i]) : batch_blocks(sl[i]);
     if (blk < acc + nb) {
       *b0 = acc;
       return i;
@@ -85,10 +92,22 @@ __global__ __launch_bounds__(256, 2) void k_sweep_batch(const BatchSlot* __restr
   const int t = (int)threadIdx.x, blk = (int)blockIdx.x - b0;
   char* lds = (char*)ldsb;
   if (B.ns == 0) {
-    if constexpr (PREC == 0)
-      sweep_big_body<double, double, DEPTH>(gref(B.S), gref(B.W), gref(B.Pd), A, B.cap, blk, t, lds);
-    else
-      sweep_big_body<float, float, DEPTH>(gref(B.S), gref(B.W), gref(B.Pf), A, B.cap, blk, t, lds);
+    // more than 64 electrons of a spin: B.wpb walkers per block, a wavefront each
+    const int lane = t & 63, wid = t >> 6;
+    if (wid >= B.wpb) return;
+    const int w = blk * B.wpb + wid;
+    char* base = lds + (size_t)wid * B.lds_w;
+    if constexpr (PREC == 0) {
+      if (B.nr == 2)
+        sweep_w_body<double, double, DEPTH, 2, kBatchWpb>(gref(B.S), gref(B.W), gref(B.Pd), A, B.cap, w, lane, base);
+      else
+        sweep_w_body<double, double, DEPTH, 4, kBatchWpb>(gref(B.S), gref(B.W), gref(B.Pd), A, B.cap, w, lane, base);
+    } else {
+      if (B.nr == 2)
+        sweep_w_body<float, float, DEPTH, 2, kBatchWpb>(gref(B.S), gref(B.W), gref(B.Pf), A, B.cap, w, lane, base);
+      else
+        sweep_w_body<float, float, DEPTH, 4, kBatchWpb>(gref(B.S), gref(B.W), gref(B.Pf), A, B.cap, w, lane, base);
+    }
     return;
   }
   const int lane = t & 63, wid = t >> 6, w = blk * kBatchWpb + wid;
@@ -131,16 +150,15 @@ union InvBatchLds {
   InvRowsLds<8> r8[kBatchWpb];
   InvRowsLds<16> r16[kBatchWpb];
   InvRowsLds<32> r32[kBatchWpb];
-  InvRowsLds<64> r64[kBatchWpb];
   InvBlkLds blk[kBatchWpb];
   InvGridLds<16> grid;
 };
-__global__ __launch_bounds__(256) void k_inv_batch(const BatchSlot* __restrict__ slots, int nslot, int s, int k0,
+__global__ __launch_bounds__(256, 2) void k_inv_batch(const BatchSlot* __restrict__ slots, int nslot, int s, int k0,
                                                    int depth, int sw) {
   __shared__ InvBatchLds L;
   const QW_CONST BatchSlot* sl = cptr(slots);
   int b0 = 0;
-  const int i = batch_find(sl, nslot, s, k0, depth, sw, (int)blockIdx.x, &b0);
+  const int i = batch_find<true>(sl, nslot, s, k0, depth, sw, (int)blockIdx.x, &b0);
   if (i < 0) return;
   const QW_CONST BatchSlot& B = sl[i];
   const int t = (int)threadIdx.x, blk = (int)blockIdx.x - b0;
@@ -158,10 +176,9 @@ __global__ __launch_bounds__(256) void k_inv_batch(const BatchSlot* __restrict__
     case 16: k_inv_rows_body<16, kBatchWpb>(gref(B.S), mo, inv, B.W.logdet, B.W.sgn, w, s, lane, L.r16[wid]); break;
     case 32: k_inv_rows_body<32, kBatchWpb>(gref(B.S), mo, inv, B.W.logdet, B.W.sgn, w, s, lane, L.r32[wid]); break;
     default:
-      if (B.inv_rows)
-        k_inv_rows_body<64, kBatchWpb>(gref(B.S), mo, inv, B.W.logdet, B.W.sgn, w, s, lane, L.r64[wid]);
-      else
-        k_inv_blk_body<kBatchWpb>(gref(B.S), mo, inv, B.W.logdet, B.W.sgn, w, s, lane, L.blk[wid]);
+      // 33..64 electrons of a spin: the 8x8-block layout (bitwise equal to the row layout an
+      // engine may be set to, so one body serves both and the kernel keeps two waves a SIMD)
+      k_inv_blk_body<kBatchWpb>(gref(B.S), mo, inv, B.W.logdet, B.W.sgn, w, s, lane, L.blk[wid]);
       break;
   }
 }
@@ -390,10 +407,16 @@ struct qw_batch {
             precision == QW_PREC_FP64
                 ? ((sweep_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, kCapF64, 8, B.depth) + 15) & ~(size_t)15)
                 : sweep_mx_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.ns, LCAO_FP32, e.MT.ldchi, B.depth);
+        B.wpb = kBatchWpb;
         lds_sweep = std::max(lds_sweep, (size_t)kBatchWpb * per);
       } else {
-        lds_sweep = std::max(lds_sweep, sweep_big_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.S.nsp, B.cap,
-                                                            precision == QW_PREC_FP64 ? 8 : 4));
+        const size_t per = sweep_w_lds_bytes(e.S.ne, e.S.na, e.S.nsh, e.S.ntab, e.S.nsp, B.cap,
+                                             precision == QW_PREC_FP64 ? 8 : 4);
+        // walkers per block while their LDS leaves room for two blocks per compute unit
+        B.wpb = (int)std::max<size_t>(1, std::min<size_t>(kBatchWpb, (80u << 10) / per));
+        B.lds_w = (int)per;
+        B.nr = std::max(e.T.nup, e.T.ndn) <= 2 * kWave ? 2 : 4;
+        lds_sweep = std::max(lds_sweep, (size_t)B.wpb * per);
       }
       lds_local = std::max(lds_local, e.lds_local());
       slots.push_back(B);
@@ -419,8 +442,7 @@ struct qw_batch {
   unsigned sweep_grid(int s, int k0, int depth, int sw) const {
     unsigned g = 0;
     for (auto& B : slots)
-      if (B.n[s] > k0 && B.depth == depth && sw < B.ndecorr)
-        g += B.ns ? (unsigned)((B.W.nw + kBatchWpb - 1) / kBatchWpb) : (unsigned)B.W.nw;
+      if (B.n[s] > k0 && B.depth == depth && sw < B.ndecorr) g += (unsigned)((B.W.nw + B.wpb - 1) / B.wpb);
     return g;
   }
   template <int DEPTH>
@@ -437,7 +459,10 @@ struct qw_batch {
     });
   }
   void recompute(int s, int k0, int depth, int sw) {
-    const unsigned g = sweep_grid(s, k0, depth, sw);
+    unsigned g = 0;
+    for (auto& B : slots)
+      if (B.n[s] > k0 && B.depth == depth && sw < B.ndecorr)
+        g += B.ns ? (unsigned)((B.W.nw + kBatchWpb - 1) / kBatchWpb) : (unsigned)B.W.nw;
     if (!g) return;
     launch(1, [&] {
       hipLaunchKernelGGL(k_inv_batch, dim3(g), dim3(256), 0, stream, d_slots, nslot(), s, k0, depth, sw);

@@ -1,25 +1,24 @@
-// Walker kernels for more than 64 electrons of one spin (up to 256): one 256-thread block
-// (four wavefronts) per walker.  The inverse stays in device memory (too large for one
-// wavefront's registers) and each Sherman-Morrison update streams it: thread i owns row i of
-// the transposed inverse, as lane i does in the register-resident kernels.  Wavefront 0
-// evaluates the basis (two-pass sparse, into the LDS tile) and the Jastrow sums; all threads
-// contract the tile with their MO's coefficients (thread j = MO j).
+// Walker kernels for more than 64 electrons of one spin (up to 256).
 //
-//   k_sweep_big<TI, R>  moves of one spin channel (src/method/Split_sample.cpp:302-540,
-//                       src/program_utils/MatrixAlgebrac.cpp:766-838); TI = inverse / MO cache
-//                       element type, R = Jastrow arithmetic
+//   k_sweep_w<TI, R, DEPTH, NR>  moves of one spin channel by one 64-lane wavefront per walker
+//                       (src/method/Split_sample.cpp:302-540,
+//                       src/program_utils/MatrixAlgebrac.cpp:766-838): lane l owns rows and MOs
+//                       l, l+64, ... (NR of them); the inverse stays in device memory (too
+//                       large for one wavefront's registers) and each Sherman-Morrison update
+//                       streams the lane's rows; TI = inverse / MO cache element type, R =
+//                       Jastrow arithmetic
 //   k_mo_rows_big<T>    basis + LCAO of every electron (src/orbitals/MO_matrix_cutoff.h:480-560)
+//   k_inv_grid<G>       FP64 pivoted Gauss-Jordan over a G x G thread grid (up to 128 per spin)
 //   k_lu_gather / k_lu_logdet / k_lu_scatter
-//                       around the batched LU and inverse of the device solver library: the
-//                       Slater matrices are gathered in column-major order so that the inverse
-//                       comes back as the transposed inverse the kernels use
+//                       around the batched LU and inverse of the device solver library (129 to
+//                       256 per spin): the Slater matrices are gathered in column-major order so
+//                       that the inverse comes back as the transposed inverse the kernels use
 //                       (src/program_utils/MatrixAlgebrac.cpp:316-421)
 #pragma once
 #include "qw_kernels.hpp"
 
 namespace qw {
 
-constexpr int kBigThreads = 256;
 constexpr int kBigMaxSpin = 256;
 // Electrons of a spin up to which the inverse recompute is the hand-written k_inv_grid<16>
 // (above: the device solver library's batched LU).
@@ -51,26 +50,6 @@ __device__ inline const JRad<double>* jei<double>(const DevSys& S) { return S.ei
 template <>
 __device__ inline const JRad<float>* jei<float>(const DevSys& S) { return S.eitf; }
 
-// Sum of NV values over the block (256 threads), the same in every thread.
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double* red, int t) {
-  const int wv = t >> 6, lane = t & 63;
-#pragma unroll
-  for (int c = 0; c < NV; c++) v[c] = wsum(v[c]);
-  if (lane == 0)
-#pragma unroll
-    for (int c = 0; c < NV; c++) red[wv * 8 + c] = v[c];
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < NV; c++) v[c] = (red[c] + red[8 + c]) + (red[16 + c] + red[24 + c]);
-  __syncthreads();
-}
-
-__host__ __device__ inline size_t sweep_big_lds_bytes(int ne, int na, int nsh, int ntab, int nsp, int cap, int tsize) {
-  return (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15) + ((sys_lds_bytes(na, nsh, ntab) + 15) & ~(size_t)15) +
-         (size_t)5 * cap * tsize + (size_t)cap * sizeof(int) + (size_t)2 * nsp * tsize + 48 * sizeof(double) + 64;
-}
-
 // Pointers of one precision for the big kernels.
 template <typename TI>
 struct BigPtrs {
@@ -80,55 +59,66 @@ struct BigPtrs {
   const void* spline;  // double or float table, matching TI
 };
 
-// The sweep of walker w by a 256-thread block (t = thread, lds0 = the block's LDS).
-template <typename TI, typename R, int DEPTH>
-__device__ __forceinline__ void sweep_big_body(const DevSys& S, const DevWalkers& W, const BigPtrs<TI>& P,
-                                               const SweepArgs& A, int cap, int w, int t, char* lds0) {
-  double* lds = (double*)lds0;
-  const int wv = t >> 6, lane = t & 63;
+// ---------------------------------------------------------------- one wavefront per walker
+// The same moves with one 64-lane wavefront per walker: lane l owns rows and MOs l, l+64, ...
+// (NR of them, nsigma <= 64 NR), so the whole move runs without block barriers and a compute
+// unit holds several walkers per SIMD.  The inverse stays in device memory (row-major, as the
+// recompute writes it); row k and the accepted point's MO values go through LDS.
+
+// LDS of one walker: positions | system tables | basis tile | inverse row k and new MO values.
+__host__ __device__ inline size_t sweep_w_lds_bytes(int ne, int na, int nsh, int ntab, int nsp, int cap, int tsize) {
+  return ((((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15) + ((sys_lds_bytes(na, nsh, ntab) + 15) & ~(size_t)15) +
+          (size_t)5 * cap * tsize + (size_t)cap * sizeof(int) + (size_t)2 * nsp * tsize + 15) &
+         ~(size_t)15;
+}
+
+// The sweep of walker w by one wavefront (base: its LDS region of sweep_w_lds_bytes); WPB > 1:
+// other wavefronts of the block run other walkers (wavefront-level barriers only).
+template <typename TI, typename R, int DEPTH, int NR, int WPB>
+__device__ __forceinline__ void sweep_w_body(const DevSys& S, const DevWalkers& W, const BigPtrs<TI>& P,
+                                             const SweepArgs& A, int cap, int w, int lane, char* base) {
   if (w >= W.nw) return;
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
-  double* pos = lds;
+  double* pos = (double*)base;
   SysLds V;
-  char* q = sys_lds_carve(S, (char*)lds + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
+  char* q = sys_lds_carve(S, base + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
   TI* chi = (TI*)q;
   int* roff = (int*)(chi + 5 * cap);
   TI* rowk = (TI*)(roff + cap);
   TI* phi0 = rowk + nsp;
-  double* red = (double*)(((size_t)(phi0 + nsp) - (size_t)lds + 15) / 16 * 16 + (char*)lds);
-  double* bc = red + 32;  // broadcast slots: nact, take, nfun, Jastrow u and gradient
-  for (int i = t; i < 3 * ne; i += kBigThreads) pos[i] = W.pos[(size_t)w * ne * 3 + i];
-  if (wv == 0) stage_sys(S, V, lane);
+  for (int i = lane; i < 3 * ne; i += kWave) pos[i] = W.pos[(size_t)w * ne * 3 + i];
+  stage_sys(S, V, lane);
   TI* ig = P.inv + ((size_t)w * 2 + s) * nsp * nsp;
   TI* mog = P.mo + (size_t)w * ne * 5 * nsp;
   const TS_SPLINE_T<TI>* spline = (const TS_SPLINE_T<TI>*)P.spline;
   double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
   int nacc = 0;
   const double tau = A.tau, stau = sqrt(tau);
-  __syncthreads();
+  walker_sync<WPB>();
   for (int k = A.k0; k < A.k1; k++) {
     const int e = off + k;
     const double r0[3] = {pos[3 * e], pos[3 * e + 1], pos[3 * e + 2]};
-    const double rk = t < n ? (double)ig[(size_t)k * nsp + t] : 0.0;
-    double gd[3] = {0, 0, 0};
-    if (t < n)
-      for (int c = 0; c < 3; c++) gd[c] = rk * (double)mog[((size_t)e * 5 + 1 + c) * nsp + t];
-    block_sum<3>(gd, red, t);
-    if (wv == 0) {
-      R g[3];
-      const R u = jastrow_e_t<R, false>(S, jee<R>(S), jei<R>(S), e, r0, pos, g, nullptr, lane, V.epos, 4);
-      if (lane == 0) {
-        bc[3] = (double)u;
-        bc[4] = (double)g[0];
-        bc[5] = (double)g[1];
-        bc[6] = (double)g[2];
-      }
+    // row k of the inverse: lane l holds its entries l + 64 r
+    double rk[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int j = lane + kWave * r;
+      const TI v = j < n ? ig[(size_t)k * nsp + j] : (TI)0;
+      rk[r] = (double)v;
+      if (j < nsp) rowk[j] = v;
     }
-    __syncthreads();
-    const double u0 = bc[3];
-    const double d0[3] = {gd[0] + bc[4], gd[1] + bc[5], gd[2] + bc[6]};
-    __syncthreads();
-    double phi[5];
+    double gd[3] = {0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int j = lane + kWave * r;
+      if (j < n)
+        for (int c = 0; c < 3; c++) gd[c] += rk[r] * (double)mog[((size_t)e * 5 + 1 + c) * nsp + j];
+    }
+    for (int c = 0; c < 3; c++) gd[c] = wsum(gd[c]);
+    R g0[3];
+    const double u0 = (double)jastrow_e_t<R, false>(S, jee<R>(S), jei<R>(S), e, r0, pos, g0, nullptr, lane, V.epos, 4);
+    const double d0[3] = {gd[0] + (double)g0[0], gd[1] + (double)g0[1], gd[2] + (double)g0[2]};
+    TI acc[NR][5];
     double r1[3], d1[3], ln1 = 0, Rt = 0;
     bool accepted = false;
     double acc_first = 0;
@@ -138,50 +128,33 @@ __device__ __forceinline__ void sweep_big_body(const DevSys& S, const DevWalkers
       gauss3(A.seed, (unsigned)w, step_of(A.step, A.stepp), (unsigned)e, (dep == 1 ? TAG_GAUSS1 : TAG_GAUSS2) | (A.sweep << 8), g);
       lim_drift(d0, tt, dr);
       for (int d = 0; d < 3; d++) rn[d] = r0[d] + (g[d] * st + dr[d]);
-      // basis (wavefront 0) and contraction (thread = MO)
-      TI acc[5] = {0, 0, 0, 0, 0};
-      if (wv == 0) {
-        const int na_ = basis_active(S, V, rn, lane);
-        if (lane == 0) bc[0] = na_;
-      }
-      __syncthreads();
-      const int nact = (int)bc[0];
+#pragma unroll
+      for (int r = 0; r < NR; r++)
+        for (int c = 0; c < 5; c++) acc[r][c] = (TI)0;
+      const int nact = basis_active(S, V, rn, lane);
+      walker_sync<WPB>();
       for (int i0 = 0; i0 < nact;) {
-        if (wv == 0) {
-          int nfun;
-          const int take = basis_batch_lap<TI, TS_SPLINE_T<TI>>(S, V, spline, rn, i0, nact, n, chi, cap, roff, cap,
-                                                                &nfun, lane);
-          if (lane == 0) {
-            bc[1] = take;
-            bc[2] = nfun;
-          }
-        }
-        __syncthreads();
-        const int take = (int)bc[1], nfun = (int)bc[2];
-        if (t < n) lcao_batch<TI, 4>(P.C[s], roff, chi, cap, nfun, acc, t);
-        __syncthreads();
+        int nfun;
+        const int take =
+            basis_batch_lap<TI, TS_SPLINE_T<TI>>(S, V, spline, rn, i0, nact, n, chi, cap, roff, cap, &nfun, lane);
+        walker_sync<WPB>();
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+          if (lane + kWave * r < n) lcao_batch<TI, 4>(P.C[s] + kWave * r, roff, chi, cap, nfun, acc[r], lane);
+        walker_sync<WPB>();
         i0 += take;
       }
-      for (int c = 0; c < 5; c++) phi[c] = (double)acc[c];
-      double tv[4] = {rk * phi[0], rk * phi[1], rk * phi[2], rk * phi[3]};
-      block_sum<4>(tv, red, t);
+      double tv[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < NR; r++)
+        for (int c = 0; c < 4; c++) tv[c] += rk[r] * (double)acc[r][c];
+      for (int c = 0; c < 4; c++) tv[c] = wsum(tv[c]);
       const double t0 = tv[0];
-      if (wv == 0) {
-        R g2[3];
-        const R u = jastrow_e_t<R, false>(S, jee<R>(S), jei<R>(S), e, rn, pos, g2, nullptr, lane, V.epos, 4);
-        if (lane == 0) {
-          bc[3] = (double)u;
-          bc[4] = (double)g2[0];
-          bc[5] = (double)g2[1];
-          bc[6] = (double)g2[2];
-        }
-      }
-      __syncthreads();
-      const double un = bc[3];
-      dn[0] = tv[1] / t0 + bc[4];
-      dn[1] = tv[2] / t0 + bc[5];
-      dn[2] = tv[3] / t0 + bc[6];
-      __syncthreads();
+      R g2[3];
+      const double un = (double)jastrow_e_t<R, false>(S, jee<R>(S), jei<R>(S), e, rn, pos, g2, nullptr, lane, V.epos, 4);
+      dn[0] = tv[1] / t0 + (double)g2[0];
+      dn[1] = tv[2] / t0 + (double)g2[1];
+      dn[2] = tv[3] / t0 + (double)g2[2];
       const double lnn = log(fabs(t0)) + (un - u0);
       const bool neg = t0 < 0;
       double acc_p;
@@ -212,15 +185,19 @@ __device__ __forceinline__ void sweep_big_body(const DevSys& S, const DevWalkers
       }
     }
     if (accepted) {
-      // Sherman-Morrison on the transposed inverse in device memory: thread i owns row i
-      if (t < nsp) {
-        phi0[t] = t < n ? (TI)phi[0] : (TI)0;
-        rowk[t] = t < n ? (TI)rk : (TI)0;
+      // Sherman-Morrison on the transposed inverse in device memory: lane l owns rows l + 64 r
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const int j = lane + kWave * r;
+        if (j < nsp) phi0[j] = j < n ? acc[r][0] : (TI)0;
       }
-      __syncthreads();
-      if (t < n) {
-        TI* row = ig + (size_t)t * nsp;
-        if (t == k) {
+      walker_sync<WPB>();
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const int i = lane + kWave * r;
+        if (i >= n) continue;
+        TI* row = ig + (size_t)i * nsp;
+        if (i == k) {
           const double f = 1.0 / Rt;
           for (int j = 0; j < n; j++) row[j] = (TI)((double)rowk[j] * f);
         } else {
@@ -229,41 +206,46 @@ __device__ __forceinline__ void sweep_big_body(const DevSys& S, const DevWalkers
           const double pk = tk * (-1.0 / Rt);
           for (int j = 0; j < n; j++) row[j] = (TI)((double)row[j] + (double)rowk[j] * pk);
         }
-        for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + t] = (TI)phi[c];
+        for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + i] = acc[r][c];
       }
       logdet += log(fabs(Rt));
       if (Rt < 0) sgn = -sgn;
       nacc++;
-      if (t < 3) pos[3 * e + t] = r1[t];
+      if (lane < 3) pos[3 * e + lane] = r1[lane];
     }
-    // the device-memory inverse rows written above are read by other threads next move
+    // the device-memory rows written above are read by other lanes (row k) on the next move
     __threadfence_block();
-    __syncthreads();
+    walker_sync<WPB>();
   }
-  for (int i = t; i < 3 * ne; i += kBigThreads) W.pos[(size_t)w * ne * 3 + i] = pos[i];
-  if (t == 0) {
+  for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
+  if (lane == 0) {
     W.logdet[2 * w + s] = logdet;
     W.sgn[2 * w + s] = sgn;
     W.acc[w] += nacc;
   }
-  if (A.dets && t < n) {
-    // determinant parts of the local energy for this thread's electron
-    const TI* row = ig + (size_t)t * nsp;
-    const TI* mr = mog + (size_t)(off + t) * 5 * nsp;
-    double dv[4] = {0, 0, 0, 0};
-    for (int j = 0; j < n; j++) {
-      const double iv = (double)row[j];
-      for (int c = 0; c < 4; c++) dv[c] += (double)mr[(size_t)(c + 1) * nsp + j] * iv;
+  if (A.dets) {
+    // determinant parts of the local energy for the lane's electrons
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int t = lane + kWave * r;
+      if (t >= n) continue;
+      const TI* row = ig + (size_t)t * nsp;
+      const TI* mr = mog + (size_t)(off + t) * 5 * nsp;
+      double dv[4] = {0, 0, 0, 0};
+      for (int j = 0; j < n; j++) {
+        const double iv = (double)row[j];
+        for (int c = 0; c < 4; c++) dv[c] += (double)mr[(size_t)(c + 1) * nsp + j] * iv;
+      }
+      double* o = W.dgl + ((size_t)w * ne + off + t) * 4;
+      for (int c = 0; c < 4; c++) o[c] = dv[c];
     }
-    double* o = W.dgl + ((size_t)w * ne + off + t) * 4;
-    for (int c = 0; c < 4; c++) o[c] = dv[c];
   }
 }
 
-template <typename TI, typename R, int DEPTH>
-__global__ __launch_bounds__(256) void k_sweep_big(DevSys S, DevWalkers W, BigPtrs<TI> P, SweepArgs A, int cap) {
+template <typename TI, typename R, int DEPTH, int NR>
+__global__ __launch_bounds__(64) void k_sweep_w(DevSys S, DevWalkers W, BigPtrs<TI> P, SweepArgs A, int cap) {
   extern __shared__ double lds[];
-  sweep_big_body<TI, R, DEPTH>(S, W, P, A, cap, (int)blockIdx.x, (int)threadIdx.x, (char*)lds);
+  sweep_w_body<TI, R, DEPTH, NR, 1>(S, W, P, A, cap, (int)blockIdx.x, (int)threadIdx.x, (char*)lds);
 }
 
 // Basis + LCAO of electron e of walker w (one 256-thread block per electron): wavefront 0

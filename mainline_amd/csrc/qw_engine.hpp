@@ -786,12 +786,17 @@ struct qw_engine {
   }
   template <typename TI, typename R>
   void big_sweep_t(const SweepArgs& A, int depth, const BigPtrs<TI>& P, int cap) {
-    const size_t lds = sweep_big_lds_bytes(S.ne, S.na, S.nsh, S.ntab, S.nsp, cap, sizeof(TI));
+    // one wavefront per walker, lanes over two (<= 128 per spin) or four rows
+    const size_t lds = sweep_w_lds_bytes(S.ne, S.na, S.nsh, S.ntab, S.nsp, cap, sizeof(TI));
+    const bool nr2 = std::max(T.nup, T.ndn) <= 2 * kWave;
     launch("k_sweep", [&] {
-      if (depth == 1)
-        hipLaunchKernelGGL((k_sweep_big<TI, R, 1>), dim3(nw), dim3(kBigThreads), lds, stream, S, W, P, A, cap);
-      else
-        hipLaunchKernelGGL((k_sweep_big<TI, R, 2>), dim3(nw), dim3(kBigThreads), lds, stream, S, W, P, A, cap);
+      if (depth == 1) {
+        if (nr2) hipLaunchKernelGGL((k_sweep_w<TI, R, 1, 2>), dim3(nw), dim3(kWave), lds, stream, S, W, P, A, cap);
+        else hipLaunchKernelGGL((k_sweep_w<TI, R, 1, 4>), dim3(nw), dim3(kWave), lds, stream, S, W, P, A, cap);
+      } else {
+        if (nr2) hipLaunchKernelGGL((k_sweep_w<TI, R, 2, 2>), dim3(nw), dim3(kWave), lds, stream, S, W, P, A, cap);
+        else hipLaunchKernelGGL((k_sweep_w<TI, R, 2, 4>), dim3(nw), dim3(kWave), lds, stream, S, W, P, A, cap);
+      }
     });
   }
   void big_sweep(const SweepArgs& A0, int depth) {

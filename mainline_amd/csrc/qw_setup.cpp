@@ -552,16 +552,49 @@ SysTables read_deck(const std::string& path, bool* unsup) {
     if (!section0(wf1, st, "STATES")) bad("need STATES");
     if ((int)st.size() != T.ndet * T.ne()) bad("STATES must list one orbital per electron for every determinant");
     if (!section0(wf1, orb, "ORBITALS")) unsupported("only ORBITALS (real MOs) are supported");
-    if (!keyword(orb, "CUTOFF_MO")) unsupported("only CUTOFF_MO orbitals are supported");
+    // CUTOFF_MO, or BLAS_MO: the same linear combination, which the reference evaluates densely
+    // and without the functions whose value is below 1e-8 at the point
+    // (src/orbitals/MO_matrix_blas.h:336-491); here both are evaluated exactly
+    if (!keyword(orb, "CUTOFF_MO") && !keyword(orb, "BLAS_MO"))
+      unsupported("only CUTOFF_MO and BLAS_MO orbitals are supported");
+    // basis centers: the atoms (USEATOMS, or USEGLOBAL of a molecule), or the centers of a file
+    // (READ: a count, then a label and x y z per center; src/orbitals/Center_set.cpp:51-90,
+    // 227-244).  File centers carry the basis functions of their label's BASIS sections and
+    // nothing else: they join the atom tables as centers of charge zero, with no
+    // pseudopotential and no electron-ion Jastrow, after the real atoms.
+    const int nreal = T.na;
+    std::vector<int> cen_atom;         // atom-table index of each basis center
+    std::vector<std::string> cen_label;
     {
-      // basis centers: the atoms (USEATOMS, or USEGLOBAL of a molecule); a centers file
-      // (READ) places basis functions away from the atoms (src/orbitals/Center_set.cpp:51-90)
       Words cw;
-      if (section0(orb, cw, "CENTERS")) {
-        if (keyword(cw, "READ")) unsupported("CENTERS { READ file } (basis centers other than the atoms) is not supported");
-        if (!keyword(cw, "USEATOMS") && !keyword(cw, "USEGLOBAL")) bad("CENTERS needs USEATOMS, USEGLOBAL or READ");
+      std::string cfile;
+      if (section0(orb, cw, "CENTERS") && value(cw, cfile, "READ")) {
+        std::ifstream cin(resolve(dir, cfile));
+        if (!cin) bad("cannot open centers file " + cfile);
+        int nc = 0;
+        if (!(cin >> nc) || nc <= 0) bad("centers file " + cfile + " needs a positive center count");
+        for (int c = 0; c < nc; c++) {
+          std::string lab;
+          double x[3];
+          if (!(cin >> lab >> x[0] >> x[1] >> x[2])) bad("centers file " + cfile + " ends early");
+          for (int d = 0; d < 3; d++) T.atom_pos.push_back(x[d]);
+          T.atom_Z.push_back(0.0);
+          T.atom_pp.push_back(-1);
+          labels.push_back("");  // matches no PSEUDO or JASTROW label
+          cen_atom.push_back(T.na + c);
+          cen_label.push_back(lab);
+        }
+        T.na += nc;
+      } else {
+        if (!cw.empty() && !keyword(cw, "USEATOMS") && !keyword(cw, "USEGLOBAL"))
+          bad("CENTERS needs USEATOMS, USEGLOBAL or READ");
+        for (int a = 0; a < T.na; a++) {
+          cen_atom.push_back(a);
+          cen_label.push_back(labels[a]);
+        }
       }
     }
+    const int ncen = (int)cen_atom.size();
     int nmo_tot = 0;
     if (!ivalue(orb, nmo_tot, "NMO")) bad("need NMO");
     double magnify = 1;
@@ -588,12 +621,13 @@ SysTables read_deck(const std::string& path, bool* unsup) {
         T.tab_spacing.push_back(f.spacing);
         T.tab_invspacing.push_back(1.0 / f.spacing);
       }
-    std::vector<int> maxf_center(T.na, 0);
-    T.atom_shell0.push_back(0);
-    for (int a = 0; a < T.na; a++) {
+    std::vector<int> maxf_center(ncen, 0), center_f0(ncen, 0);
+    for (int c = 0; c < ncen; c++) {
+      const int a = cen_atom[c];
       int fa = 0;
+      center_f0[c] = T.nb;
       for (size_t oi = 0; oi < objs.size(); oi++) {
-        if (objs[oi].label != labels[a]) continue;
+        if (objs[oi].label != cen_label[c]) continue;
         for (size_t s = 0; s < objs[oi].spl.size(); s++) {
           ShellDesc d{};
           d.table = obj_tab0[oi] + (int)s;
@@ -608,8 +642,7 @@ SysTables read_deck(const std::string& path, bool* unsup) {
           fa += d.nf;
         }
       }
-      maxf_center[a] = fa;
-      T.atom_shell0.push_back((int)T.shells.size());
+      maxf_center[c] = fa;
     }
     int maxbasis = *std::max_element(maxf_center.begin(), maxf_center.end());
     // orbital file: "mo basis center label" lines, COEFFICIENTS, values
@@ -630,7 +663,7 @@ SysTables read_deck(const std::string& path, bool* unsup) {
       if (m > nmo_tot) continue;
       Ent e{m, std::atoi(b1.c_str()) - 1, std::atoi(c1.c_str()) - 1, std::atoi(l1.c_str()) - 1};
       if (e.f < 0 || e.f >= maxbasis) bad("orbital file basis index out of range");
-      if (e.c < 0 || e.c >= T.na) bad("orbital file center index out of range");
+      if (e.c < 0 || e.c >= ncen) bad("orbital file center index out of range");
       ents.push_back(e);
       maxlabel = std::max(maxlabel, e.l + 1);
       nmo_read = std::max(nmo_read, m + 1);
@@ -638,9 +671,6 @@ SysTables read_deck(const std::string& path, bool* unsup) {
     std::vector<double> coef(maxlabel);
     for (auto& v : coef)
       if (!(oin >> v)) bad("orbital file ends before all coefficients");
-    // per-center function offsets
-    std::vector<int> center_f0(T.na);
-    for (int a = 0; a < T.na; a++) center_f0[a] = T.shells[T.atom_shell0[a]].f0;
     // the last line naming a (mo, center, function) wins; tiny coefficients are dropped after
     std::vector<std::vector<double>> moc(nmo_tot, std::vector<double>(T.nb, 0.0));
     for (auto& e : ents) {
@@ -724,7 +754,7 @@ SysTables read_deck(const std::string& path, bool* unsup) {
               }
             }
           }
-          for (int a = 0; a < T.na; a++) {
+          for (int a = 0; a < nreal; a++) {
             auto it = cf.find(labels[a]);
             if (it == cf.end()) bad("ONEBODY has no coefficients for atom " + labels[a]);
             int k = 0;

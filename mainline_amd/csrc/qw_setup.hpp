@@ -72,7 +72,6 @@ struct SysTables {
   std::vector<double> atom_pos;   // [na][3]
   std::vector<double> atom_Z;     // [na]
   std::vector<ShellDesc> shells;  // grouped by atom
-  std::vector<int> atom_shell0;   // [na+1]
   int ntab = 0, nint = 0;         // spline tables, intervals per table (padded to the max)
   std::vector<double> spline;     // [ntab][nint][4]
   std::vector<double> tab_invspacing, tab_spacing;

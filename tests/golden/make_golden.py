@@ -18,6 +18,9 @@ Cases:
   h2_jast   the reference's tests/h2 deck with its two-group Jastrow (cusp + POLYPADE)
   n2_pp     the reference's tests/n2 basis, orbitals and BFD pseudopotential (AIP 6), one
             determinant, with a POLYPADE Jastrow of nonzero coefficients
+  n2_cen    n2_pp with its basis on two centers read from a file (CENTERS { READ }),
+            displaced from the nuclei
+  n2_blas   n2_pp with BLAS_MO orbitals (the reference drops basis values below 1e-8)
   n2_ci     the reference's tests/n2 CI wave function (28 determinants, qw.ci + cidet);
   n2_cij    the same with the POLYPADE Jastrow of n2_pp
   n2_csf    the n2_ci expansion written as CSF sections
@@ -31,7 +34,7 @@ Output files: the reference program (oracle/_ref/qwalk) runs a short DMC of the 
 and its block log and per-block JSON are kept as ref_output/h_dmc.qwlog and h_dmc.json (the
 layout of the files the qwalk_amd driver writes is checked against them).
 
-Usage: python tests/golden/make_golden.py   (from the repository root)
+Usage: python tests/golden/make_golden.py [case ...]   (from the repository root)
 """
 import hashlib
 import os
@@ -163,6 +166,22 @@ def ref_deck_cases(tmp):
         f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW WF1 { INCLUDE n2.slater } "
                 "WF2 { INCLUDE n2.jast2 } }\n")
     cases["n2_pp"] = (d, "n2.wfeval", clouds([[0, 0, -1.60626709], [0, 0, 1.60626709]], [5, 5], 12, 13, 0.9))
+    # the same wave function with its basis on centers read from a file, displaced from the
+    # nuclei (CENTERS { READ }), and with BLAS_MO orbitals
+    with open(os.path.join(d, "n2.centers"), "w") as f:
+        f.write("2\nN 0.05 -0.03 -1.50626709\nN -0.04 0.02 1.70626709\n")
+    with open(os.path.join(d, "n2cen.slater"), "w") as f:
+        f.write(N2_SLATER.replace("CENTERS { USEGLOBAL }", "CENTERS { READ n2.centers }"))
+    with open(os.path.join(d, "n2cen.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW WF1 { INCLUDE n2cen.slater } "
+                "WF2 { INCLUDE n2.jast2 } }\n")
+    cases["n2_cen"] = (d, "n2cen.wfeval", clouds([[0, 0, -1.60626709], [0, 0, 1.60626709]], [5, 5], 12, 19, 0.9))
+    with open(os.path.join(d, "n2blas.slater"), "w") as f:
+        f.write(N2_SLATER.replace("CUTOFF_MO", "BLAS_MO"))
+    with open(os.path.join(d, "n2blas.wfeval"), "w") as f:
+        f.write("INCLUDE qw.sys\nTRIALFUNC { SLATER-JASTROW WF1 { INCLUDE n2blas.slater } "
+                "WF2 { INCLUDE n2.jast2 } }\n")
+    cases["n2_blas"] = (d, "n2blas.wfeval", clouds([[0, 0, -1.60626709], [0, 0, 1.60626709]], [5, 5], 12, 23, 0.9))
     # N2 CI (tests/n2/qw.ci): 28 determinants over 8 MOs, BFD pseudopotential; without and
     # with the Jastrow above
     d = os.path.join(DECKS, "n2_ci")
@@ -218,16 +237,23 @@ def ref_output_files(tmp):
 
 
 def main():
+    """python make_golden.py [case ...]: all fixtures by default, else only the named cases."""
     if not os.path.exists(REF_GOLDEN):
         sys.exit("build the reference first: make -f oracle/Makefile.ref -j8")
+    only = set(sys.argv[1:])
     with tempfile.TemporaryDirectory() as tmp:
-        ref_output_files(tmp)
+        if not only:
+            ref_output_files(tmp)
         cases = ref_deck_cases(tmp)
         for name, (d, deck, pos) in cases.items():
+            if only and name not in only:
+                continue
             res = run_ref(d, deck, pos)
             np.savez_compressed(os.path.join(OUT, name + ".npz"), deck=os.path.relpath(os.path.join(d, deck), OUT), **res)
             print(name, res["pos"].shape)
         for mol, k in [(synth.h4(), 16), (synth.pp16(), 8), (synth.ae32(), 4), (synth.fg_molecule(), 12)]:
+            if only and mol.name not in only:
+                continue
             d = os.path.join(tmp, mol.name)
             deck = os.path.basename(mol.write(d))
             orb = open(os.path.join(d, mol.name + ".orb"), "rb").read()

@@ -33,8 +33,8 @@ def test_missing_deck_is_an_input_error():
 
 def test_determinant_input_checks(tmp_path):
     """STATES must hold ndet x Ne orbitals, a CSF a coefficient and weights, and CSF excludes
-    DETWT (bad input); basis centers from a file, and several determinants with a mixed
-    precision, are valid QWalk the engine does not run (unsupported)."""
+    DETWT, and a centers file must exist (bad input); several determinants with a mixed
+    precision are valid QWalk the engine does not run (unsupported)."""
     g, deck = golden("h_atom")
     d = os.path.dirname(deck)
     base = open(os.path.join(d, "qw.slater")).read()
@@ -44,7 +44,7 @@ def test_determinant_input_checks(tmp_path):
     for txt, status in [(base.replace("DETWT { 1.0 }", "DETWT { 1.0 0.5 }"), -2),
                         (base.replace("DETWT { 1.0 }", "CSF { 1.0 }"), -2),
                         (base.replace("DETWT { 1.0 }", "DETWT { 1.0 } CSF { 1.0 1.0 }"), -2),
-                        (re.sub(r"CENTERS\s*\{[^}]*\}", "CENTERS { READ qw.centers }", base, flags=re.I), -3)]:
+                        (re.sub(r"CENTERS\s*\{[^}]*\}", "CENTERS { READ qw.centers }", base, flags=re.I), -2)]:
         (tmp_path / "qw.slater").write_text(txt)
         with pytest.raises(E.QwError) as ex:
             E.Engine(str(tmp_path / "x.wfeval"))
@@ -53,6 +53,14 @@ def test_determinant_input_checks(tmp_path):
     with pytest.raises(E.QwError) as ex:
         E.Engine(deck, "mixed_fp32")
     assert ex.value.status == -3
+    # BLAS_MO orbitals and basis centers read from a file are read (on a machine without a GPU
+    # engine creation then stops at the device)
+    for name in ("n2_blas", "n2_cen"):
+        g, deck = golden(name)
+        try:
+            E.Engine(deck, "fp64").close()
+        except E.QwError as ex:
+            assert ex.status == -4, (name, ex)
 
 
 def test_method_read_applies_reference_defaults(tmp_path):

@@ -12,7 +12,7 @@ from tests._util import golden, relerr, scaled_err, synth_deck
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["h_atom", "h2_jast", "n2_pp", "h4", "pp16", "ae32", "fg3"]
+CASES = ["h_atom", "h2_jast", "n2_pp", "n2_cen", "h4", "pp16", "ae32", "fg3"]
 TOL = 1e-10  # FP64 path: relative tolerance against the reference's FP64 evaluation
 
 
@@ -35,6 +35,28 @@ def test_fp64_eval_matches_reference_golden(name):
     assert relerr(r["eparts"][:, 0], g["kinetic"]) < TOL
     assert relerr(r["eparts"][:, 1], g["vloc"]) < TOL
     assert np.max(np.abs(r["eparts"][:, 2] - g["vnonloc"])) <= TOL * max(1.0, np.max(np.abs(g["vnonloc"])))
+
+
+def test_blas_mo_deck_matches_reference_golden():
+    """ORBITALS { BLAS_MO }: the reference evaluates the same orbitals densely and leaves out the
+    basis functions whose value at the point is below 1e-8 (src/orbitals/MO_matrix_blas.h:
+    411-491); the engine evaluates them exactly (as CUTOFF_MO), so the values agree to the size
+    of the dropped terms (measured: ln|Psi| 7e-10, gradients 2e-8, laplacians 8.5e-7, kinetic
+    2.6e-7 relative), against 1e-10 for the other decks."""
+    g, deck = golden("n2_blas")
+    eng = E.Engine(deck, "fp64")
+    r = eng.eval(g["pos"])
+    err = dict(logpsi=relerr(r["logpsi"], g["logpsi"]),
+               grad=max(scaled_err(r["grad"][k], g["grad"][k]) for k in range(len(g["logpsi"]))),
+               lap=max(scaled_err(r["lap"][k], g["lap"][k]) for k in range(len(g["logpsi"]))),
+               kin=relerr(r["eparts"][:, 0], g["kinetic"]),
+               vnl=float(np.max(np.abs(r["eparts"][:, 2] - g["vnonloc"])) / max(1.0, np.max(np.abs(g["vnonloc"])))))
+    _say("blas_mo", {k: "%.1e" % v for k, v in err.items()})
+    assert np.array_equal(r["sign"], g["sign"])
+    tol = dict(logpsi=1e-8, grad=1e-7, lap=5e-6, kin=2e-6, vnl=1e-7)
+    for k, v in err.items():
+        assert v < tol[k], (k, v)
+    assert relerr(r["eparts"][:, 1], g["vloc"]) < TOL
 
 
 @pytest.mark.parametrize("name", ["h4", "pp16", "ae32"])
