@@ -169,6 +169,27 @@ qw_status qw_walkers_get(qw_engine* e, int nw, double* pos, double* weights);
  *   (src/wavefunction/Slat_wf.h:1021-1080).  Any pointer may be NULL. */
 qw_status qw_walkers_state(qw_engine* e, int nw, double* logdet, double* sign, double* elast, double* dgl);
 
+/* Walker files (<runid>.config), host only, in the reference's text layout so either program
+ * restarts from the other's checkpoint.
+ * qw_configs_read: read_configurations<Dmc_point> (src/program_utils/qmc_io.h:427-503,
+ *   src/system/Sample_point.cpp:120-139, src/method/Dmc_method.cpp:1252-1265).  *nw = walkers
+ *   in the file; the first min(*nw, max_nw) go to pos[.][Ne][3], weights[.] (1 for a walker
+ *   without a DMC record) and signs[.]; any of the three may be NULL (count only).  A walker
+ *   with another electron count is an error (QW_ERR_INPUT).
+ * qw_configs_write: write_configurations (src/program_utils/qmc_io.h:371-423) of
+ *   Config_save_point (weights NULL: VMC, src/system/Sample_point.cpp:140-148) or Dmc_point
+ *   records (weights given; signs NULL = all 1; src/method/Dmc_method.cpp:1228-1250); the old
+ *   file becomes <path>.backup.
+ * qw_method_files: READCONFIG and STORECONFIG of the index-th METHOD section, each
+ *   <deck>.config when absent (src/method/Vmc_method.cpp:59-62, src/method/Dmc_method.cpp:62-63,
+ *   91-92), into buffers of len bytes.
+ * qw_configs_error: message of the calling thread's last failed call of these three. */
+qw_status qw_configs_read(const char* path, int ne, int max_nw, int* nw, double* pos, double* weights, int* signs);
+qw_status qw_configs_write(const char* path, int nw, int ne, const double* pos, const double* weights,
+                           const int* signs);
+qw_status qw_method_files(const char* deck_path, int index, char* readconfig, char* storeconfig, int len);
+const char* qw_configs_error(void);
+
 /* Fields of one block record written by qw_run (the quantities of a QWalk block log,
  * src/properties/Properties.cpp:578-641, src/properties/Properties_block.cpp:73-138):
  *   0 E, 1 kinetic, 2 Coulomb, 3 nonlocal      weighted block averages

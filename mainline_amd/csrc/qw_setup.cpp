@@ -882,7 +882,13 @@ std::vector<MethodDesc> read_methods(const std::string& path) {
   lex(path, all);
   std::vector<MethodDesc> out;
   size_t pos = 0;
-  while (section(all, pos, m, "METHOD")) out.push_back(method_of(m));
+  while (section(all, pos, m, "METHOD")) {
+    out.push_back(method_of(m));
+    MethodDesc& d = out.back();
+    d.readconfig = d.storeconfig = path + ".config";
+    value(m, d.readconfig, "READCONFIG");
+    value(m, d.storeconfig, "STORECONFIG");
+  }
   if (out.empty()) throw std::runtime_error("no METHOD section in " + path);
   return out;
 }

@@ -114,6 +114,9 @@ struct MethodDesc {
   int tmoves = 0;  // 1 TMOVES, 2 TMOVESSC
   double start_feedback = 1.0;  // DMC: feedback of the first trial energy
   std::string label = "vmc";    // block-log label (LABEL)
+  // walker files: READCONFIG / STORECONFIG, both <deck>.config by default
+  // (src/method/Vmc_method.cpp:59-62, src/method/Dmc_method.cpp:62-63,91-92)
+  std::string readconfig, storeconfig;
 };
 
 // Throws std::runtime_error with a message on bad or unsupported input.

@@ -6,12 +6,13 @@
 //   <deck>.json    one record per block, each the one-block final average followed by <RS>
 //                  (src/method/Dmc_method.cpp:541-549, src/method/Vmc_method.cpp:526-534,
 //                  src/properties/Properties_average.cpp:449-520)
-//   <deck>.config  walker positions (+ DMC weight) (src/program_utils/qmc_io.h:371-423,
-//                  src/system/Sample_point.cpp:120-148, src/method/Dmc_method.cpp:1228-1265)
-// Walkers come from <deck>.config when it exists (READCONFIG), otherwise from the reference's
-// random guess followed by VMC equilibration (src/method/Generate_sample.cpp:24-71).
-//
-// Every METHOD section runs in input order on the same walkers (src/program_utils/ooqmc.cpp:148-170);
+//   STORECONFIG    walker positions (+ DMC weight and sign), <deck>.config by default
+//                  (qw_configs_write: src/program_utils/qmc_io.h:371-423)
+// Every METHOD section runs in input order (src/program_utils/ooqmc.cpp:148-170); each takes
+// its walkers from its READCONFIG file (<deck>.config by default, so the previous method's
+// STORECONFIG) when it exists, otherwise from the reference's random guess followed by VMC
+// equilibration (src/method/Generate_sample.cpp:24-71, src/method/Vmc_method.cpp:362,
+// src/method/Dmc_method.cpp:347,733-759);
 // a VMC section without TIMESTEP gets a timestep tuned toward acceptance 0.7.
 //
 // usage: qwalk_amd <deck> [--precision fp64|mixed_fp32|mixed_bf16|mixed_bf16x6] [--device N]
@@ -33,53 +34,6 @@ namespace {
 int fail(const char* what, const qw_engine* e) {
   fprintf(stderr, "qwalk_amd: %s: %s\n", what, qw_last_error(e));
   return 1;
-}
-
-bool read_configs(const std::string& path, int ne, std::vector<double>& pos, std::vector<double>& w) {
-  std::ifstream in(path);
-  if (!in) return false;
-  std::string tok;
-  while (in >> tok) {
-    if (tok != "nElec") continue;
-    int n = 0, nd = 0;
-    std::string d;
-    in >> n >> d >> nd;
-    if (n != ne || nd != 3) return false;
-    for (int i = 0; i < 3 * n; i++) {
-      double x;
-      in >> x;
-      pos.push_back(x);
-    }
-    double wt = 1.0;
-    std::streampos at = in.tellg();
-    std::string next;
-    if (in >> next && next == "DMC") {
-      std::string br, key;
-      in >> br >> key >> wt;
-    } else {
-      in.clear();
-      in.seekg(at);
-    }
-    w.push_back(wt);
-  }
-  return !w.empty();
-}
-
-void write_configs(const std::string& path, int nw, int ne, const std::vector<double>& pos,
-                   const std::vector<double>& w, bool dmc) {
-  std::rename(path.c_str(), (path + ".backup").c_str());
-  std::ofstream os(path);
-  os.precision(15);
-  for (int i = 0; i < nw; i++) {
-    os << " walker { \n";
-    os << "nElec " << ne << " ndim 3\n";
-    for (int e = 0; e < ne; e++) {
-      const double* p = &pos[((size_t)i * ne + e) * 3];
-      os << p[0] << " " << p[1] << " " << p[2] << " \n";
-    }
-    if (dmc) os << "DMC { \nweight " << w[i] << "\nsign 1\n}\n";
-    os << " } \n";
-  }
 }
 
 void write_log(const std::string& path, const char* label, double tau, bool dmc, const std::vector<double>& eb,
@@ -237,47 +191,52 @@ int main(int argc, char** argv) {
   int info[6];
   qw_engine_info(e, info);
   const int ne = info[0] + info[1];
-  const std::string cfg = deck + ".config";
   std::vector<double> pos, w;
-  int nw = 0;  // walkers resident on the device
   for (int mi = 0; mi < nmethod; mi++) {
     qw_method_desc m;
     char label[64];
     if (qw_method_read_at(deck.c_str(), mi, &m, label, sizeof label) != QW_OK) return fail("METHOD", nullptr);
     const bool dmc = m.method == 1;
     if (walkers_override > 0) m.nconfig = walkers_override;
-    // walkers: the previous method's population (the reference hands them over through the
-    // .config file), else <deck>.config (READCONFIG), else the random guess + VMC equilibration
-    std::vector<double> p2, w2;
-    bool have = false, fresh = false;
-    if (nw > 0) {
-      p2.resize((size_t)nw * ne * 3);
-      w2.resize(nw);
-      if (qw_walkers_get(e, nw, p2.data(), w2.data()) != QW_OK) return fail("walkers", e);
-      have = true;
-    } else {
-      have = read_configs(cfg, ne, p2, w2);
+    char readcfg[4096], storecfg[4096];
+    if (qw_method_files(deck.c_str(), mi, readcfg, storecfg, sizeof readcfg) != QW_OK) {
+      fprintf(stderr, "qwalk_amd: METHOD: %s\n", qw_configs_error());
+      return 1;
     }
+    // walkers, as the reference takes them: every method reads its READCONFIG file (the
+    // previous method's STORECONFIG by default) when that exists.  DMC keeps the first NCONFIG
+    // walkers with their weights and stops when the file holds fewer
+    // (src/method/Dmc_method.cpp:733-759); VMC keeps the first NCONFIG and fills the rest with
+    // the random guess (src/method/Vmc_method.cpp:270-289).  Without a file both start from the
+    // random guess; VMC always runs its warm-up first (src/method/Vmc_method.cpp:362-364).
     const int nwn = m.nconfig;
     pos.assign((size_t)nwn * ne * 3, 0.0);
     w.assign(nwn, 1.0);
-    if (have) {
-      // round-robin reuse of the stored walkers, as read_configurations distributes them
-      const int nr = (int)w2.size();
-      for (int i = 0; i < nwn; i++) {
-        for (int k = 0; k < ne * 3; k++) pos[(size_t)i * ne * 3 + k] = p2[(size_t)(i % nr) * ne * 3 + k];
-        w[i] = dmc ? w2[i % nr] : 1.0;
+    int nr = 0;
+    if (FILE* f = fopen(readcfg, "r")) {
+      fclose(f);
+      if (qw_configs_read(readcfg, ne, nwn, &nr, pos.data(), w.data(), nullptr) != QW_OK) {
+        fprintf(stderr, "qwalk_amd: %s\n", qw_configs_error());
+        return 1;
       }
-    } else {
-      if (qw_walkers_random(e, nwn, opts.seed, 3.0, pos.data()) != QW_OK) return fail("walkers", e);
-      fresh = true;
+      if (dmc && nr < nwn) {
+        fprintf(stderr, "qwalk_amd: not enough configurations in %s (%d, DMC needs %d)\n", readcfg, nr, nwn);
+        return 1;
+      }
+      printf("read %d walkers from %s\n", std::min(nr, nwn), readcfg);
+      if (!dmc) std::fill(w.begin(), w.end(), 1.0);
     }
-    nw = nwn;
+    const bool fresh = nr == 0;
+    if (nr < nwn &&
+        qw_walkers_random(e, nwn - nr, opts.seed + (unsigned long long)mi, 3.0, pos.data() + (size_t)nr * ne * 3) !=
+            QW_OK)
+      return fail("walkers", e);
+    const int nw = nwn;
     if (qw_walkers_set(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
     const bool auto_tau = m.timestep <= 0;
-    if (fresh) {
-      // equilibration from the random guess (VMC, depth-2 split moves; the reference's
-      // warm-up starts from timestep 1 when it tunes it)
+    if (fresh || !dmc) {
+      // warm-up from the random guess, and before every VMC method: VMC with depth-2 split
+      // moves (the reference's warm-up starts from timestep 1 when it tunes it)
       qw_method_desc eq = m;
       eq.method = 0;
       eq.nblock = 1;
@@ -294,7 +253,10 @@ int main(int argc, char** argv) {
     write_log(deck + ".log", label, m.timestep, dmc, eb, m.nblock);
     write_json(deck + ".json", eb, m.nblock);
     if (qw_walkers_get(e, nw, pos.data(), w.data()) != QW_OK) return fail("walkers", e);
-    write_configs(cfg, nw, ne, pos, w, dmc);
+    if (qw_configs_write(storecfg, nw, ne, pos.data(), dmc ? w.data() : nullptr, nullptr) != QW_OK) {
+      fprintf(stderr, "qwalk_amd: %s\n", qw_configs_error());
+      return 1;
+    }
     double s = 0, s2 = 0;
     for (int b = 0; b < m.nblock; b++) s += eb[(size_t)b * QW_BLOCK_FIELDS];
     s /= std::max(m.nblock, 1);

@@ -101,6 +101,12 @@ def lib():
         L.qw_server_error.restype = ctypes.c_char_p
         L.qw_server_error.argtypes = [ctypes.c_void_p, ctypes.c_longlong]
         L.qw_server_destroy.argtypes = [ctypes.c_void_p]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.qw_configs_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _ip]
+        L.qw_configs_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp, _ip]
+        L.qw_method_files.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.qw_configs_error.restype = ctypes.c_char_p
+        L.qw_configs_error.argtypes = []
         _lib = L
     return _lib
 
@@ -116,7 +122,8 @@ def exported_symbols():
               "qw_parm_deriv", "qw_batch_create", "qw_batch_destroy", "qw_batch_error", "qw_batch_run",
               "qw_batch_open", "qw_batch_join", "qw_batch_step", "qw_batch_leave",
               "qw_batch_kernel_times", "qw_server_create", "qw_server_submit",
-              "qw_server_wait", "qw_server_error", "qw_server_destroy"]:
+              "qw_server_wait", "qw_server_error", "qw_server_destroy", "qw_configs_read", "qw_configs_write",
+              "qw_method_files", "qw_configs_error"]:
         if hasattr(L, n):
             names.append(n)
     return names
@@ -149,6 +156,43 @@ def read_methods(deck):
             raise QwError(st, lib().qw_last_error(None).decode())
         out.append((m, lab.value.decode()))
     return out
+
+
+def method_files(deck, index=0):
+    """(READCONFIG, STORECONFIG) of the index-th METHOD section (<deck>.config by default)."""
+    r, w = ctypes.create_string_buffer(4096), ctypes.create_string_buffer(4096)
+    st = lib().qw_method_files(deck.encode(), index, r, w, 4096)
+    if st:
+        raise QwError(st, lib().qw_configs_error().decode())
+    return r.value.decode(), w.value.decode()
+
+
+def read_configs(path, ne):
+    """Walker file -> (pos[nw][ne][3], weights[nw], signs[nw]) (weights 1 without DMC records)."""
+    L, n = lib(), ctypes.c_int()
+    st = L.qw_configs_read(path.encode(), ne, 0, ctypes.byref(n), None, None, None)
+    if st:
+        raise QwError(st, L.qw_configs_error().decode())
+    pos = np.zeros((n.value, ne, 3))
+    w = np.zeros(n.value)
+    sg = np.zeros(n.value, dtype=np.int32)
+    st = L.qw_configs_read(path.encode(), ne, n.value, ctypes.byref(n), _p(pos), _p(w),
+                           sg.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if st:
+        raise QwError(st, L.qw_configs_error().decode())
+    return pos, w, sg
+
+
+def write_configs(path, pos, weights=None, signs=None):
+    """Walker file in the reference's layout: VMC records, or DMC records when weights are given."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    nw, ne = pos.shape[0], pos.shape[1]
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    sg = None if signs is None else np.ascontiguousarray(signs, dtype=np.int32)
+    st = lib().qw_configs_write(path.encode(), nw, ne, _p(pos), _p(w),
+                                None if sg is None else sg.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if st:
+        raise QwError(st, lib().qw_configs_error().decode())
 
 
 def method(kind, nconfig, nblock, nstep, timestep, ndecorr=2, feedback_interval=5, feedback=1.0,

@@ -11,8 +11,10 @@ import shutil
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
+from mainline_amd import engine as E
 from tests._util import GOLDEN, ROOT
 
 pytestmark = pytest.mark.gpu
@@ -125,7 +127,9 @@ def test_n2_ci_vmc_through_driver_and_gosling(tmp_path):
 @pytest.mark.skipif(not os.path.exists(GOSLING), reason="reference gosling not built (oracle/Makefile.ref)")
 def test_vmc_then_dmc_deck_through_driver_and_gosling(tmp_path):
     """A deck with a VMC section (no TIMESTEP: tuned toward acceptance 0.7) followed by a DMC
-    section with its own LABEL: both run in order on the same walkers, and the reference's
+    section with its own LABEL: both run in order, the DMC on the walkers the VMC stored in
+    the deck's walker file (as the reference hands them over; the reference's DMC needs at
+    least NCONFIG of them, src/method/Dmc_method.cpp:752-759), and the reference's
     gosling reblocks each label of the one log (HF -0.4998098113 and exact -0.5, within 4
     sigma: the tuned timestep is near 2, where the split moves of VMC stay exact -- the oracle
     gives -0.49967 +- 0.00037 there -- but a 40-block error bar is itself uncertain by ~11 %)."""
@@ -133,7 +137,7 @@ def test_vmc_then_dmc_deck_through_driver_and_gosling(tmp_path):
     for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb"]:
         shutil.copy(os.path.join(d, f), tmp_path)
     (tmp_path / "qw.run").write_text("method { VMC nconfig 256 nstep 100 nblock 40 }\n"
-                                     "method { DMC timestep 0.005 nblock 60 label hdmc }\n"
+                                     "method { DMC timestep 0.005 nblock 60 nconfig 256 label hdmc }\n"
                                      "include qw.sys\ntrialfunc { include qw.slater }\n")
     print("drop-in: VMC then DMC", flush=True)
     out = subprocess.run([DRIVER, "qw.run"], cwd=tmp_path, capture_output=True, text=True, timeout=200)
@@ -151,3 +155,60 @@ def test_vmc_then_dmc_deck_through_driver_and_gosling(tmp_path):
         print("gosling", label, v, "+/-", e, flush=True)
         assert e < 0.1
         assert abs(v - ref) / e <= 4.0
+
+
+QWALK_REF = os.path.join(ROOT, "oracle", "_ref", "qwalk")
+
+
+def _gosling_energy(cwd, log):
+    js = subprocess.run([GOSLING, "-json", log], cwd=cwd, capture_output=True, text=True, timeout=60)
+    assert js.returncode == 0, js.stderr
+    props = json.loads(js.stdout)["properties"]
+    return props["total_energy"]["value"][0], props["total_energy"]["error"][0]
+
+
+@pytest.mark.skipif(not (os.access(QWALK_REF, os.X_OK) and os.path.exists(GOSLING)),
+                    reason="reference qwalk / gosling not built (oracle/Makefile.ref)")
+def test_restart_through_walker_files_in_both_directions(tmp_path):
+    """Checkpoint and restart across programs through walker files (READCONFIG / STORECONFIG,
+    src/method/Dmc_method.cpp:62-63,91-92,347,478,733-759): the reference program runs H-atom
+    DMC and stores its walkers; the driver restarts DMC from that file (no warm-up) and stores
+    its own; the reference restarts DMC from the driver's file; the driver restarts once more
+    from its own file.  Each restart reads the walkers the previous run stored (count, and the
+    driver, run with no blocks on its own file, stores the same bytes), and every DMC energy
+    agrees with the exact -0.5 within 3 sigma."""
+    d = os.path.join(GOLDEN, "decks", "h_atom")
+    for f in ["qw.sys", "qw.slater", "qw.basis", "qw.orb"]:
+        shutil.copy(os.path.join(d, f), tmp_path)
+    tail = "include qw.sys\ntrialfunc { include qw.slater }\n"
+    dmc = "method { DMC timestep 0.005 nblock 30 nconfig 128 readconfig %s storeconfig %s }\n"
+
+    def run(prog, name, rd, st, extra=""):
+        (tmp_path / name).write_text((dmc % (rd, st)) + extra + tail)
+        print("restart:", prog, name, rd, "->", st, flush=True)
+        out = subprocess.run([prog, name], cwd=tmp_path, capture_output=True, text=True, timeout=200)
+        print(out.stdout[-2000:], out.stderr[-2000:], flush=True)
+        assert out.returncode == 0
+        v, e = _gosling_energy(tmp_path, name + ".log")
+        print(name, "energy", v, "+/-", e, flush=True)
+        assert e < 0.1 and abs(v + 0.5) / e <= 3.0
+        return out.stdout
+
+    run(QWALK_REF, "ref1.in", "none1.config", "ref1.config", "randomseed { 11 23 }\n")
+    assert E.read_configs(str(tmp_path / "ref1.config"), 1)[0].shape[0] == 128
+    out = run(DRIVER, "ours1.in", "ref1.config", "ours1.config")
+    assert "read 128 walkers from ref1.config" in out
+    pos1, w1, _ = E.read_configs(str(tmp_path / "ours1.config"), 1)
+    assert pos1.shape[0] == 128 and not np.all(w1 == 1.0)
+    run(QWALK_REF, "ref2.in", "ours1.config", "ref2.config", "randomseed { 5 7 }\n")
+    assert E.read_configs(str(tmp_path / "ref2.config"), 1)[0].shape[0] == 128
+    # the driver reads its own file back and, with no blocks to run, stores the same walkers:
+    # the same bytes (positions and weights survive read -> device -> write unchanged)
+    (tmp_path / "chk.in").write_text((dmc % ("ours1.config", "chk.config")).replace("nblock 30", "nblock 0") + tail)
+    print("restart: driver chk.in (no blocks)", flush=True)
+    out = subprocess.run([DRIVER, "chk.in"], cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    print(out.stdout[-2000:], out.stderr[-2000:], flush=True)
+    assert out.returncode == 0
+    assert (tmp_path / "chk.config").read_bytes() == (tmp_path / "ours1.config").read_bytes()
+    out = run(DRIVER, "ours2.in", "ours1.config", "ours2.config")
+    assert "read 128 walkers from ours1.config" in out
