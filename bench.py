@@ -73,12 +73,15 @@ N2_DECK = os.path.join(ROOT, "tests", "golden", "decks", "n2_pp", "n2.wfeval")
 
 
 def accuracy_leg_n2(precs, nw=4096, nblock=40, nstep=20, tau=0.02, nequil=200, refresh=32):
-    """The paired comparison on a physical system with a stationary DMC energy: the reference's
-    N2 deck (tests/n2 basis, orbitals and BFD pseudopotential, with a Jastrow).  One FP64
-    equilibration (walkers from the reference's random guess, 50 VMC steps, nequil DMC steps)
-    gives a common population (positions and weights); every precision, FP64 included, then
-    runs nblock blocks of nstep steps from it with the same random streams, so the block-paired
-    differences measure what the precision changes, not the equilibration."""
+    """The precision comparison on a physical system with a stationary DMC energy: the
+    reference's N2 deck (tests/n2 basis, orbitals and BFD pseudopotential, with a Jastrow).
+    One FP64 equilibration (walkers from the reference's random guess, 50 VMC steps, nequil
+    DMC steps) gives the first population.  Then, block by block, FP64 runs nstep DMC steps
+    from population P_b and leaves P_(b+1); every other precision runs the same block from the
+    same P_b with the same random streams (each block starts a fresh DMC run on both sides:
+    trial energy from the population, step counters equal).  The trajectories of a block stay
+    close, so the block-paired differences measure what the precision changes with an error
+    far below the FP64 energy's own; the FP64 blocks give the energy and its error bar."""
     from mainline_amd import engine as E
     eq = E.Engine(N2_DECK, "fp64", seed=98)
     eq.set_walkers(eq.random_walkers(nw, seed=7))
@@ -86,23 +89,36 @@ def accuracy_leg_n2(precs, nw=4096, nblock=40, nstep=20, tau=0.02, nequil=200, r
     eq.run(E.method("DMC", nw, nblock=1, nstep=nequil, timestep=tau))
     pos, wts = eq.get_walkers(nw)
     eq.close()
-    out = {}
-    for p in precs:
-        eng = E.Engine(N2_DECK, p, seed=99, refresh_interval=0 if p == "fp64" else refresh)
-        eng.set_walkers(pos, wts)
-        eb = eng.run(E.method("DMC", nw, nblock=nblock, nstep=nstep, timestep=tau))["eblock"][:, 0]
-        out[p] = dict(mean=float(eb.mean()), err=float(eb.std(ddof=1) / np.sqrt(len(eb))), blocks=eb.tolist())
-        log("[accuracy N2 %s] E = %.5f +- %.5f" % (p, out[p]["mean"], out[p]["err"]))
+    m = E.method("DMC", nw, nblock=1, nstep=nstep, timestep=tau)
+    ref = E.Engine(N2_DECK, "fp64", seed=99)
+    others = {p: E.Engine(N2_DECK, p, seed=99, refresh_interval=refresh) for p in precs if p != "fp64"}
+    blocks = {p: [] for p in ["fp64"] + list(others)}
+    for b in range(nblock):
+        for p, eng in others.items():
+            eng.set_walkers(pos, wts)
+            blocks[p].append(float(eng.run(m)["eblock"][0, 0]))
+        ref.set_walkers(pos, wts)
+        blocks["fp64"].append(float(ref.run(m)["eblock"][0, 0]))
+        pos, wts = ref.get_walkers(nw)
+    ref.close()
+    for eng in others.values():
         eng.close()
-    ref = out.get("fp64")
+    out = {}
+    for p, eb in blocks.items():
+        eb = np.array(eb)
+        out[p] = dict(mean=float(eb.mean()), err=float(eb.std(ddof=1) / np.sqrt(len(eb))), blocks=eb.tolist())
+    r = out["fp64"]
+    log("[accuracy N2 fp64] E = %.5f +- %.5f" % (r["mean"], r["err"]))
     for p, v in out.items():
-        if ref is None or p == "fp64":
+        if p == "fp64":
             continue
-        d = np.array(v["blocks"]) - np.array(ref["blocks"])
+        d = np.array(v["blocks"]) - np.array(r["blocks"])
         v["paired_delta"] = float(d.mean())
         v["paired_err"] = float(d.std(ddof=1) / np.sqrt(len(d)))
         v["agrees_2sigma"] = bool(abs(v["paired_delta"]) <= 2 * v["paired_err"])
-        v["within_fp64_error_bar"] = bool(abs(v["paired_delta"]) <= ref["err"])
+        v["within_fp64_error_bar"] = bool(abs(v["paired_delta"]) <= r["err"])
+        log("[accuracy N2 %s] E = %.5f +- %.5f, paired difference from FP64 %+.2e +- %.2e Ha" % (
+            p, v["mean"], v["err"], v["paired_delta"], v["paired_err"]))
     return out
 
 
@@ -225,6 +241,54 @@ def batching_leg(precision, steps, tmp, warmup=90):
     return res
 
 
+def small_configs_leg(tmp, c2_walkers=4096, c2_steps=10, c2_warmup=3):
+    """The two smaller workloads at their own sizes in FP64: the 4-electron H4 chain VMC (256
+    walkers, timestep 0.5, 50 blocks of 20 steps, the whole run timed) and the 64-electron
+    pseudopotential molecule DMC (16 Z=4 atoms, 208 functions, 4096 walkers, tau 0.01;
+    c2_warmup untimed and c2_steps timed steps) with that step's kernel times: the local
+    energy (k_local, which holds the nonlocal pseudopotential's 12-point quadrature ratios)
+    beside the sweeps.  The pseudopotential's algorithmic flops per walker-step are the cost
+    model's estimate: 12 points x (2 Nb Nsigma + 2 Nsigma + Ne n_ee 12) per electron-atom pair
+    inside the cutoff, at 1.5 such pairs per electron."""
+    from mainline_amd import engine as E
+    from mainline_amd import synth
+    out = {}
+    h4 = synth.h4()
+    eng = E.Engine(h4.write(os.path.join(tmp, "c1")), "fp64", seed=11)
+    eng.set_walkers(synth.random_configs(h4, 256, seed=5))
+    t0 = time.perf_counter()
+    r = eng.run(E.method("VMC", 256, nblock=50, nstep=20, timestep=0.5))
+    dt = time.perf_counter() - t0
+    out["h4_vmc"] = dict(walkers=256, blocks=50, steps=20, seconds=dt, moves_per_s=r["moves"] / dt,
+                         energy=float(r["eblock"][:, 0].mean()), acceptance=float(r["acceptance"]))
+    eng.close()
+    log("[h4 VMC] %.3e electron-moves/s, whole run %.3f s" % (out["h4_vmc"]["moves_per_s"], dt))
+    pp = synth.pp16()
+    nw = c2_walkers
+    eng = E.Engine(pp.write(os.path.join(tmp, "c2")), "fp64", seed=12)
+    eng.set_walkers(synth.nonsingular_configs(pp, nw, lambda q: eng.eval(q)["logpsi"], seed=6))
+    eng.run(E.method("DMC", nw, nblock=1, nstep=c2_warmup, timestep=0.01))
+    t0 = time.perf_counter()
+    r = eng.run(E.method("DMC", nw, nblock=1, nstep=c2_steps, timestep=0.01))
+    dt = time.perf_counter() - t0
+    kt = {k: (a, int(c)) for k, (a, c) in eng.kernel_times().items() if c > 0}
+    eng.close()
+    per_step = {k: a * c / c2_steps for k, (a, c) in kt.items()}
+    local = sum(v for k, v in per_step.items() if k.startswith("k_local"))
+    sweep = sum(v for k, v in per_step.items() if k.startswith("k_sweep"))
+    ns, nb, ne = pp.nup, pp.nbasis, pp.nelec
+    f_pp = ne * 1.5 * 12 * (2.0 * nb * ns + 2 * ns + ne * 6 * 12)
+    f_em = flops_per_move(nb, ns, ne, pp.natoms, 5 * pp.natoms, float(r["acceptance"]), n_ee=6, n_ei=4)
+    out["pp16_dmc"] = dict(walkers=nw, steps=c2_steps, seconds=dt, moves_per_s=nw * ne * c2_steps / dt,
+                           ms_per_step=1e3 * dt / c2_steps, kernel_ms_per_step=per_step,
+                           local_energy_ms_per_step=local, sweep_ms_per_step=sweep,
+                           pp_flops_per_walker_step=f_pp, move_flops_per_walker_step=f_em * ne,
+                           pp_over_moves_flops=f_pp / (f_em * ne))
+    log("[pp16 DMC] %.3e electron-moves/s, %.2f ms/step (sweeps %.2f, local energy + PP %.2f)" % (
+        out["pp16_dmc"]["moves_per_s"], out["pp16_dmc"]["ms_per_step"], sweep, local))
+    return out
+
+
 def serving_leg(precision, njobs, rate, seed, tmp):
     """Poisson stream of VMC/DMC requests (mean `rate` jobs/s) over molecules of 8..512
     electrons (log-uniform, up to 1664 basis functions), 1024 walkers and 1000..20000
@@ -253,19 +317,28 @@ def serving_leg(precision, njobs, rate, seed, tmp):
             time.sleep(wait)
         jobs.append((srv.submit(decks[ne][0], m, 1024, precision=precision, seed=1000 + i,
                                 equil_steps=3 if dmc else 0), ne))
-    lat, moves, t_end = [], 0.0, 0.0
+    lat, moves, t_end, queue, setup, run, batched = [], 0.0, 0.0, [], [], [], 0
     for jid, ne in jobs:
         eb, st = srv.wait(jid)
         lat.append(st[3])
         moves += st[4]
         t_end = max(t_end, st[2])
+        queue.append(st[1] - st[0])
+        setup.append(st[8])
+        run.append(st[2] - st[1] - st[8])
+        batched += int(st[9])
     srv.close()
     lat = np.array(lat)
     span = t_end - 0.0
+    q50 = lambda x: float(np.percentile(x, 50))  # noqa: E731
     res = dict(jobs=njobs, rate_per_s=rate, p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-               mean_s=float(lat.mean()), moves_per_s=float(moves / max(span, 1e-9)), wall_s=float(span))
-    log("[serving %s] %d jobs: p50 %.3f s, p99 %.3f s, %.3e electron-moves/s" % (
-        precision, njobs, res["p50_s"], res["p99_s"], res["moves_per_s"]))
+               mean_s=float(lat.mean()), moves_per_s=float(moves / max(span, 1e-9)), wall_s=float(span),
+               queue_p50_s=q50(queue), setup_p50_s=q50(setup), setup_total_s=float(np.sum(setup)),
+               run_p50_s=q50(run), jobs_in_batches=batched)
+    log("[serving %s] %d jobs: p50 %.3f s, p99 %.3f s, %.3e electron-moves/s (p50 queue %.3f s, set-up %.3f s, "
+        "run %.3f s; set-up total %.1f s; %d jobs in batches)" % (
+            precision, njobs, res["p50_s"], res["p99_s"], res["moves_per_s"], res["queue_p50_s"], res["setup_p50_s"],
+            res["run_p50_s"], res["setup_total_s"], batched))
     return res
 
 
@@ -377,6 +450,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batching leg (0 disables it)")
     ap.add_argument("--refresh", type=int, default=32,
                     help="mixed precision: FP64 inverse recompute every this many moves")
+    ap.add_argument("--atoms", type=int, default=32,
+                    help="Z=4 atoms of the timed molecule (32 = the 128-electron workload)")
+    ap.add_argument("--small-configs", type=int, default=1,
+                    help="1 = also time the H4 VMC and 64-electron pseudopotential DMC workloads (FP64)")
     ap.add_argument("--error-configs", type=int, default=10000,
                     help="fixed configurations of the per-quantity error report (0 disables it)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -390,7 +467,9 @@ def main():
               "walkers over several GPUs is not run by this benchmark.", file=sys.stderr)
         sys.exit(2)
     from mainline_amd import synth
-    mol = synth.ae32()
+    # the 128-electron workload; --atoms N times the same construction at another size (for
+    # profiling the 65..256-electrons-per-spin kernels), reported under its own name
+    mol = synth.ae32() if a.atoms == 32 else synth.z4_molecule(a.atoms, 20261204 + a.atoms, name="ae%d" % a.atoms)
     tmp = tempfile.mkdtemp(prefix="qwbench_")
     deck = mol.write(tmp)
     tau = 0.01
@@ -444,6 +523,7 @@ def main():
             "VALU (%s, non-packed)" % ("FP64" if head == "fp64" else "FP32"),
             "kernel": kname, "avg_ms": avg_ms, "flops_per_move": f_em, "moves_per_launch": moves_per_launch,
             "launch_choice": R.get("tuning", {}).get("sweep")}
+    small = small_configs_leg(tmp) if a.small_configs else None
     batching = None
     if a.batch_steps > 0:
         batching = batching_leg("mixed_fp32", a.batch_steps, tmp)
@@ -461,12 +541,14 @@ def main():
         "dtype": "f64" if head == "fp64" else "mixed(%s LCAO, f32 inverse, f64 log-det/weights/sums)" % (
             {"mixed_fp32": "f32", "mixed_bf16": "bf16x3 MFMA", "mixed_bf16x6": "bf16x6 MFMA"}[head]),
         "data": "synthetic (seeded molecule, random-init walkers)",
-        "config": {"workload": "ae32: 128-electron DMC, 32 Z=4 atoms, 416 spline basis functions, "
-                               "Slater-Jastrow, tau 0.01", "walkers": a.walkers, "electrons": ne,
-                   "basis": nb, "precision": head,
+        "config": {"workload": "%s: %d-electron DMC, %d Z=4 atoms, %d spline basis functions, "
+                               "Slater-Jastrow, tau 0.01" % (mol.name, mol.nelec, mol.natoms, mol.nbasis),
+                   "walkers": a.walkers, "electrons": ne, "basis": nb, "precision": head,
                    "fp64_recompute_every_moves": a.refresh if head != "fp64" else None},
         "roofline": roof, "cpu_baseline": cpu, "equal_energy_gate": gate,
     }
+    if small is not None:
+        line["smaller_workloads_fp64"] = small
     for p in results:
         line[p] = {k: results[p][k] for k in ("moves_per_s", "ms_per_step", "energy", "acceptance")}
         if p != "fp64" and "fp64" in results:

@@ -91,6 +91,12 @@ typedef struct qw_method_desc {
  * paths are resolved relative to the deck's directory.  On failure *out is NULL and the
  * message is available from qw_last_error(NULL). */
 qw_status qw_engine_create(const char* deck_path, const qw_engine_opts* opts, qw_engine** out);
+/* An engine for the same deck as src (its parsed tables; the deck is not read again) with
+ * other options (precision, device, seed, refresh interval); it gives exactly what
+ * qw_engine_create of that deck with these options gives.  No reference counterpart (the
+ * reference reads its input once per process); for servers that start many jobs of one
+ * deck. */
+qw_status qw_engine_clone(const qw_engine* src, const qw_engine_opts* opts, qw_engine** out);
 void qw_engine_destroy(qw_engine* e);
 const char* qw_last_error(const qw_engine* e);
 
@@ -293,8 +299,9 @@ typedef struct qw_job_desc {
 } qw_job_desc;
 
 /* stats of a finished job (seconds on the server's clock, from qw_server_create) */
-#define QW_JOB_STATS 8   /* 0 submit, 1 admitted, 2 finished, 3 latency (finished - submit),
-                            4 electron moves, 5 run seconds, 6 electron-moves/s, 7 device bytes */
+#define QW_JOB_STATS 10  /* 0 submit, 1 admitted, 2 finished, 3 latency (finished - submit),
+                            4 electron moves, 5 run seconds, 6 electron-moves/s, 7 device bytes,
+                            8 set-up seconds (engine and walkers), 9 1 = ran in a batch */
 
 qw_status qw_server_create(const qw_server_opts* opts, qw_server** out);
 /* Queue a job; the deck is read when the job is admitted.  *job_id >= 1. */

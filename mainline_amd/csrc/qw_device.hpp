@@ -811,6 +811,58 @@ __device__ inline double jastrow_e_val(const DevSys& S, int e, const double p[3]
   return jastrow_e_t<double, false>(S, S.ee, S.eit, e, p, pos, g, nullptr, lane, S.eit_pos, 3);
 }
 
+// The same value for electron e at each of the points p[0..nq-1] (NQ >= nq), in one pass over
+// the objects and pairs: each point's terms are added in the order jastrow_e_val adds them, so
+// u[i] equals jastrow_e_val at p[i] bit for bit.
+template <int NQ>
+__device__ inline void jastrow_e_val_multi(const DevSys& S, int e, const double (*p)[3], int nq, const double* pos,
+                                           double u[NQ], int lane) {
+  double uq[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; i++) uq[i] = 0;
+  const QW_CONST int* ea0 = cptr(S.eit_a0);
+  const QW_CONST int* eo0 = cptr(S.eit_o0);
+  for (int t = 0; t < S.n_eit; t++) {
+    const int a0 = ea0[t], a1 = ea0[t + 1], o1 = eo0[t + 1];
+    for (int o = eo0[t]; o < o1; o++) {
+      with_jrad<double, false>(cptr(S.eit)[o], [&](const auto& ev) {
+        for (int k = a0 + lane; k < a1; k += kWave) {
+          const double* ap = S.eit_pos + 3 * k;
+#pragma unroll
+          for (int i = 0; i < NQ; i++) {
+            if (i >= nq) break;
+            const double x = p[i][0] - ap[0], y = p[i][1] - ap[1], z = p[i][2] - ap[2];
+            double r, ri;
+            dist_inv(x * x + y * y + z * z, r, ri);
+            double gf = 0, l = 0;
+            ev(0.0, r, ri, uq[i], gf, l);
+          }
+        }
+      });
+    }
+  }
+  const bool eup = e < S.nup;
+  for (int o = 0; o < S.nee; o++) {
+    with_jrad<double, false>(cptr(S.ee)[o], [&](const auto& ev) {
+      for (int j = lane; j < S.ne; j += kWave) {
+        if (j == e) continue;
+        const double uf = ((j < S.nup) != eup) ? 1.0 : 0.0;
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+          if (i >= nq) break;
+          const double x = p[i][0] - pos[3 * j], y = p[i][1] - pos[3 * j + 1], z = p[i][2] - pos[3 * j + 2];
+          double r, ri;
+          dist_inv(x * x + y * y + z * z, r, ri);
+          double gf = 0, l = 0;
+          ev(uf, r, ri, uq[i], gf, l);
+        }
+      }
+    });
+  }
+#pragma unroll
+  for (int i = 0; i < NQ; i++) u[i] = i < nq ? wsum(uq[i]) : 0.0;
+}
+
 // Semilocal pseudopotential channel values v_l(r) (src/basis_function/Rgaussian_function.cpp:180-202,
 // src/system/Pseudopotential.cpp:305-318).
 __device__ inline void pp_radial(const QW_CONST PPType& P, double Z, double r, double* vl) {

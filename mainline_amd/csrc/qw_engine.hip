@@ -65,12 +65,10 @@ qw_status guarded(qw_engine* e, F f) {
 }
 }  // namespace
 
-extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opts* opts, qw_engine** out) {
-  if (!out || !deck_path) {
-    g_create_error = "null argument";
-    return QW_ERR_ARG;
-  }
-  *out = nullptr;
+namespace {
+// An engine for parsed deck tables: the options, the size and feature checks, then the
+// device tables on opts->device.
+qw_status make_engine(SysTables T, const qw_engine_opts* opts, qw_engine** out) {
   std::unique_ptr<qw_engine> e(new qw_engine);
   if (opts) {
     e->precision = opts->precision;
@@ -82,20 +80,15 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
     g_create_error = "unknown precision";
     return QW_ERR_ARG;
   }
-  bool unsup = false;
-  try {
-    e->T = read_deck(deck_path, &unsup);
-  } catch (std::exception& x) {
-    g_create_error = x.what();
-    return unsup ? QW_ERR_UNSUPPORTED : QW_ERR_INPUT;
-  }
+  e->T = std::move(T);
   const int nmax = std::max(e->T.nup, e->T.ndn);
   if (nmax > kBigMaxSpin) {
     g_create_error = "more than 256 electrons of one spin are not supported by this engine build";
     return QW_ERR_UNSUPPORTED;
   }
   if (nmax > 64) {
-    // the block-per-walker path: no pseudopotential quadrature, no MFMA LCAO
+    // one wavefront per walker with the inverse in device memory: no pseudopotential
+    // quadrature, no MFMA LCAO
     if (!e->T.pptypes.empty()) {
       g_create_error = "pseudopotentials with more than 64 electrons of one spin are not supported";
       return QW_ERR_UNSUPPORTED;
@@ -139,6 +132,37 @@ extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opt
       if (x == std::atoi(c)) e->tune_u = x;
   *out = e.release();
   return QW_OK;
+}
+}  // namespace
+
+extern "C" qw_status qw_engine_create(const char* deck_path, const qw_engine_opts* opts, qw_engine** out) {
+  if (!out || !deck_path) {
+    g_create_error = "null argument";
+    return QW_ERR_ARG;
+  }
+  *out = nullptr;
+  if (opts && (opts->precision < 0 || opts->precision > 3)) {
+    g_create_error = "unknown precision";
+    return QW_ERR_ARG;
+  }
+  bool unsup = false;
+  SysTables T;
+  try {
+    T = read_deck(deck_path, &unsup);
+  } catch (std::exception& x) {
+    g_create_error = x.what();
+    return unsup ? QW_ERR_UNSUPPORTED : QW_ERR_INPUT;
+  }
+  return make_engine(std::move(T), opts, out);
+}
+
+extern "C" qw_status qw_engine_clone(const qw_engine* src, const qw_engine_opts* opts, qw_engine** out) {
+  if (!out || !src) {
+    g_create_error = "null argument";
+    return QW_ERR_ARG;
+  }
+  *out = nullptr;
+  return make_engine(src->T, opts, out);
 }
 
 extern "C" void qw_engine_destroy(qw_engine* e) { delete e; }

@@ -93,6 +93,94 @@ __device__ inline double lcao_val(const DevSys& S, int s, const double p[3], dou
   return lcao_val_n(S, s, s ? S.ndn : S.nup, p, chi, flist, lane);
 }
 
+// Basis values at the points p[0..nq-1] (NQ >= nq) over the union of their active shells of
+// shells sh0..sh1-1: chi[i*ld + b] for point i and packed function b (exactly zero where the
+// shell is outside its cutoff for that point), flist[b] the function index.  Returns the
+// number of packed functions.
+template <int NQ>
+__device__ inline int basis_val_multi(const DevSys& S, const double (*p)[3], int nq, int sh0, int sh1, double* chi,
+                                      int ld, int* flist, int lane) {
+  int base = 0;
+  for (int s0 = sh0; s0 < sh1; s0 += kWave) {
+    const int sh = s0 + lane;
+    ShellDesc d;
+    double x[NQ], y[NQ], z[NQ], r[NQ];
+    bool act[NQ];
+    bool any = false;
+    if (sh < sh1) {
+      d = S.shells[sh];
+      const double* ap = S.atom_pos + 3 * d.atom;
+#pragma unroll
+      for (int i = 0; i < NQ; i++) {
+        act[i] = false;
+        if (i >= nq) continue;
+        x[i] = p[i][0] - ap[0];
+        y[i] = p[i][1] - ap[1];
+        z[i] = p[i][2] - ap[2];
+        r[i] = sqrt(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
+        act[i] = r[i] < d.rcut && r[i] < d.objthr;
+        any |= act[i];
+      }
+    }
+    int tot;
+    const int at = base + wave_excl_scan(any ? d.nf : 0, lane, &tot);
+    if (any) {
+#pragma unroll
+      for (int i = 0; i < NQ; i++) {
+        if (i >= nq) break;
+        double* cq = chi + i * ld + at;
+        if (act[i]) {
+          int iv = (int)(r[i] * S.tab_inv[d.table]);
+          iv = iv < 0 ? 0 : (iv >= S.nint ? S.nint - 1 : iv);
+          const double* c = S.spline + ((size_t)d.table * S.nint + iv) * 4;
+          double h = r[i] - iv * S.tab_sp[d.table];
+          double f = c[0] + h * (c[1] + h * (c[2] + h * c[3]));
+          shell_val<double>(d.sym, f, x[i], y[i], z[i], cq);
+        } else {
+          for (int k = 0; k < d.nf; k++) cq[k] = 0.0;
+        }
+      }
+      for (int k = 0; k < d.nf; k++) flist[at + k] = d.f0 + k;
+    }
+    base += tot;
+  }
+  return base;
+}
+
+// MO j (lane j < n) at the points p[0..nq-1]: phi[i].  A function outside its cutoff for a
+// point adds an exact zero there, so phi[i] equals lcao_val at p[i] bit for bit; each
+// coefficient load serves every point.
+template <int NQ>
+__device__ inline void lcao_val_multi(const DevSys& S, int s, const double (*p)[3], int nq, double* chi, int* flist,
+                                      double phi[NQ], int lane) {
+  const int n = s ? S.ndn : S.nup, ld = S.chunk_nf;
+#pragma unroll
+  for (int i = 0; i < NQ; i++) phi[i] = 0;
+  for (int ch = 0; ch < S.nchunk; ch++) {
+    int sh0 = S.chunk_sh0[ch], sh1 = S.chunk_sh0[ch + 1];
+    __syncthreads();
+    const int nact = basis_val_multi<NQ>(S, p, nq, sh0, sh1, chi, ld, flist, lane);
+    __syncthreads();
+    if (lane < n) {
+      const double* Cp = S.C[s] + lane;
+      double a[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; i++) a[i] = 0;
+#pragma unroll 4
+      for (int b = 0; b < nact; b++) {
+        const double c = Cp[(size_t)flist[b] * n];
+#pragma unroll
+        for (int i = 0; i < NQ; i++) a[i] += c * chi[i * ld + b];
+      }
+#pragma unroll
+      for (int i = 0; i < NQ; i++) phi[i] += a[i];
+    }
+  }
+}
+
+// Quadrature points of a nonlocal pseudopotential term evaluated together in k_local.
+constexpr int kPPBatch = 4;
+
 // ---------------------------------------------------------------- k_mo_rows
 static __global__ __launch_bounds__(64) void k_mo_rows(DevSys S, DevWalkers W, int nw) {
   extern __shared__ double lds[];
@@ -263,16 +351,20 @@ __global__ __launch_bounds__(64) void k_inv(DevSys S, const TIN* mo, TI* inv_out
 // ---------------------------------------------------------------- k_inv_blk
 // The same inversion for 33..64 electrons of a spin with the matrix spread over the wave as
 // 8x8 blocks: lane (bi, bj) = (lane / 8, lane % 8) holds rows 8bi..8bi+7 and columns
-// 8bj..8bj+7, column c of its block as one register vector a[c][r].  A step exchanges only
-// the pivot row (8 values per lane) and the multipliers (8 per lane) through LDS, a quarter of
-// what the row-per-lane layout broadcasts, and every register index except the pivot row's is
-// a compile-time constant.  Each element sees the operations of k_inv in the same order (same
-// pivots, multipliers a[i][j] * (1/pivot), one FMA per step, column j replaced by -multiplier
-// or 1), so both kernels give bitwise-equal results.  Rows and columns >= n hold the identity
-// and are never pivots before step n.
+// 8bj..8bj+7, column c of its block as one register vector a[c][r].  A step exchanges the
+// pivot column, the pivot row and the multipliers through LDS (8 values per lane each way),
+// and every register index except the pivot row's is a compile-time constant.  The pivot
+// search runs with lane l standing for row l: the eight lanes holding column j write it to
+// LDS, every lane reads its row's entry and scale and the wave takes the largest candidate,
+// so each lane tests one candidate instead of eight, and lane l's multiplier is its own
+// entry times 1/pivot.  Each element sees the operations of k_inv in the same order (same
+// pivots, multipliers a[i][j] * (1/pivot), one FMA per step, column j replaced by
+// -multiplier or 1), so both kernels give bitwise-equal results.  Rows and columns >= n hold
+// the identity and are never pivots before step n.
 struct InvBlkLds {
   double prowL[2][64];
   double gL[2][64];
+  double colL[2][64];
   double pvs[64];
   int pl[64];
   int qinv[64];
@@ -283,6 +375,7 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
   typedef double dv8 __attribute__((ext_vector_type(8)));
   double(&prowL)[2][64] = L.prowL;
   double(&gL)[2][64] = L.gL;
+  double(&colL)[2][64] = L.colL;
   double* pvs = L.pvs;
   int* pl = L.pl;
   int* qinv = L.qinv;
@@ -299,7 +392,6 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
   dv8 a[8];
   double vv[8];
   bool zero_row = false;
-  int pivm = 0;  // bit r: row 8bi + r already a pivot (rows >= n count as such)
 #pragma unroll
   for (int r = 0; r < 8; r++) {
     const int row = 8 * bi + r;
@@ -318,8 +410,13 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
     mx = fmax(mx, dpp_d<kDppHalfMirror>(mx));
     vv[r] = mx == 0.0 ? 0.0 : 1.0 / mx;
     zero_row |= row < n && mx == 0.0;
-    if (row >= n) pivm |= 1 << r;
   }
+  // lane l = row l (row block bi, r = bj): its implicit scale, and whether it was a pivot
+  double vvl = vv[0];
+#pragma unroll
+  for (int r = 1; r < 8; r++)
+    if (bj == r) vvl = vv[r];
+  bool used = lane >= n;
   bool singular = wave_any(zero_row);
   bool done = singular;
   for (int J = 0; J < 8 && !done; J++) {
@@ -331,26 +428,22 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
         done = true;
         break;
       }
-      double best = -1.0, braw = 0.0;
-      int br = 0;
+      double* col = colL[j & 1];
+      if (mine) {
 #pragma unroll
-      for (int r = 0; r < 8; r++) {
-        const double x = a[c][r];
-        const double cd = (mine && !((pivm >> r) & 1)) ? vv[r] * fabs(x) : -1.0;
-        if (cd >= best) {
-          best = cd;
-          br = r;
-          braw = x;
-        }
+        for (int r = 0; r < 8; r++) col[8 * bi + r] = a[c][r];
       }
-      const int L = wave_argmax_last(best, lane);
-      // no candidate (NaN entries) or a zero pivot: singular
-      const double pv = L < 0 ? 0.0 : wave_bcast(braw, L);
-      if (L < 0 || !(wave_bcast(best, L) >= 0.0) || pv == 0.0) {
+      walker_sync<WPB>();
+      const double x = col[lane];
+      // candidate of row `lane` (largest vv * |a|, the last row on ties); NaN entries are none
+      double cd = used ? -1.0 : vvl * fabs(x);
+      if (!(cd >= 0.0)) cd = -1.0;
+      const int p = wave_argmax_last(cd, lane);
+      const double pv = p < 0 ? 0.0 : wave_bcast(x, p);
+      if (p < 0 || !(wave_bcast(cd, p) >= 0.0) || pv == 0.0) {
         singular = done = true;
         break;
       }
-      const int p = 8 * (L >> 3) + __builtin_amdgcn_readlane(br, L);
       const int pb = p >> 3, pr = p & 7;
       if (lane == 0) {
         pl[j] = p;
@@ -363,10 +456,7 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
         for (int c2 = 0; c2 < 8; c2++) prow[8 * bj + c2] = a[c2][pr];
       }
       const double rpv = 1.0 / pv;
-      if (mine) {
-#pragma unroll
-        for (int r = 0; r < 8; r++) gw[8 * bi + r] = (8 * bi + r == p) ? 0.0 : a[c][r] * rpv;
-      }
+      gw[lane] = lane == p ? 0.0 : x * rpv;
       walker_sync<WPB>();
       double pr8[8], g8[8];
 #pragma unroll
@@ -382,7 +472,7 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
 #pragma unroll
         for (int r = 0; r < 8; r++) a[c][r] = (8 * bi + r == p) ? 1.0 : -g8[r];
       }
-      if (bi == pb) pivm |= 1 << pr;
+      used |= lane == p;
     }
   }
   if (singular) {
@@ -1010,32 +1100,49 @@ __device__ __forceinline__ void local_body(const DevSys& S, const DevWalkers& W,
           const double ivk =
               (!MD && lane < n) ? (double)inv_store<TM>(W)[(((size_t)w * 2 + s) * nsp + k) * nsp + lane] : 0.0;
           double nl = 0, nneg = 0;
-          for (int q = 0; q < P.aip; q++) {
-            double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];
-            double qy = P.q[q][0] * rot[1] + P.q[q][1] * rot[4] + P.q[q][2] * rot[7];
-            double qz = P.q[q][0] * rot[2] + P.q[q][1] * rot[5] + P.q[q][2] * rot[8];
-            double np_[3] = {re[0] + (qx * orr - od[0]), re[1] + (qy * orr - od[1]), re[2] + (qz * orr - od[2])};
-            double nd[3] = {np_[0] - ap[0], np_[1] - ap[1], np_[2] - ap[2]};
-            double nr = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
-            double cs = (nd[0] * od[0] + nd[1] * od[1] + nd[2] * od[2]) / (nr * orr);
-            double Rq;
-            if constexpr (MD) {
-              const double phm = lcao_val_n(S, s, S.nmo[s], np_, chi, flist, lane);
-              __syncthreads();
-              if (lane < S.nmo[s]) phq[lane] = phm;
-              __syncthreads();
-              md_contract<1>(S, md_inv, s, k, phq, nsp, bw, md_den, &Rq, lane);
-            } else {
-              double phi = lcao_val(S, s, np_, chi, flist, lane);
-              Rq = wsum(ivk * phi);
+          // the quadrature points kPPBatch at a time: one pass of the basis, the LCAO
+          // (each coefficient load serving every point) and the Jastrow pairs per batch,
+          // each point's value bitwise that of a pass of its own
+          for (int q0 = 0; q0 < P.aip; q0 += kPPBatch) {
+            const int nq = P.aip - q0 < kPPBatch ? P.aip - q0 : kPPBatch;
+            double np_[kPPBatch][3], cs[kPPBatch];
+#pragma unroll
+            for (int i = 0; i < kPPBatch; i++) {
+              const int q = i < nq ? q0 + i : q0;
+              double qx = P.q[q][0] * rot[0] + P.q[q][1] * rot[3] + P.q[q][2] * rot[6];
+              double qy = P.q[q][0] * rot[1] + P.q[q][1] * rot[4] + P.q[q][2] * rot[7];
+              double qz = P.q[q][0] * rot[2] + P.q[q][1] * rot[5] + P.q[q][2] * rot[8];
+              np_[i][0] = re[0] + (qx * orr - od[0]);
+              np_[i][1] = re[1] + (qy * orr - od[1]);
+              np_[i][2] = re[2] + (qz * orr - od[2]);
+              double nd[3] = {np_[i][0] - ap[0], np_[i][1] - ap[1], np_[i][2] - ap[2]};
+              double nr = sqrt(nd[0] * nd[0] + nd[1] * nd[1] + nd[2] * nd[2]);
+              cs[i] = (nd[0] * od[0] + nd[1] * od[1] + nd[2] * od[2]) / (nr * orr);
             }
-            double uq = jastrow_e_val(S, ee, np_, pos, lane);
-            double ratio = Rq * exp(uq - u_old) * P.w[q];
-            double ts = 0;
-            for (int l = 0; l < P.L - 1; l++) ts += (2 * l + 1) * v[l] * legendre(cs, l);
-            const double vxx = ts * ratio;
-            nl += vxx;
-            if (vxx < 0.0) nneg -= vxx;
+            double Rq[kPPBatch], uq[kPPBatch];
+            if constexpr (MD) {
+              for (int i = 0; i < nq; i++) {
+                const double phm = lcao_val_n(S, s, S.nmo[s], np_[i], chi, flist, lane);
+                __syncthreads();
+                if (lane < S.nmo[s]) phq[lane] = phm;
+                __syncthreads();
+                md_contract<1>(S, md_inv, s, k, phq, nsp, bw, md_den, &Rq[i], lane);
+              }
+            } else {
+              double phi[kPPBatch];
+              lcao_val_multi<kPPBatch>(S, s, np_, nq, chi, flist, phi, lane);
+#pragma unroll
+              for (int i = 0; i < kPPBatch; i++) Rq[i] = i < nq ? wsum(ivk * phi[i]) : 0.0;
+            }
+            jastrow_e_val_multi<kPPBatch>(S, ee, np_, nq, pos, uq, lane);
+            for (int i = 0; i < nq; i++) {
+              double ratio = Rq[i] * exp(uq[i] - u_old) * P.w[q0 + i];
+              double ts = 0;
+              for (int l = 0; l < P.L - 1; l++) ts += (2 * l + 1) * v[l] * legendre(cs[i], l);
+              const double vxx = ts * ratio;
+              nl += vxx;
+              if (vxx < 0.0) nneg -= vxx;
+            }
           }
           if (lane == 0) {
             vnl += nl;

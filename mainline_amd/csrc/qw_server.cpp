@@ -1,19 +1,26 @@
 // Serving loop above the engine's C ABI: jobs (deck + method + walkers + seed) arrive at any
 // time and are admitted in arrival order while their walker state fits the HBM budget and
-// fewer than max_active run.  An admitted job's engine is made (deck read, tables and walkers
-// on the device) by the admission thread; the job then joins the batch of its precision, and
-// the stepping thread advances every batch member by one step per qw_batch_step, in launches
-// shared by all of them (continuous batching: jobs join and leave between steps, never
-// waiting for the others).  Jobs a batch cannot hold (BF16 precisions, several determinants,
-// T-moves, more than 128 electrons of a spin in mixed precision) run on their own engine and
-// host thread as before.  When a job finishes its memory is returned and the next job is
+// fewer than max_active run.  A deck is parsed once per server (a template engine per deck
+// file and precision, checked against the file's size and modification time); an admitted
+// job's engine is cloned from it and its walkers placed by a pool of set-up threads, so the
+// set-up of several jobs overlaps and never holds up admission or stepping.  The job then
+// joins the batch of its precision, and the stepping thread advances every batch member by
+// one step per qw_batch_step, in launches shared by all of them (continuous batching: jobs
+// join and leave between steps, never waiting for the others).  Jobs a batch cannot hold
+// (BF16 precisions, several determinants, T-moves, more than 128 electrons of a spin in mixed
+// precision) run on their own engine and host thread.  A finished job's engine is released
+// by the set-up pool; its memory returns to the budget when it is freed, and the next job is
 // admitted.
 //
 // A job's numbers depend only on its deck, seed and method (every random number is keyed by
-// the job's seed, walker, step and electron, and a batch member's results are bitwise those
-// of its engine alone), so a job gives the same block records alone or next to others.
+// the job's seed, walker, step and electron, a cloned engine is the engine the deck gives,
+// and a batch member's results are bitwise those of its engine alone), so a job gives the
+// same block records alone or next to others.
+#include <sys/stat.h>
+
 #include <chrono>
 #include <condition_variable>
+#include <functional>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -36,11 +43,14 @@ struct Job {
   qw_status status = QW_OK;
   std::string err;
   bool done = false;
-  qw_engine* eng = nullptr;  // created at admission
+  qw_engine* eng = nullptr;   // cloned from the template after admission
+  qw_engine* tmpl = nullptr;  // the deck's template engine (owned by the server)
   unsigned long long bytes = 0;
   int member = 0;  // batch member id while it runs in a batch
   std::chrono::steady_clock::time_point t_run;
 };
+
+constexpr int kSetupThreads = 4;  // engine clones and walker placement running at once
 
 double now_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -85,16 +95,113 @@ struct qw_server {
   std::map<int, qw_batch*> batches;  // by precision (stepping thread only)
   std::map<int, std::map<int, Job*>> members;  // precision -> member id -> job
 
-  void finish(Job& j, qw_status st, const std::string& err) {
+  // template engines by (deck path, precision), with the file's size and modification time
+  struct Tmpl {
+    qw_engine* e = nullptr;
+    long long size = -1, mtime = -1;
+  };
+  std::map<std::pair<std::string, int>, Tmpl> templates;  // admission thread (and destroy)
+  // set-up pool: engine clones + walker placement, engine releases
+  std::mutex tmu;
+  std::condition_variable tcv;
+  std::deque<std::function<void()>> tasks;
+  std::vector<std::thread> pool;
+  bool pool_stop = false;
+
+  void submit_task(std::function<void()> f) {
+    std::lock_guard<std::mutex> g(tmu);
+    tasks.push_back(std::move(f));
+    tcv.notify_one();
+  }
+  void pool_loop() {
+    std::unique_lock<std::mutex> lk(tmu);
+    while (true) {
+      tcv.wait(lk, [&] { return pool_stop || !tasks.empty(); });
+      if (tasks.empty()) return;
+      std::function<void()> f = std::move(tasks.front());
+      tasks.pop_front();
+      lk.unlock();
+      f();
+      lk.lock();
+    }
+  }
+
+  // The job's result is final (waiters wake); its memory is still held until release().
+  void complete(Job& j, qw_status st, const std::string& err) {
     std::lock_guard<std::mutex> g(mu);
     j.status = st;
     j.err = err;
     j.stats[2] = now_since(t0);
     j.stats[3] = j.stats[2] - j.stats[0];
     j.done = true;
+    cv.notify_all();
+  }
+  // The job's engine has been freed: its bytes and slot go back to admission.
+  void release(Job& j) {
+    std::lock_guard<std::mutex> g(mu);
     used -= j.bytes;
     j.bytes = 0;
     active--;
+    cv.notify_all();
+  }
+  void finish(Job& j, qw_status st, const std::string& err) {
+    complete(j, st, err);
+    release(j);
+  }
+  // Completes the job now and frees its engine on the set-up pool (freeing device memory
+  // synchronises the device, which the stepping thread should not wait for).
+  void finish_later(Job* j, qw_status st, const std::string& err) {
+    qw_engine* e = j->eng;
+    j->eng = nullptr;
+    complete(*j, st, err);
+    submit_task([this, j, e] {
+      qw_engine_destroy(e);
+      release(*j);
+    });
+  }
+
+  // The deck's template engine for this precision (parsed on first use, and again when the
+  // deck file changed); the error message in *err on failure.
+  qw_status get_template(const Job& j, qw_engine** out, std::string* err) {
+    struct stat st {};
+    const long long size = stat(j.deck.c_str(), &st) == 0 ? (long long)st.st_size : -1;
+    const long long mtime = size >= 0 ? (long long)st.st_mtime : -1;
+    Tmpl& t = templates[{j.deck, j.desc.precision}];
+    if (t.e && (t.size != size || t.mtime != mtime)) {
+      qw_engine_destroy(t.e);
+      t.e = nullptr;
+    }
+    if (!t.e) {
+      qw_engine_opts eo{j.desc.precision, opts.device, 0, 0};
+      qw_status s = qw_engine_create(j.deck.c_str(), &eo, &t.e);
+      if (s != QW_OK) {
+        *err = qw_last_error(nullptr);
+        t.e = nullptr;
+        return s;
+      }
+      t.size = size;
+      t.mtime = mtime;
+    }
+    *out = t.e;
+    return QW_OK;
+  }
+
+  // Set-up of an admitted job on the pool: its engine, then its walkers; then it is ready.
+  void setup_job(Job* j) {
+    const double ts = now_since(t0);
+    qw_engine_opts eo{j->desc.precision, opts.device, j->desc.seed, 0};
+    qw_engine* e = nullptr;
+    qw_status st = qw_engine_clone(j->tmpl, &eo, &e);
+    if (st != QW_OK) {
+      finish(*j, st, qw_last_error(nullptr));
+      return;
+    }
+    j->eng = e;
+    const bool ok = place_walkers(j);
+    j->stats[8] += now_since(t0) - ts;
+    if (!ok) return;
+    std::lock_guard<std::mutex> g(mu);
+    ready.push_back(j);
     cv.notify_all();
   }
 
@@ -170,12 +277,7 @@ struct qw_server {
         if (st != QW_OK) {
           // a failed step fails every job of that batch
           std::string err = qw_batch_error(p.second);
-          for (auto& m : members[p.first]) {
-            Job* j = m.second;
-            qw_engine_destroy(j->eng);
-            j->eng = nullptr;
-            finish(*j, st, err);
-          }
+          for (auto& m : members[p.first]) finish_later(m.second, st, err);
           members[p.first].clear();
           qw_batch_destroy(p.second);
           p.second = nullptr;
@@ -190,9 +292,7 @@ struct qw_server {
           j->stats[4] = rs[1];
           j->stats[5] = now_since(t0) - j->stats[1];
           j->stats[6] = j->stats[5] > 0 ? rs[1] / j->stats[5] : 0.0;
-          qw_engine_destroy(j->eng);
-          j->eng = nullptr;
-          finish(*j, ls, ls == QW_OK ? std::string() : std::string(qw_batch_error(p.second)));
+          finish_later(j, ls, ls == QW_OK ? std::string() : std::string(qw_batch_error(p.second)));
         }
       }
       for (auto it = batches.begin(); it != batches.end();)
@@ -218,6 +318,7 @@ struct qw_server {
       if (qw_batch_join(b, j->eng, (int)runs.size(), runs.data(), &mid) == QW_OK) {
         members[prec][mid] = j;
         j->member = mid;
+        j->stats[9] = 1;
         return;
       }
     }
@@ -232,18 +333,22 @@ struct qw_server {
       cv.wait(lk, [&] { return stopping || !pending.empty(); });
       if (pending.empty() && stopping) return;
       Job* j = jobs[pending.front()].get();
-      if (!j->eng) {
+      if (!j->tmpl) {
         lk.unlock();
-        qw_engine_opts eo{j->desc.precision, opts.device, j->desc.seed, 0};
-        qw_engine* e = nullptr;
-        qw_status st = qw_engine_create(j->deck.c_str(), &eo, &e);
+        const double ts = now_since(t0);
+        qw_engine* t = nullptr;
+        std::string err;
+        qw_status st = get_template(*j, &t, &err);
         unsigned long long b = 0;
-        if (st == QW_OK) st = qw_engine_walker_bytes(e, j->desc.nwalkers, &b);
+        if (st == QW_OK) st = qw_engine_walker_bytes(t, j->desc.nwalkers, &b);
         lk.lock();
-        if (st != QW_OK) {
-          j->status = st;
-          j->err = qw_last_error(e);
-          if (e) qw_engine_destroy(e);
+        j->stats[8] = now_since(t0) - ts;
+        if (st != QW_OK || b > opts.hbm_budget) {
+          // a deck that cannot be read, or a job that can never fit: fail it rather than
+          // block the queue
+          j->status = st != QW_OK ? st : QW_ERR_NOMEM;
+          j->err = st != QW_OK ? err : "job needs more device memory than the server's budget";
+          j->stats[7] = (double)b;
           j->stats[2] = now_since(t0);
           j->stats[3] = j->stats[2] - j->stats[0];
           j->done = true;
@@ -251,22 +356,8 @@ struct qw_server {
           cv.notify_all();
           continue;
         }
-        j->eng = e;
+        j->tmpl = t;
         j->stats[7] = (double)b;
-        if (b > opts.hbm_budget) {  // can never fit: fail it rather than block the queue
-          lk.unlock();
-          qw_engine_destroy(e);
-          lk.lock();
-          j->eng = nullptr;
-          j->status = QW_ERR_NOMEM;
-          j->err = "job needs more device memory than the server's budget";
-          j->stats[2] = now_since(t0);
-          j->stats[3] = j->stats[2] - j->stats[0];
-          j->done = true;
-          pending.pop_front();
-          cv.notify_all();
-          continue;
-        }
       }
       const unsigned long long b = (unsigned long long)j->stats[7];
       if (used + b > opts.hbm_budget || active >= opts.max_active) {
@@ -278,13 +369,7 @@ struct qw_server {
       active++;
       j->stats[1] = now_since(t0);
       pending.pop_front();
-      lk.unlock();
-      const bool ok = place_walkers(j);
-      lk.lock();
-      if (ok) {
-        ready.push_back(j);
-        cv.notify_all();
-      }
+      submit_task([this, j] { setup_job(j); });
     }
   }
 };
@@ -298,6 +383,7 @@ extern "C" qw_status qw_server_create(const qw_server_opts* opts, qw_server** ou
   if (s->opts.hbm_budget == 0) s->opts.hbm_budget = 64ull << 30;
   if (s->opts.max_active <= 0) s->opts.max_active = 32;
   if (s->opts.max_active > 64) s->opts.max_active = 64;  // members of one batch
+  for (int i = 0; i < kSetupThreads; i++) s->pool.emplace_back([s] { s->pool_loop(); });
   s->sched = std::thread([s] { s->schedule(); });
   s->stepper = std::thread([s] { s->step_loop(); });
   *out = s;
@@ -349,7 +435,15 @@ extern "C" void qw_server_destroy(qw_server* s) {
   s->sched.join();
   s->stepper.join();
   for (auto& t : s->workers) t.join();
+  {
+    std::lock_guard<std::mutex> g(s->tmu);
+    s->pool_stop = true;
+    s->tcv.notify_all();
+  }
+  for (auto& t : s->pool) t.join();
   for (auto& p : s->batches)
     if (p.second) qw_batch_destroy(p.second);
+  for (auto& t : s->templates)
+    if (t.second.e) qw_engine_destroy(t.second.e);
   delete s;
 }

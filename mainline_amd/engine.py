@@ -41,7 +41,7 @@ class JobDesc(ctypes.Structure):
                 ("equil_steps", ctypes.c_int), ("method", MethodDesc)]
 
 
-JOB_STATS = 8
+JOB_STATS = 10
 
 
 class QwError(RuntimeError):
@@ -59,6 +59,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         L.qw_engine_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(EngineOpts), ctypes.POINTER(ctypes.c_void_p)]
         L.qw_engine_destroy.argtypes = [ctypes.c_void_p]
+        L.qw_engine_clone.argtypes = [ctypes.c_void_p, ctypes.POINTER(EngineOpts), ctypes.POINTER(ctypes.c_void_p)]
         L.qw_last_error.restype = ctypes.c_char_p
         L.qw_last_error.argtypes = [ctypes.c_void_p]
         L.qw_engine_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
@@ -115,7 +116,7 @@ def exported_symbols():
     """Names of the qw_* functions the loaded library exports (no device calls)."""
     L = lib()
     names = []
-    for n in ["qw_engine_create", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
+    for n in ["qw_engine_create", "qw_engine_clone", "qw_engine_destroy", "qw_last_error", "qw_engine_info", "qw_eval_fixed",
               "qw_eval_fixed_static", "qw_eval_fixed_static_max",
               "qw_method_read", "qw_method_count", "qw_method_read_at", "qw_walkers_set", "qw_walkers_get", "qw_walkers_state", "qw_walkers_random", "qw_run",
               "qw_kernel_times", "qw_engine_walker_bytes", "qw_engine_tuning", "qw_parm_count",
@@ -206,10 +207,14 @@ def method(kind, nconfig, nblock, nstep, timestep, ndecorr=2, feedback_interval=
 class Engine:
     """One walker engine on one device for the SYSTEM / PSEUDO / TRIALFUNC of a deck."""
 
-    def __init__(self, deck, precision="fp64", device=0, seed=20261202, refresh_interval=0):
+    def __init__(self, deck, precision="fp64", device=0, seed=20261202, refresh_interval=0, _from=None):
+        """_from: an Engine whose parsed deck is reused (qw_engine_clone) instead of reading `deck`."""
         h = ctypes.c_void_p()
         opts = EngineOpts(PREC[precision], device, seed, refresh_interval)
-        st = lib().qw_engine_create(deck.encode(), ctypes.byref(opts), ctypes.byref(h))
+        if _from is not None:
+            st = lib().qw_engine_clone(_from.h, ctypes.byref(opts), ctypes.byref(h))
+        else:
+            st = lib().qw_engine_create(deck.encode(), ctypes.byref(opts), ctypes.byref(h))
         if st:
             raise QwError(st, lib().qw_last_error(None).decode())
         self.h = h
@@ -218,6 +223,10 @@ class Engine:
         self.nup, self.ndn, self.natoms, self.nbasis, self.has_pp = list(info)[:5]
         self.ne = self.nup + self.ndn
         self.precision = precision
+
+    def clone(self, precision=None, device=0, seed=20261202, refresh_interval=0):
+        """A new engine for the same deck (not read again) with other options."""
+        return Engine(None, precision or self.precision, device, seed, refresh_interval, _from=self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -429,7 +438,8 @@ class Server:
 
     def wait(self, jid):
         """Block records of the job and its stats: submit, admitted, finished, latency (s),
-        electron moves, run seconds, electron-moves/s, device bytes."""
+        electron moves, run seconds, electron-moves/s, device bytes, set-up seconds, 1 = ran in
+        a batch."""
         eb = np.zeros((self.nblock[jid], BLOCK_FIELDS))
         stats = np.zeros(JOB_STATS)
         st = lib().qw_server_wait(self.h, jid, _p(eb), _p(stats))

@@ -112,3 +112,25 @@ def test_serving_workload_sizes_alone_and_in_the_mix(tmp_path):
     for (ne, m, prec, seed), a, (eb, st) in zip(jobs, alone, res):
         print("mix", ne, prec, eb[:, 0], "latency %.3f s" % st[3], flush=True)
         assert np.array_equal(a, eb), (ne, prec)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed_fp32"])
+def test_cloned_engine_equals_a_fresh_engine(prec):
+    """qw_engine_clone (the server's per-deck template) gives the engine qw_engine_create gives:
+    the same DMC block records bit for bit, also when the clone changes precision and seed."""
+    deck, mol = synth_deck("pp16")
+    src = E.Engine(deck, "fp64", seed=1)
+    print("clone", prec, flush=True)
+    a = E.Engine(deck, prec, seed=77)
+    b = src.clone(precision=prec, seed=77)
+    m = E.method("DMC", 64, nblock=2, nstep=4, timestep=0.01)
+    pos = a.random_walkers(64, seed=3)
+    a.set_walkers(pos)
+    b.set_walkers(pos)
+    ra, rb = a.run(m), b.run(m)
+    assert np.array_equal(ra["eblock"], rb["eblock"])
+    pa, wa = a.get_walkers(64)
+    pb, wb = b.get_walkers(64)
+    assert np.array_equal(pa, pb) and np.array_equal(wa, wb)
+    for e in (a, b, src):
+        e.close()
