@@ -4,9 +4,11 @@
 // file and precision, checked against the file's size and modification time); an admitted
 // job's engine is cloned from it and its walkers placed by a pool of set-up threads, so the
 // set-up of several jobs overlaps and never holds up admission or stepping.  The job then
-// joins the batch of its precision, and the stepping thread advances every batch member by
-// one step per qw_batch_step, in launches shared by all of them (continuous batching: jobs
-// join and leave between steps, never waiting for the others).  Jobs a batch cannot hold
+// joins the batch of its precision in its size lane (at most 64 electrons of a spin, or
+// more), and the lane's stepping thread advances every batch member by one step per
+// qw_batch_step, in launches shared by all of them (continuous batching: jobs join and leave
+// between steps, never waiting for the others; the two lanes step independently on their own
+// streams, so small jobs are not held to the step time of large ones).  Jobs a batch cannot hold
 // (BF16 precisions, several determinants, T-moves, more than 128 electrons of a spin in mixed
 // precision) run on their own engine and host thread.  A finished job's engine is released
 // by the set-up pool; its memory returns to the budget when it is freed, and the next job is
@@ -18,6 +20,7 @@
 // same block records alone or next to others.
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <functional>
@@ -89,11 +92,17 @@ struct qw_server {
   int active = 0;
   long long next_id = 1;
   bool stopping = false;
-  std::thread sched, stepper;
-  // jobs whose engines are ready to join a batch, handed from the admission thread
-  std::deque<Job*> ready;
-  std::map<int, qw_batch*> batches;  // by precision (stepping thread only)
-  std::map<int, std::map<int, Job*>> members;  // precision -> member id -> job
+  std::thread sched;
+  // Two stepping threads, one for jobs with at most 64 electrons of a spin and one for larger
+  // ones: each advances its own batches (one per precision) on their own streams, so a small
+  // job's step never waits for a large job's, and the device runs both sets of launches.
+  struct Lane {
+    std::deque<Job*> ready;                      // set-up done, waiting to join (under mu)
+    std::map<int, qw_batch*> batches;            // by precision (this lane's thread only)
+    std::map<int, std::map<int, Job*>> members;  // precision -> member id -> job
+    std::thread th;
+  };
+  Lane lanes[2];
 
   // template engines by (deck path, precision), with the file's size and modification time
   struct Tmpl {
@@ -197,11 +206,14 @@ struct qw_server {
       return;
     }
     j->eng = e;
+    int info[6] = {0, 0, 0, 0, 0, 0};
+    qw_engine_info(e, info);
+    const int lane = std::max(info[0], info[1]) > 64 ? 1 : 0;
     const bool ok = place_walkers(j);
     j->stats[8] += now_since(t0) - ts;
     if (!ok) return;
     std::lock_guard<std::mutex> g(mu);
-    ready.push_back(j);
+    lanes[lane].ready.push_back(j);
     cv.notify_all();
   }
 
@@ -257,35 +269,36 @@ struct qw_server {
     finish(*j, st, err);
   }
 
-  // Stepping thread: new jobs join their precision's batch, every batch takes one step, and
-  // finished members leave with their results.
-  void step_loop() {
+  // Stepping thread of lane L: new jobs join their precision's batch, every batch takes one
+  // step, and finished members leave with their results.
+  void step_loop(Lane& L) {
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
       size_t live = 0;
-      for (auto& p : members) live += p.second.size();
-      cv.wait(lk, [&] { return !ready.empty() || live > 0 || (stopping && pending.empty()); });
+      for (auto& p : L.members) live += p.second.size();
+      cv.wait(lk, [&] { return !L.ready.empty() || live > 0 || (stopping && pending.empty()); });
       std::deque<Job*> take;
-      take.swap(ready);
+      take.swap(L.ready);
       if (take.empty() && live == 0 && stopping) return;
       lk.unlock();
-      for (Job* j : take) join(j);
-      for (auto& p : batches) {
-        if (members[p.first].empty()) continue;
+      for (Job* j : take) join(L, j);
+      for (auto& p : L.batches) {
+        auto& mem = L.members[p.first];
+        if (mem.empty()) continue;
         int fin[64], nf = 0;
         qw_status st = qw_batch_step(p.second, fin, &nf);
         if (st != QW_OK) {
           // a failed step fails every job of that batch
           std::string err = qw_batch_error(p.second);
-          for (auto& m : members[p.first]) finish_later(m.second, st, err);
-          members[p.first].clear();
+          for (auto& m : mem) finish_later(m.second, st, err);
+          mem.clear();
           qw_batch_destroy(p.second);
           p.second = nullptr;
           continue;
         }
         for (int k = 0; k < nf; k++) {
-          Job* j = members[p.first][fin[k]];
-          members[p.first].erase(fin[k]);
+          Job* j = mem[fin[k]];
+          mem.erase(fin[k]);
           j->eblock.assign((size_t)j->desc.method.nblock * QW_BLOCK_FIELDS, 0.0);
           double rs[4] = {0, 0, 0, 0};
           qw_status ls = qw_batch_leave(p.second, fin[k], j->eblock.data(), rs);
@@ -295,28 +308,28 @@ struct qw_server {
           finish_later(j, ls, ls == QW_OK ? std::string() : std::string(qw_batch_error(p.second)));
         }
       }
-      for (auto it = batches.begin(); it != batches.end();)
-        if (!it->second) it = batches.erase(it);
+      for (auto it = L.batches.begin(); it != L.batches.end();)
+        if (!it->second) it = L.batches.erase(it);
         else ++it;
       lk.lock();
     }
   }
 
-  // Joins the batch of the job's precision, or runs the job alone when no batch holds it.
-  void join(Job* j) {
+  // Joins the lane's batch of the job's precision, or runs the job alone when no batch holds it.
+  void join(Lane& L, Job* j) {
     const int prec = j->desc.precision;
     qw_batch* b = nullptr;
-    auto it = batches.find(prec);
-    if (it != batches.end()) {
+    auto it = L.batches.find(prec);
+    if (it != L.batches.end()) {
       b = it->second;
     } else if (qw_batch_open(opts.device, prec, 0, &b) == QW_OK) {
-      batches[prec] = b;
+      L.batches[prec] = b;
     }
     if (b) {
       const auto runs = job_runs(j->desc);
       int mid = 0;
       if (qw_batch_join(b, j->eng, (int)runs.size(), runs.data(), &mid) == QW_OK) {
-        members[prec][mid] = j;
+        L.members[prec][mid] = j;
         j->member = mid;
         j->stats[9] = 1;
         return;
@@ -385,7 +398,7 @@ extern "C" qw_status qw_server_create(const qw_server_opts* opts, qw_server** ou
   if (s->opts.max_active > 64) s->opts.max_active = 64;  // members of one batch
   for (int i = 0; i < kSetupThreads; i++) s->pool.emplace_back([s] { s->pool_loop(); });
   s->sched = std::thread([s] { s->schedule(); });
-  s->stepper = std::thread([s] { s->step_loop(); });
+  for (auto& L : s->lanes) L.th = std::thread([s, &L] { s->step_loop(L); });
   *out = s;
   return QW_OK;
 }
@@ -433,7 +446,7 @@ extern "C" void qw_server_destroy(qw_server* s) {
     s->cv.notify_all();
   }
   s->sched.join();
-  s->stepper.join();
+  for (auto& L : s->lanes) L.th.join();
   for (auto& t : s->workers) t.join();
   {
     std::lock_guard<std::mutex> g(s->tmu);
@@ -441,8 +454,9 @@ extern "C" void qw_server_destroy(qw_server* s) {
     s->tcv.notify_all();
   }
   for (auto& t : s->pool) t.join();
-  for (auto& p : s->batches)
-    if (p.second) qw_batch_destroy(p.second);
+  for (auto& L : s->lanes)
+    for (auto& p : L.batches)
+      if (p.second) qw_batch_destroy(p.second);
   for (auto& t : s->templates)
     if (t.second.e) qw_engine_destroy(t.second.e);
   delete s;
