@@ -447,6 +447,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--accuracy-blocks", type=int, default=10, help="0 disables the energy-agreement leg")
     ap.add_argument("--serving-jobs", type=int, default=100, help="0 disables the serving leg")
+    ap.add_argument("--sustainable-rate", type=float, default=2.0,
+                    help="arrival rate (jobs/s) of the second serving stream")
+    ap.add_argument("--sustainable-jobs", type=int, default=40, help="jobs of the second serving stream (0: none)")
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batching leg (0 disables it)")
     ap.add_argument("--refresh", type=int, default=32,
                     help="mixed precision: FP64 inverse recompute every this many moves")
@@ -531,6 +534,11 @@ def main():
     if a.serving_jobs > 0:
         for p in dict.fromkeys(["fp64", head]):
             serving[p] = serving_leg(p, a.serving_jobs, 20.0, 4242, tmp)
+            if a.sustainable_jobs > 0:
+                # a rate the device keeps up with (the 20/s stream queues: its latency is mostly
+                # waiting for admission)
+                serving[p + "_at_%g_per_s" % a.sustainable_rate] = serving_leg(
+                    p, a.sustainable_jobs, a.sustainable_rate, 4343, tmp)
     cpu = None
     if not a.no_cpu:
         cpu = (cpu_baseline_reference if os.access(REF_QWALK, os.X_OK) else cpu_baseline)(deck, mol, a.cpu_seconds)

@@ -27,6 +27,9 @@ Cases:
   n2_cio    n2_ci with OPTIMIZE_DET (the determinant weights 2.. as parameters)
   n2_csfj   n2_csf with OPTIMIZE_DET and the Jastrow (CSF coefficients, then Jastrow parameters)
   h4, pp16, ae32   the seeded synthetic molecules of mainline_amd/synth.py (4, 64, 128 e)
+  ae32_vmc  16 configurations of the 128-electron molecule from a seeded FP64 VMC
+            equilibration (the oracle's VMC: 30 steps at timestep 0.3 from Gaussian clouds),
+            the kind of configuration the mixed-precision error report uses
   fg3       three atoms with every supported shell symmetry (S P 6D 7F 7F_crystal 10F 9G
             9G_pyscf 15G), 12 electrons
 
@@ -262,6 +265,19 @@ def main():
             np.savez_compressed(os.path.join(OUT, mol.name + ".npz"), deck="synth:" + mol.name,
                                 orb_sha256=hashlib.sha256(orb).hexdigest(), **res)
             print(mol.name, res["pos"].shape)
+        if not only or "ae32_vmc" in only:
+            from oracle.oracle import Oracle
+            mol = synth.ae32()
+            d = os.path.join(tmp, "ae32_vmc")
+            deck = os.path.basename(mol.write(d))
+            orb = open(os.path.join(d, mol.name + ".orb"), "rb").read()
+            start = synth.random_configs(mol, 16, seed=202)
+            pos = Oracle(os.path.join(d, deck)).run(start, "VMC", nblock=1, nstep=30, timestep=0.3, seed=20261230,
+                                                   nthreads=8)["pos"]
+            res = run_ref(d, deck, pos)
+            np.savez_compressed(os.path.join(OUT, "ae32_vmc.npz"), deck="synth:" + mol.name,
+                                orb_sha256=hashlib.sha256(orb).hexdigest(), **res)
+            print("ae32_vmc", res["pos"].shape)
 
 
 if __name__ == "__main__":

@@ -52,6 +52,33 @@ def test_mixed_eval_errors_against_fp64_oracle(name, prec):
         assert v < t[k], (k, v)
 
 
+@pytest.mark.parametrize("prec", ["mixed_fp32", "mixed_bf16x6"])
+def test_mixed_errors_on_vmc_configurations_against_the_reference(prec):
+    """Per-quantity errors of the mixed modes on 16 configurations of the 128-electron molecule
+    from a seeded FP64 VMC equilibration, against the reference program's own FP64 values
+    (tests/golden/ae32_vmc.npz): ln|Psi| and E_L absolute, the gradient as max error over max
+    magnitude per configuration.  The bounds are those the bench's 10^4-configuration report
+    stays within (max |dE_L| 2.4e-2 Ha, median 3e-5 Ha for mixed FP32)."""
+    from tests._util import golden
+    g, deck = golden("ae32_vmc")
+    eng = E.Engine(deck, prec)
+    a = eng.eval(g["pos"])
+    K = len(g["logpsi"])
+    el_ref = g["kinetic"] + g["vloc"]
+    el = a["eparts"][:, 0] + a["eparts"][:, 1]
+    err = dict(
+        logpsi=np.max(np.abs(a["logpsi"] - g["logpsi"])),
+        grad=max(np.max(np.abs(a["grad"][k] - g["grad"][k])) / np.max(np.abs(g["grad"][k])) for k in range(K)),
+        el_max=np.max(np.abs(el - el_ref)),
+        el_median=np.median(np.abs(el - el_ref)),
+    )
+    _say("vmc-configs", prec, {k: "%.2e" % v for k, v in err.items()})
+    assert np.array_equal(a["sign"], g["sign"])
+    bound = dict(logpsi=1e-4, grad=1e-4, el_max=5e-2, el_median=1e-3)
+    for k, v in err.items():
+        assert v < bound[k], (k, v)
+
+
 def _dmc_blocks(deck, mol, prec, nw=256, nblock=4):
     pos = synth.random_configs(mol, nw, seed=8)
     eng = E.Engine(deck, prec, seed=777)

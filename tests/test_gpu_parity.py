@@ -12,7 +12,7 @@ from tests._util import golden, relerr, scaled_err, synth_deck
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["h_atom", "h2_jast", "n2_pp", "n2_cen", "h4", "pp16", "ae32", "fg3"]
+CASES = ["h_atom", "h2_jast", "n2_pp", "n2_cen", "h4", "pp16", "ae32", "ae32_vmc", "fg3"]
 TOL = 1e-10  # FP64 path: relative tolerance against the reference's FP64 evaluation
 
 

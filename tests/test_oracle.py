@@ -9,7 +9,7 @@ import pytest
 from oracle import oracle as O
 from tests._util import golden, relerr, scaled_err, synth_deck
 
-CASES = ["h_atom", "h2_jast", "n2_pp", "n2_ci", "n2_cij", "n2_csf", "h4", "pp16", "ae32", "fg3"]
+CASES = ["h_atom", "h2_jast", "n2_pp", "n2_ci", "n2_cij", "n2_csf", "h4", "pp16", "ae32", "ae32_vmc", "fg3"]
 TOL = 1e-10
 
 
