@@ -9,19 +9,17 @@
 //                       Jastrow arithmetic
 //   k_mo_rows_big<T>    basis + LCAO of every electron (src/orbitals/MO_matrix_cutoff.h:480-560)
 //   k_inv_grid<G>       FP64 pivoted Gauss-Jordan over a G x G thread grid (up to 128 per spin)
-//   k_lu_gather / k_lu_logdet / k_lu_scatter
-//                       around the batched LU and inverse of the device solver library (129 to
-//                       256 per spin): the Slater matrices are gathered in column-major order so
-//                       that the inverse comes back as the transposed inverse the kernels use
-//                       (src/program_utils/MatrixAlgebrac.cpp:316-421)
+//   k_inv_panel         the same Gauss-Jordan for 129 to 256 per spin, panel by panel with
+//                       the working matrix in device memory (src/program_utils/MatrixAlgebrac.cpp:
+//                       108-224,316-421)
 #pragma once
 #include "qw_kernels.hpp"
 
 namespace qw {
 
 constexpr int kBigMaxSpin = 256;
-// Electrons of a spin up to which the inverse recompute is the hand-written k_inv_grid<16>
-// (above: the device solver library's batched LU).
+// Electrons of a spin up to which the inverse recompute is k_inv_grid<16> (the whole matrix in
+// registers); above, k_inv_panel.
 constexpr int kGridMaxSpin = 128;
 
 // Spline table element type that goes with an inverse / MO cache element type.
@@ -521,49 +519,190 @@ __global__ __launch_bounds__(G * G) void k_inv_grid(DevSys S, const TIN* mo, TI*
   k_inv_grid_body<G>(S, mo, inv_out, logdet_out, sgn_out, w, s, (int)threadIdx.x, L);
 }
 
-// Column-major Slater matrices of spin s for the batched solver: work[w][c][r] = phi_c(r_r)
-// (value rows of the MO cache), so that the solver's inverse, read row-major, is the
-// transposed inverse inv(k, i) = A^-1(i, k).
-template <typename TIN>
-__global__ __launch_bounds__(256) void k_lu_gather(DevSys S, const TIN* mo, double* work, int nw, int s) {
-  const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
-  const size_t per = (size_t)n * n, i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)nw * per) return;
-  const size_t w = i / per, rem = i % per, c = rem / n, r = rem % n;
-  work[i] = (double)mo[((w * S.ne + off + r) * 5) * nsp + c];
-}
-
-// log|det| and sign from the LU factors (diagonal of U, 1-based pivots); singular matrices
-// (solver info > 0) get the reference's -1e201.
-static __global__ __launch_bounds__(256) void k_lu_logdet(const double* work, const int* ipiv, const int* info, double* logdet,
-                                                   double* sgn, int nw, int s, int n) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+// ---------------------------------------------------------------- k_inv_panel
+// FP64 pivoted Gauss-Jordan for 129..256 electrons of a spin, one 256-thread block per matrix,
+// the working matrix W (n x n, row-major) in device memory and the current panel of kPanel
+// columns in LDS.  The elimination is that of k_inv_blk / k_inv_grid (rows left in place,
+// pivot = the unpivoted row with the largest vv[i]*|a[i][j]|, the last row on ties, multipliers
+// a[i][j] * (1/pivot), column j replaced by -multiplier or 1), grouped by panels:
+//   1. the panel's columns are eliminated in LDS (thread = row), recording the multipliers
+//      g^(k) (k < kPanel) and pivot rows p_k;
+//   2. every other column c (thread = column) takes the panel's steps at once: the pivot-row
+//      values the steps used follow from the values before the panel by a triangular solve,
+//      r_k = W[p_k][c] - sum_(m<k) g^(m)[p_k] r_m, and W[:,c] -= sum_k g^(k) r_k,
+// so W is read and written once per panel instead of once per column
+// (src/program_utils/MatrixAlgebrac.cpp:108-224,316-421 for the pivot rule).  The log|det|,
+// sign and the transposed inverse follow from the pivots as in k_inv_blk.
+constexpr int kPanel = 16;
+struct InvPanelLds {
+  double pan[kBigMaxSpin][kPanel];  // the panel's columns
+  double g[kBigMaxSpin][kPanel];    // multipliers of the panel's steps
+  double vv[kBigMaxSpin];           // 1 / max |row|
+  double pvs[kBigMaxSpin];
+  int pl[kBigMaxSpin];
+  int qinv[kBigMaxSpin];
+  int used[kBigMaxSpin];
+  double wbest[4];
+  int wrow[4];
+  double redd[4];
+  int redi[4];
+};
+template <typename TIN, typename TI>
+__global__ __launch_bounds__(256) void k_inv_panel(DevSys S, const TIN* mo, double* work, TI* inv_out,
+                                                   double* logdet_out, double* sgn_out, int nw, int s) {
+  __shared__ InvPanelLds L;
+  const int w = (int)blockIdx.x, t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
   if (w >= nw) return;
-  if (info[w] != 0) {
-    logdet[2 * w + s] = -1e201;
-    sgn[2 * w + s] = 1;
+  const int n = s ? S.ndn : S.nup, off = s ? S.nup : 0, nsp = S.nsp;
+  TI* out = inv_out + ((size_t)w * 2 + s) * nsp * nsp;
+  double* W = work + (size_t)w * n * n;
+  if (n == 0) {
+    if (t == 0) {
+      logdet_out[2 * w + s] = 0;
+      sgn_out[2 * w + s] = 1;
+    }
     return;
   }
-  const double* a = work + (size_t)w * n * n;
-  double ld = 0;
-  int odd = 0;
+  // W = A (A[i][c] = phi_c(r_i), the MO value rows); row scales; thread t = column t
   for (int i = 0; i < n; i++) {
-    const double d = a[(size_t)i * n + i];
-    ld += log(fabs(d));
-    odd ^= (d < 0) ^ (ipiv[(size_t)w * n + i] != i + 1);
+    if (t < n) W[(size_t)i * n + t] = (double)mo[((size_t)w * S.ne + off + i) * 5 * nsp + t];
   }
-  logdet[2 * w + s] = ld;
-  sgn[2 * w + s] = odd ? -1.0 : 1.0;
+  bool zero_row = false;
+  if (t < n) {
+    const TIN* rp = mo + ((size_t)w * S.ne + off + t) * 5 * nsp;
+    double mx = 0;
+    for (int c = 0; c < n; c++) mx = fmax(mx, fabs((double)rp[c]));
+    L.vv[t] = mx == 0.0 ? 0.0 : 1.0 / mx;
+    zero_row = mx == 0.0;
+  }
+  L.used[t] = t >= n;
+  if (lane == 0) L.redi[wv] = wave_any(zero_row) ? 1 : 0;
+  __syncthreads();
+  bool singular = (L.redi[0] | L.redi[1] | L.redi[2] | L.redi[3]) != 0;
+  for (int j0 = 0; j0 < n && !singular; j0 += kPanel) {
+    const int jb = n - j0 < kPanel ? n - j0 : kPanel;
+    // 1. the panel into LDS (16 consecutive doubles of a row per 16 threads)
+    for (int x = t; x < n * kPanel; x += 256) {
+      const int i = x / kPanel, k = x % kPanel;
+      L.pan[i][k] = k < jb ? W[(size_t)i * n + j0 + k] : 0.0;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < jb; kk++) {
+      const int j = j0 + kk;
+      // pivot: thread = row; per-wavefront winner, then the highest row among the best
+      double cd = -1.0;
+      if (t < n && !L.used[t]) {
+        cd = L.vv[t] * fabs(L.pan[t][kk]);
+        if (!(cd >= 0.0)) cd = -1.0;
+      }
+      const int Lw = wave_argmax_last(cd, lane);
+      if (lane == 0) {
+        L.wbest[wv] = Lw < 0 ? -2.0 : wave_bcast(cd, Lw);
+        L.wrow[wv] = Lw < 0 ? -1 : 64 * wv + Lw;
+      }
+      __syncthreads();
+      double gb = -2.0;
+      int p = -1;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (L.wbest[q] >= gb) {
+          gb = L.wbest[q];
+          p = L.wrow[q];
+        }
+      const double pv = p < 0 ? 0.0 : L.pan[p][kk];
+      if (p < 0 || !(gb >= 0.0) || pv == 0.0) {
+        singular = true;
+        break;
+      }
+      // every thread holds the pivot before the pivot row's column kk is rewritten
+      __syncthreads();
+      if (t == 0) {
+        L.pl[j] = p;
+        L.pvs[j] = pv;
+      }
+      const double rpv = 1.0 / pv;
+      double g = 0.0;
+      if (t < n && t != p) {
+        g = L.pan[t][kk] * rpv;
+        for (int c = 0; c < jb; c++)
+          if (c != kk) L.pan[t][c] = fma(-g, L.pan[p][c], L.pan[t][c]);
+      }
+      if (t < n) {
+        L.g[t][kk] = g;
+        L.pan[t][kk] = t == p ? 1.0 : -g;
+      }
+      if (t == p) L.used[t] = 1;
+      __syncthreads();
+    }
+    if (singular) break;
+    // the panel back to W
+    for (int x = t; x < n * kPanel; x += 256) {
+      const int i = x / kPanel, k = x % kPanel;
+      if (k < jb) W[(size_t)i * n + j0 + k] = L.pan[i][k];
+    }
+    // 2. every other column: thread = column
+    if (t < n && (t < j0 || t >= j0 + jb)) {
+      double r[kPanel];
+#pragma unroll
+      for (int k = 0; k < kPanel; k++) {
+        r[k] = 0.0;
+        if (k < jb) {
+          const int pk = L.pl[j0 + k];
+          double v = W[(size_t)pk * n + t];
+#pragma unroll
+          for (int m = 0; m < kPanel; m++)
+            if (m < k) v = fma(-L.g[pk][m], r[m], v);
+          r[k] = v;
+        }
+      }
+      for (int i = 0; i < n; i++) {
+        double v = W[(size_t)i * n + t];
+#pragma unroll
+        for (int k = 0; k < kPanel; k++)
+          if (k < jb) v = fma(-L.g[i][k], r[k], v);
+        W[(size_t)i * n + t] = v;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+  if (singular) {
+    if (t == 0) {
+      logdet_out[2 * w + s] = -1e201;
+      sgn_out[2 * w + s] = 1;
+    }
+    for (int k = t; k < n * nsp; k += 256) out[k] = (TI)0;
+    return;
+  }
+  for (int j = t; j < n; j += 256) L.qinv[L.pl[j]] = j;
+  // log|det|, negative pivots and the inversion count of the pivot sequence
+  double lg = 0;
+  int neg = 0, inv_cnt = 0;
+  if (t < n) {
+    lg = log(fabs(L.pvs[t]));
+    neg = L.pvs[t] < 0;
+    const int pj = L.pl[t];
+    for (int m = t + 1; m < n; m++) inv_cnt += L.pl[m] < pj;
+  }
+  lg = wsum(lg);
+  const int par = wave_isum(neg + inv_cnt);
+  if (lane == 0) {
+    L.redd[wv] = lg;
+    L.redi[wv] = par;
+  }
+  __syncthreads();
+  if (t == 0) {
+    logdet_out[2 * w + s] = ((L.redd[0] + L.redd[1]) + L.redd[2]) + L.redd[3];
+    sgn_out[2 * w + s] = ((L.redi[0] + L.redi[1] + L.redi[2] + L.redi[3]) & 1) ? -1.0 : 1.0;
+  }
+  // row p_q holds row q of the inverse of the row-permuted matrix scaled by 1/pivot_q: its
+  // column m goes to out[p_m][q]; thread = m, reading row p_q contiguously
+  for (int q = 0; q < n; q++) {
+    const int pq = L.pl[q];
+    const double sc = 1.0 / L.pvs[q];
+    if (t < n) out[(size_t)L.pl[t] * nsp + q] = (TI)(W[(size_t)pq * n + t] * sc);
+  }
 }
 
-template <typename TI>
-__global__ __launch_bounds__(256) void k_lu_scatter(DevSys S, const double* work, const int* info, TI* inv, int nw,
-                                                    int s) {
-  const int n = s ? S.ndn : S.nup, nsp = S.nsp;
-  const size_t per = (size_t)n * n, i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)nw * per) return;
-  const size_t w = i / per, rem = i % per, k = rem / n, c = rem % n;
-  inv[((w * 2 + s) * nsp + k) * nsp + c] = info[w] ? (TI)0 : (TI)work[i];
-}
 
 }  // namespace qw

@@ -480,7 +480,7 @@ extern "C" qw_status qw_eval_fixed_static(qw_engine* e, int K, const double* pos
   if (!e || K <= 0 || !pos || !logpsi || !sign || !grad || !lap || !eparts) return QW_ERR_ARG;
   if (pp_mode == 1 && !pp_rand && e->S.has_pp) return QW_ERR_ARG;
   if (std::max(e->T.nup, e->T.ndn) > kGridMaxSpin) {
-    e->err = "qw_eval_fixed_static: more than 128 electrons of one spin need the solver library's workspace";
+    e->err = "qw_eval_fixed_static: more than 128 electrons of one spin need working matrices outside the fixed array";
     return QW_ERR_UNSUPPORTED;
   }
   DevWalkers V;

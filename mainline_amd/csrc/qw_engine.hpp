@@ -16,7 +16,6 @@
 #include <string>
 #include <vector>
 
-#include <rocsolver/rocsolver.h>
 
 #include "../../include/qwalk_amd.h"
 #include "qw_big.hpp"
@@ -127,13 +126,10 @@ struct qw_engine {
   std::vector<double> ktime;
   std::vector<int> kcount;
 
-  // more than 64 electrons of one spin: 256-thread walker blocks, solver-library refresh
+  // more than 64 electrons of one spin: one wavefront per walker, inverse in device memory
   bool big = false;
   bool md = false;  // several determinants (FP64, k_inv_md / k_sweep_md / k_local<double, true>)
-  rocblas_handle rb = nullptr;
-  double* luw = nullptr;
-  int* lupiv = nullptr;
-  int* luinfo = nullptr;
+  double* luw = nullptr;  // k_inv_panel working matrices (more than 128 of a spin)
   int lucap = 0;
   // T-moves: per-(atom, electron) negative nonlocal sums of the last local energy
   double* tneg = nullptr;
@@ -144,7 +140,6 @@ struct qw_engine {
   ~qw_engine() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
-    if (rb) (void)rocblas_destroy_handle(rb);
     if (gx) (void)hipGraphExecDestroy(gx);
     {
       std::lock_guard<std::mutex> g(g_devmem_mu);
@@ -407,7 +402,7 @@ struct qw_engine {
     b += P * (8 + 4) + 2 * (N / 2 + 1) * 4;
     if (big) {
       const unsigned long long nm = std::max(T.nup, T.ndn);
-      b += N * nm * nm * 8 + N * nm * 4 + N * 4;
+      if (nm > kGridMaxSpin) b += N * nm * nm * 8;  // k_inv_panel working matrices
     }
     if (md) b += N * 2 * S.ndet * (S.nsig * S.nsig + 2) * 8;
     return b;
@@ -464,9 +459,7 @@ struct qw_engine {
       for (void* p : walker_allocs) (void)hipFree(p);
     }
     walker_allocs.clear();
-    luw = nullptr;  // the solver and T-move buffers are walker allocations too
-    lupiv = nullptr;
-    luinfo = nullptr;
+    luw = nullptr;  // the inversion and T-move buffers are walker allocations too
     lucap = 0;
     tneg = nullptr;
     tnegcap = 0;
@@ -715,20 +708,14 @@ struct qw_engine {
     launch("k_local", [&] { hipLaunchKernelGGL(k_local<double>, dim3(nw), dim3(64), lds_local(), stream, S, W, A); });
   }
   // ---- more than 64 electrons per spin
+  // working matrices of k_inv_panel (more than 128 electrons of a spin)
   void lu_buffers() {
     const int n = std::max(T.nup, T.ndn);
     if (lucap >= nw && luw) return;
-    if (!rb) {
-      std::lock_guard<std::mutex> g(g_devmem_mu);
-      if (rocblas_create_handle(&rb) != rocblas_status_success) throw DevErr("rocblas_create_handle failed");
-      if (rocblas_set_stream(rb, stream) != rocblas_status_success) throw DevErr("rocblas_set_stream failed");
-    }
     luw = walloc<double>((size_t)nw * n * n);
-    lupiv = walloc<int>((size_t)nw * n);
-    luinfo = walloc<int>(nw);
     lucap = nw;
   }
-  // FP64 LU + inverse of spin s from the MO value rows (FP64 or FP32 cache)
+  // FP64 inverse, log|det| and sign of spin s from the MO value rows (FP64 or FP32 cache)
   void lu_refresh(int s) {
     const int n = s ? T.ndn : T.nup;
     if (n == 0) return;
@@ -745,30 +732,13 @@ struct qw_engine {
       return;
     }
     lu_buffers();
-    const size_t tot = (size_t)nw * n * n;
-    const unsigned g = (unsigned)((tot + 255) / 256);
-    // the solver library may allocate its workspace: serialised with other engines' graph
-    // recording like every device allocation
-    std::lock_guard<std::mutex> lk(g_devmem_mu);
     launch("k_inv", [&] {
       if (precision == QW_PREC_FP64)
-        hipLaunchKernelGGL(k_lu_gather<double>, dim3(g), dim3(256), 0, stream, S, (const double*)W.mo, luw, nw, s);
+        hipLaunchKernelGGL((k_inv_panel<double, double>), dim3(nw), dim3(256), 0, stream, S, (const double*)W.mo,
+                           luw, W.inv, W.logdet, W.sgn, nw, s);
       else
-        hipLaunchKernelGGL(k_lu_gather<float>, dim3(g), dim3(256), 0, stream, S, (const float*)W.mof, luw, nw, s);
-      if (rocsolver_dgetrf_strided_batched(rb, n, n, luw, n, (rocblas_stride)n * n, lupiv, n, luinfo, nw) !=
-          rocblas_status_success)
-        throw DevErr("rocsolver_dgetrf_strided_batched failed");
-      hipLaunchKernelGGL(k_lu_logdet, dim3((nw + 255) / 256), dim3(256), 0, stream, (const double*)luw,
-                         (const int*)lupiv, (const int*)luinfo, W.logdet, W.sgn, nw, s, n);
-      if (rocsolver_dgetri_strided_batched(rb, n, luw, n, (rocblas_stride)n * n, lupiv, n, luinfo, nw) !=
-          rocblas_status_success)
-        throw DevErr("rocsolver_dgetri_strided_batched failed");
-      if (precision == QW_PREC_FP64)
-        hipLaunchKernelGGL(k_lu_scatter<double>, dim3(g), dim3(256), 0, stream, S, (const double*)luw,
-                           (const int*)luinfo, W.inv, nw, s);
-      else
-        hipLaunchKernelGGL(k_lu_scatter<float>, dim3(g), dim3(256), 0, stream, S, (const double*)luw,
-                           (const int*)luinfo, W.invf, nw, s);
+        hipLaunchKernelGGL((k_inv_panel<float, float>), dim3(nw), dim3(256), 0, stream, S, (const float*)W.mof, luw,
+                           W.invf, W.logdet, W.sgn, nw, s);
     });
   }
   void big_refresh_all() {

@@ -13,7 +13,9 @@ _DECK_DIR = os.path.join(tempfile.gettempdir(), "qw_test_decks_%d" % os.getuid()
 
 SYNTH = {"h4": synth.h4, "pp16": synth.pp16, "ae32": synth.ae32, "fg3": synth.fg_molecule,
          # 80 + 80 electrons: the block-per-walker path for more than 64 electrons of a spin
-         "ae40": lambda: synth.z4_molecule(40, 20261240, name="ae40")}
+         "ae40": lambda: synth.z4_molecule(40, 20261240, name="ae40"),
+         # 144 + 144 electrons: above 128 of a spin the recompute is the panel Gauss-Jordan
+         "ae72": lambda: synth.z4_molecule(72, 20261272, name="ae72")}
 
 
 def synth_deck(name):
