@@ -433,9 +433,10 @@ __device__ __forceinline__ void k_inv_grid_body(const DevSys& S, const TIN* mo, 
         for (int c2 = 0; c2 < 8; c2++) prow[8 * bj + c2] = a[c2][pr];
       }
       const double rpv = 1.0 / pv;
+      const bool pblk = bi == pb;  // row 8bi + r is the pivot row p = 8pb + pr
       if (mine) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) gw8[8 * bi + r] = (8 * bi + r == p) ? 0.0 : a[c][r] * rpv;
+        for (int r = 0; r < 8; r++) gw8[8 * bi + r] = (pblk && r == pr) ? 0.0 : a[c][r] * rpv;
       }
       __syncthreads();
       double pr8[8], g8[8];
@@ -450,9 +451,9 @@ __device__ __forceinline__ void k_inv_grid_body(const DevSys& S, const TIN* mo, 
         for (int r = 0; r < 8; r++) a[c2][r] = fma(-g8[r], pr8[c2], a[c2][r]);
       if (mine) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) a[c][r] = (8 * bi + r == p) ? 1.0 : -g8[r];
+        for (int r = 0; r < 8; r++) a[c][r] = (pblk && r == pr) ? 1.0 : -g8[r];
       }
-      if (bi == pb) pivm |= 1 << pr;
+      if (pblk) pivm |= 1 << pr;
     }
   }
   if (singular) {

@@ -392,15 +392,32 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
   dv8 a[8];
   double vv[8];
   bool zero_row = false;
+  // a lane's 8 entries of a row are contiguous: 16-byte loads when every MO row starts on a
+  // 16-byte boundary (row length 5 * nsp values; entries past n are read, then replaced)
+  const bool vec16 = sizeof(TIN) == 4 ? (nsp & 3) == 0 : (nsp & 1) == 0;
 #pragma unroll
   for (int r = 0; r < 8; r++) {
     const int row = 8 * bi + r;
-    const TIN* rp = mo + ((size_t)w * S.ne + off + (row < n ? row : 0)) * 5 * nsp;
+    const TIN* rp = mo + ((size_t)w * S.ne + off + (row < n ? row : 0)) * 5 * nsp + 8 * bj;
+    TIN vin[8];
+    if (vec16) {
+      typedef TIN tv16 __attribute__((ext_vector_type(16 / sizeof(TIN))));
+      constexpr int per = 16 / sizeof(TIN);
+#pragma unroll
+      for (int h = 0; h < 8 / per; h++) {
+        const tv16 u = *(const tv16*)(rp + per * h);
+#pragma unroll
+        for (int i = 0; i < per; i++) vin[per * h + i] = u[i];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; c++) vin[c] = 8 * bj + c < n ? rp[c] : (TIN)0;
+    }
     double mx = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
       const int col = 8 * bj + c;
-      const double v = (row < n && col < n) ? (double)rp[col] : (row == col ? 1.0 : 0.0);
+      const double v = (row < n && col < n) ? (double)vin[c] : (row == col ? 1.0 : 0.0);
       a[c][r] = v;
       mx = fmax(mx, fabs(v));
     }
@@ -469,8 +486,10 @@ __device__ inline void k_inv_blk_body(const DevSys& S, const TIN* mo, TI* inv_ou
 #pragma unroll
         for (int r = 0; r < 8; r++) a[c2][r] = fma(-g8[r], pr8[c2], a[c2][r]);
       if (mine) {
+        // row 8bi + r is the pivot row p = 8pb + pr (pr wave-uniform, so no per-row lane constants)
+        const bool pblk = bi == pb;
 #pragma unroll
-        for (int r = 0; r < 8; r++) a[c][r] = (8 * bi + r == p) ? 1.0 : -g8[r];
+        for (int r = 0; r < 8; r++) a[c][r] = (pblk && r == pr) ? 1.0 : -g8[r];
       }
       used |= lane == p;
     }
