@@ -512,6 +512,8 @@ struct qw_engine {
   size_t lds_local() const {
     return (7 * (size_t)S.ne + 5 * (size_t)S.chunk_nf) * sizeof(double) + (size_t)S.chunk_nf * sizeof(int);
   }
+  // without a pseudopotential: positions, determinant gradients and laplacians only
+  size_t lds_local_nopp() const { return 7 * (size_t)S.ne * sizeof(double); }
 
   void run_mo_rows() {
     launch("k_mo_rows", [&] {
@@ -705,7 +707,10 @@ struct qw_engine {
       mixed_local(*this, A);
       return;
     }
-    launch("k_local", [&] { hipLaunchKernelGGL(k_local<double>, dim3(nw), dim3(64), lds_local(), stream, S, W, A); });
+    launch("k_local", [&] {
+      if (S.has_pp) hipLaunchKernelGGL(k_local<double>, dim3(nw), dim3(64), lds_local(), stream, S, W, A);
+      else hipLaunchKernelGGL((k_local<double, false, false>), dim3(nw), dim3(64), lds_local_nopp(), stream, S, W, A);
+    });
   }
   // ---- more than 64 electrons per spin
   // working matrices of k_inv_panel (more than 128 electrons of a spin)

@@ -164,6 +164,7 @@ void mixed_sweep(qw_engine& e, const SweepArgs& A0, int depth) {
 
 void mixed_local(qw_engine& e, const LocalArgs& A) {
   e.launch("k_local", [&] {
-    hipLaunchKernelGGL(k_local<float>, dim3(e.nw), dim3(64), e.lds_local(), e.stream, e.S, e.W, A);
+    if (e.S.has_pp) hipLaunchKernelGGL(k_local<float>, dim3(e.nw), dim3(64), e.lds_local(), e.stream, e.S, e.W, A);
+    else hipLaunchKernelGGL((k_local<float, false, false>), dim3(e.nw), dim3(64), e.lds_local_nopp(), e.stream, e.S, e.W, A);
   });
 }
