@@ -184,7 +184,8 @@ __global__ __launch_bounds__(256, 2) void k_inv_batch(const BatchSlot* __restric
 }
 
 // Local energy and DMC weight of every walker of every slot (one 64-thread block each).
-template <int PREC>
+// PP = false when no slot's system has a pseudopotential (the leaner kernel).
+template <int PREC, bool PP>
 __global__ __launch_bounds__(64) void k_local_batch(const BatchSlot* __restrict__ slots, int nslot) {
   extern __shared__ double ldsl[];
   const QW_CONST BatchSlot* sl = cptr(slots);
@@ -199,9 +200,9 @@ __global__ __launch_bounds__(64) void k_local_batch(const BatchSlot* __restrict_
   A.seed = B.seed;
   A.ctl = B.ctl;
   if constexpr (PREC == 0)
-    local_body<double>(gref(B.S), gref(B.W), A, (int)blockIdx.x - b0, (int)threadIdx.x, ldsl);
+    local_body<double, false, PP>(gref(B.S), gref(B.W), A, (int)blockIdx.x - b0, (int)threadIdx.x, ldsl);
   else
-    local_body<float>(gref(B.S), gref(B.W), A, (int)blockIdx.x - b0, (int)threadIdx.x, ldsl);
+    local_body<float, false, PP>(gref(B.S), gref(B.W), A, (int)blockIdx.x - b0, (int)threadIdx.x, ldsl);
 }
 
 // Trial energy, effective timestep and the pair comb of each branching slot (one block each):
@@ -281,6 +282,8 @@ struct qw_batch {
   double ktime[4] = {0, 0, 0, 0};
   int kcount[4] = {0, 0, 0, 0};
   size_t lds_sweep = 0, lds_local = 0;
+  bool any_pp = false;  // some slot's system has a pseudopotential
+  bool g_pp = false;    // any_pp of the recorded graph
   bool capturing = false;
   bool step_mode = false;  // made by qw_batch_open: members join and leave between steps
 
@@ -368,6 +371,7 @@ struct qw_batch {
   void build_slots(bool step_mode) {
     slots.clear();
     lds_sweep = lds_local = 0;
+    any_pp = false;
     std::vector<size_t> order(mem.size());
     for (size_t k = 0; k < order.size(); k++) order[k] = k;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return mem[a].e->S.ne > mem[b].e->S.ne; });
@@ -418,7 +422,8 @@ struct qw_batch {
         B.nr = std::max(e.T.nup, e.T.ndn) <= 2 * kWave ? 2 : 4;
         lds_sweep = std::max(lds_sweep, (size_t)B.wpb * per);
       }
-      lds_local = std::max(lds_local, e.lds_local());
+      any_pp |= e.S.has_pp != 0;
+      lds_local = std::max(lds_local, e.S.has_pp ? e.lds_local() : e.lds_local_nopp());
       slots.push_back(B);
     }
     if ((int)slots.size() > slot_cap) {
@@ -493,10 +498,13 @@ struct qw_batch {
     unsigned nwt = 0;
     for (auto& B : slots) nwt += (unsigned)B.W.nw;
     launch(2, [&] {
-      if (precision == QW_PREC_FP64)
-        hipLaunchKernelGGL(k_local_batch<0>, dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
-      else
-        hipLaunchKernelGGL(k_local_batch<1>, dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
+      if (precision == QW_PREC_FP64) {
+        if (any_pp) hipLaunchKernelGGL((k_local_batch<0, true>), dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
+        else hipLaunchKernelGGL((k_local_batch<0, false>), dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
+      } else {
+        if (any_pp) hipLaunchKernelGGL((k_local_batch<1, true>), dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
+        else hipLaunchKernelGGL((k_local_batch<1, false>), dim3(nwt), dim3(64), lds_local, stream, d_slots, nslot());
+      }
       hipLaunchKernelGGL(k_step_batch, dim3(1), dim3(64), 0, stream, d_slots, nslot());
     });
   }
@@ -519,7 +527,7 @@ struct qw_batch {
     branch(np);
   }
   void interval_graph(int np) {
-    if (!gx || g_np != np) {
+    if (!gx || g_np != np || g_pp != any_pp) {
       if (gx) {
         HIPCHK(hipGraphExecDestroy(gx));
         gx = nullptr;
@@ -541,6 +549,7 @@ struct qw_batch {
       HIPCHK(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0));
       HIPCHK(hipGraphDestroy(g));
       g_np = np;
+      g_pp = any_pp;
     }
     HIPCHK(hipGraphLaunch(gx, stream));
   }
