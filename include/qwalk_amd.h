@@ -65,8 +65,9 @@ typedef struct qw_engine_opts {
   int device;               /* HIP device ordinal */
   unsigned long long seed;  /* key of the counter-based random streams */
   int refresh_interval;     /* mixed precision: FP64 LU recompute every this many moves
-                               (0 = the default, 32; a value >= the electrons of a spin =
-                               once per sweep of that spin channel) */
+                               (0 = the default: 32 up to 64 electrons of a spin, above that
+                               half a sweep rounded up to a multiple of 32; a value >= the
+                               electrons of a spin = once per sweep of that spin channel) */
 } qw_engine_opts;
 
 typedef struct qw_method_desc {
@@ -232,10 +233,11 @@ qw_status qw_engine_walker_bytes(const qw_engine* e, int nw, unsigned long long*
 qw_status qw_engine_tuning(const qw_engine* e, int* out4);
 
 /* ---------------------------------------------------------------- batches of systems
- * Several engines (systems of different sizes on one device, all fp64 or all mixed_fp32 with
- * one recompute interval) run DMC together: each step is one sweep launch per spin channel
- * (per FP64 recompute interval on the mixed path, each followed by one recompute launch)
- * covering every system, one local-energy launch, and per feedback interval one branching
+ * Several engines (systems of different sizes on one device, all fp64 or all mixed_fp32) run
+ * DMC together: each step is one sweep launch per spin channel (on the mixed path per launch
+ * interval, the largest common divisor of the engines' recompute intervals, each followed by
+ * one recompute launch covering the systems whose own interval ends there) covering every
+ * system, one local-energy launch, and per feedback interval one branching
  * and one copy launch, recorded once as a HIP graph and replayed.  The blocks of a launch read
  * their system from a slot table; the LCAO coefficient loads in flight are tuned online per
  * size bucket (systems with the same register tile).  Each engine keeps its own walkers,
@@ -265,6 +267,8 @@ qw_status qw_batch_run(qw_batch* b, const qw_method_desc* m, double* eblock, dou
  * shares the launches with.  finished[*nfinished] (up to 64) = members whose runs ended in
  * this step; qw_batch_leave then gives the last run's eblock[nblock][QW_BLOCK_FIELDS] and
  * stats[4] (as qw_run) and removes the member (the caller then owns the engine again). */
+/* refresh_interval (mixed path): the batch's launch interval (0 = 32); a member's own recompute
+ * interval must be a multiple of it or cover a whole sweep of its spin channels. */
 qw_status qw_batch_open(int device, int precision, int refresh_interval, qw_batch** out);
 qw_status qw_batch_join(qw_batch* b, qw_engine* e, int nruns, const qw_method_desc* runs, int* member);
 qw_status qw_batch_step(qw_batch* b, int* finished, int* nfinished);

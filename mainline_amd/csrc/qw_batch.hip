@@ -7,6 +7,8 @@
 // the engine's own launches, so a system's walkers, weights and block energies in a batch
 // are bitwise those of the same engine run alone.  No reference counterpart: QWalk runs one
 // system per process (src/program_utils/ooqmc.cpp:148-170, src/method/Dmc_method.cpp:328-567).
+#include <numeric>
+
 #include "qw_engine.hpp"
 
 namespace qw {
@@ -38,6 +40,10 @@ struct BatchSlot {
   // block-per-... layout: walkers (wavefronts) per 256-thread block, the LDS bytes of each,
   // rows per lane of the one-wavefront sweep above 64 electrons of a spin (2 or 4)
   int wpb, lds_w, nr;
+  // the slot's FP64 recompute interval (mixed path): a multiple of the batch's launch interval
+  // R, or at least n (once per sweep); the recompute launch after moves [k0, k0 + R) covers
+  // the slot when k0 + R is a multiple of rf or ends the sweep
+  int rf;
 };
 
 constexpr int kBatchWpb = 4;  // walkers (wavefronts) per 256-thread block for <= 64 per spin
@@ -56,14 +62,21 @@ __device__ inline int batch_blocks_inv(const QW_CONST BatchSlot& B) {
   return B.ns ? (B.W.nw + kBatchWpb - 1) / kBatchWpb : B.W.nw;
 }
 
+// Whether slot B recomputes spin s after the moves [k0, k0 + R) (host and device).
+__host__ __device__ inline bool batch_recomputes(int rf, int n, int k0, int R) {
+  return (k0 + R) % rf == 0 || k0 + R >= n;
+}
+
 // The slot of block `blk` among the slots of method depth `depth` with a sweep `sw` and moves
-// k0.. of spin s, and the slot's first block (*b0).  Uniform over the block; nslot is small.
+// k0.. of spin s (INV: and a recompute after the R moves from k0), and the slot's first block
+// (*b0).  Uniform over the block; nslot is small.
 template <bool INV = false>
 __device__ inline int batch_find(const QW_CONST BatchSlot* sl, int nslot, int s, int k0, int depth, int sw, int blk,
-                                 int* b0) {
+                                 int* b0, int R = 0) {
   int acc = 0;
   for (int i = 0; i < nslot; i++) {
     if (sl[i].n[s] <= k0 || sl[i].depth != depth || sw >= sl[i].ndecorr) continue;
+    if (INV && !batch_recomputes(sl[i].rf, sl[i].n[s], k0, R)) continue;
     const int nb = INV ? batch_blocks_inv(sl[i]) : batch_blocks(sl[i]);
     if (blk < acc + nb) {
       *b0 = acc;
@@ -154,11 +167,11 @@ union InvBatchLds {
   InvGridLds<16> grid;
 };
 __global__ __launch_bounds__(256, 2) void k_inv_batch(const BatchSlot* __restrict__ slots, int nslot, int s, int k0,
-                                                   int depth, int sw) {
+                                                   int depth, int sw, int R) {
   __shared__ InvBatchLds L;
   const QW_CONST BatchSlot* sl = cptr(slots);
   int b0 = 0;
-  const int i = batch_find<true>(sl, nslot, s, k0, depth, sw, (int)blockIdx.x, &b0);
+  const int i = batch_find<true>(sl, nslot, s, k0, depth, sw, (int)blockIdx.x, &b0, R);
   if (i < 0) return;
   const QW_CONST BatchSlot& B = sl[i];
   const int t = (int)threadIdx.x, blk = (int)blockIdx.x - b0;
@@ -339,10 +352,14 @@ struct qw_batch {
     timed.clear();
   }
 
-  // Whether an engine can be a member (same device, precision and recompute interval).
+  // Whether an engine can be a member (same device and precision, and a recompute interval
+  // that is a multiple of the batch's launch interval or a whole sweep).
   std::string admissible(const qw_engine* e) const {
-    if (e->device != device || e->precision != precision || e->refresh != refresh)
-      return "the engines of a batch share one device, precision and recompute interval";
+    const bool rf_ok = precision == QW_PREC_FP64 || e->refresh % refresh == 0 ||
+                       e->refresh >= std::max(e->T.nup, e->T.ndn);
+    if (e->device != device || e->precision != precision || !rf_ok)
+      return "the engines of a batch share one device and precision, and their recompute intervals are multiples "
+             "of the batch's";
     if (precision != QW_PREC_FP64 && precision != QW_PREC_MIXED_FP32) return "batches run in fp64 or mixed_fp32";
     if (e->md) return "several determinants are not batched";
     if (e->big && std::max(e->T.nup, e->T.ndn) > kGridMaxSpin && precision != QW_PREC_FP64)
@@ -406,6 +423,7 @@ struct qw_batch {
       B.pp_mode = m.pp_stochastic ? 1 : 0;
       B.tau = m.timestep;
       B.branch_np = (step_mode && m.method == 1 && mb.ip + 1 == mb.np) ? mb.np : 0;
+      B.rf = std::max(e.refresh, refresh);
       if (B.ns) {
         const size_t per =
             precision == QW_PREC_FP64
@@ -466,11 +484,11 @@ struct qw_batch {
   void recompute(int s, int k0, int depth, int sw) {
     unsigned g = 0;
     for (auto& B : slots)
-      if (B.n[s] > k0 && B.depth == depth && sw < B.ndecorr)
+      if (B.n[s] > k0 && B.depth == depth && sw < B.ndecorr && batch_recomputes(B.rf, B.n[s], k0, refresh))
         g += B.ns ? (unsigned)((B.W.nw + kBatchWpb - 1) / kBatchWpb) : (unsigned)B.W.nw;
     if (!g) return;
     launch(1, [&] {
-      hipLaunchKernelGGL(k_inv_batch, dim3(g), dim3(256), 0, stream, d_slots, nslot(), s, k0, depth, sw);
+      hipLaunchKernelGGL(k_inv_batch, dim3(g), dim3(256), 0, stream, d_slots, nslot(), s, k0, depth, sw, refresh);
     });
   }
   // the moves of one step of every slot of method depth DEPTH: its sweeps of both spin
@@ -722,8 +740,17 @@ extern "C" qw_status qw_batch_create(int n, qw_engine* const* engines, qw_batch*
   }
   for (int i = 0; i < n; i++)
     if (!engines[i]) return QW_ERR_ARG;
+  // launch interval: the largest common divisor of the members' recompute intervals (a
+  // member that recomputes once per sweep does not constrain it)
+  int R = 0, nmax = 1;
+  for (int i = 0; i < n; i++) {
+    const int ns = std::max(engines[i]->T.nup, engines[i]->T.ndn);
+    nmax = std::max(nmax, ns);
+    if (engines[i]->refresh < ns) R = std::gcd(R, engines[i]->refresh);
+  }
+  if (R == 0) R = nmax;
   qw_batch* b = nullptr;
-  qw_status st = batch_new(engines[0]->device, engines[0]->precision, engines[0]->refresh, &b);
+  qw_status st = batch_new(engines[0]->device, engines[0]->precision, R, &b);
   if (st != QW_OK) return st;
   for (int i = 0; i < n; i++) {
     const std::string why = b->admissible(engines[i]);

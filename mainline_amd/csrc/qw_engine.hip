@@ -82,6 +82,7 @@ qw_status make_engine(SysTables T, const qw_engine_opts* opts, qw_engine** out) 
   }
   e->T = std::move(T);
   const int nmax = std::max(e->T.nup, e->T.ndn);
+  if (!opts || opts->refresh_interval <= 0) e->refresh = default_refresh(nmax);
   if (nmax > kBigMaxSpin) {
     g_create_error = "more than 256 electrons of one spin are not supported by this engine build";
     return QW_ERR_UNSUPPORTED;

@@ -47,6 +47,11 @@ struct Unsupported : std::runtime_error {
 
 constexpr int kChunkNF = 640;  // basis functions per LDS tile
 constexpr int kRefreshDefault = 32;
+// Default FP64 recompute interval of the mixed path for n electrons of a spin: every 32 moves
+// up to 64; above, half a sweep rounded up to a multiple of 32.  The recompute costs O(n^3) a
+// spin and an update O(n^2) a move, so a fixed 32 would make the recompute the larger part
+// (eight recomputes per sweep at 256 a spin).
+inline int default_refresh(int n) { return n <= 64 ? kRefreshDefault : (n / 2 + 31) / 32 * 32; }
 
 inline int pick_ns(int n) {
   int ns = 4;
