@@ -304,6 +304,8 @@ __global__ __launch_bounds__(256) void k_mo_rows_big(DevSys S, const double* wpo
 // same order, so the two agree bit for bit on matrices both handle; G = 16 (256 threads)
 // inverts up to 128 electrons of a spin in registers.
 template <int G>
+// The pivot row and the multipliers are stored interleaved (entry 8i + b at index G*b + i), so
+// the eight values a thread reads in a step are spread over the LDS banks.
 struct InvGridLds {
   double prowL[2][8 * G];
   double gL[2][8 * G];
@@ -430,20 +432,20 @@ __device__ __forceinline__ void k_inv_grid_body(const DevSys& S, const TIN* mo, 
       double* gw8 = gL[j & 1];
       if (bi == pb) {
 #pragma unroll
-        for (int c2 = 0; c2 < 8; c2++) prow[8 * bj + c2] = a[c2][pr];
+        for (int c2 = 0; c2 < 8; c2++) prow[G * c2 + bj] = a[c2][pr];
       }
       const double rpv = 1.0 / pv;
       const bool pblk = bi == pb;  // row 8bi + r is the pivot row p = 8pb + pr
       if (mine) {
 #pragma unroll
-        for (int r = 0; r < 8; r++) gw8[8 * bi + r] = (pblk && r == pr) ? 0.0 : a[c][r] * rpv;
+        for (int r = 0; r < 8; r++) gw8[G * r + bi] = (pblk && r == pr) ? 0.0 : a[c][r] * rpv;
       }
       __syncthreads();
       double pr8[8], g8[8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        pr8[k] = prow[8 * bj + k];
-        g8[k] = gw8[8 * bi + k];
+        pr8[k] = prow[G * k + bj];
+        g8[k] = gw8[G * k + bi];
       }
 #pragma unroll
       for (int c2 = 0; c2 < 8; c2++)
@@ -535,9 +537,13 @@ __global__ __launch_bounds__(G * G) void k_inv_grid(DevSys S, const TIN* mo, TI*
 // (src/program_utils/MatrixAlgebrac.cpp:108-224,316-421 for the pivot rule).  The log|det|,
 // sign and the transposed inverse follow from the pivots as in k_inv_blk.
 constexpr int kPanel = 16;
+constexpr int kRowsPanel = 16;  // rows of W in flight per thread in the whole-matrix passes
+// Rows of the panel and multiplier tiles are padded to kPanel + 1 doubles: a thread reads and
+// writes its own row, and with a 16-double stride every lane of a wavefront would hit the
+// same LDS banks (a 64-way conflict on each such access).
 struct InvPanelLds {
-  double pan[kBigMaxSpin][kPanel];  // the panel's columns
-  double g[kBigMaxSpin][kPanel];    // multipliers of the panel's steps
+  double pan[kBigMaxSpin][kPanel + 1];  // the panel's columns
+  double g[kBigMaxSpin][kPanel + 1];    // multipliers of the panel's steps
   double vv[kBigMaxSpin];           // 1 / max |row|
   double pvs[kBigMaxSpin];
   int pl[kBigMaxSpin];
@@ -564,15 +570,35 @@ __global__ __launch_bounds__(256) void k_inv_panel(DevSys S, const TIN* mo, doub
     }
     return;
   }
-  // W = A (A[i][c] = phi_c(r_i), the MO value rows); row scales; thread t = column t
-  for (int i = 0; i < n; i++) {
-    if (t < n) W[(size_t)i * n + t] = (double)mo[((size_t)w * S.ne + off + i) * 5 * nsp + t];
+  // W = A (A[i][c] = phi_c(r_i), the MO value rows); row scales; thread t = column t.  W is
+  // read and written kRowsPanel rows at a time, every load of a group issued before its
+  // stores, so the device-memory latency is paid once per group and not once per row.
+  for (int i0 = 0; i0 < n; i0 += kRowsPanel) {
+    double v[kRowsPanel];
+#pragma unroll
+    for (int u = 0; u < kRowsPanel; u++)
+      v[u] = (t < n && i0 + u < n) ? (double)mo[((size_t)w * S.ne + off + i0 + u) * 5 * nsp + t] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kRowsPanel; u++)
+      if (t < n && i0 + u < n) W[(size_t)(i0 + u) * n + t] = v[u];
   }
   bool zero_row = false;
+  // a thread's own row is contiguous: 16-byte loads when every row starts 16-byte aligned
+  constexpr int per = 16 / sizeof(TIN);
+  const bool vec16 = (nsp * 5) % per == 0;
   if (t < n) {
     const TIN* rp = mo + ((size_t)w * S.ne + off + t) * 5 * nsp;
     double mx = 0;
-    for (int c = 0; c < n; c++) mx = fmax(mx, fabs((double)rp[c]));
+    int c = 0;
+    if (vec16) {
+      typedef TIN tv16 __attribute__((ext_vector_type(per)));
+      for (; c + per <= n; c += per) {
+        const tv16 u = *(const tv16*)(rp + c);
+#pragma unroll
+        for (int i = 0; i < per; i++) mx = fmax(mx, fabs((double)u[i]));
+      }
+    }
+    for (; c < n; c++) mx = fmax(mx, fabs((double)rp[c]));
     L.vv[t] = mx == 0.0 ? 0.0 : 1.0 / mx;
     zero_row = mx == 0.0;
   }
@@ -656,12 +682,20 @@ __global__ __launch_bounds__(256) void k_inv_panel(DevSys S, const TIN* mo, doub
           r[k] = v;
         }
       }
-      for (int i = 0; i < n; i++) {
-        double v = W[(size_t)i * n + t];
+      for (int i0 = 0; i0 < n; i0 += kRowsPanel) {
+        double v[kRowsPanel];
 #pragma unroll
-        for (int k = 0; k < kPanel; k++)
-          if (k < jb) v = fma(-L.g[i][k], r[k], v);
-        W[(size_t)i * n + t] = v;
+        for (int u = 0; u < kRowsPanel; u++) v[u] = i0 + u < n ? W[(size_t)(i0 + u) * n + t] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kRowsPanel; u++) {
+          const int i = i0 + u < n ? i0 + u : n - 1;
+#pragma unroll
+          for (int k = 0; k < kPanel; k++)
+            if (k < jb) v[u] = fma(-L.g[i][k], r[k], v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kRowsPanel; u++)
+          if (i0 + u < n) W[(size_t)(i0 + u) * n + t] = v[u];
       }
     }
     __threadfence_block();
@@ -698,10 +732,34 @@ __global__ __launch_bounds__(256) void k_inv_panel(DevSys S, const TIN* mo, doub
   }
   // row p_q holds row q of the inverse of the row-permuted matrix scaled by 1/pivot_q: its
   // column m goes to out[p_m][q]; thread = m, reading row p_q contiguously
-  for (int q = 0; q < n; q++) {
-    const int pq = L.pl[q];
-    const double sc = 1.0 / L.pvs[q];
-    if (t < n) out[(size_t)L.pl[t] * nsp + q] = (TI)(W[(size_t)pq * n + t] * sc);
+  // thread m = t writes its output row p_m, kRowsPanel consecutive entries at a time (16-byte
+  // stores when the output rows start 16-byte aligned)
+  const int pt = t < n ? L.pl[t] : 0;
+  constexpr int po = 16 / sizeof(TI);
+  const bool ovec = nsp % po == 0;
+  for (int q0 = 0; q0 < n; q0 += kRowsPanel) {
+    TI v[kRowsPanel];
+#pragma unroll
+    for (int u = 0; u < kRowsPanel; u++) {
+      const int q = q0 + u;
+      v[u] = (t < n && q < n) ? (TI)(W[(size_t)L.pl[q] * n + t] * (1.0 / L.pvs[q])) : (TI)0;
+    }
+    if (t >= n) continue;
+    TI* orow = out + (size_t)pt * nsp + q0;
+    if (ovec && q0 + kRowsPanel <= n) {
+      typedef TI ov16 __attribute__((ext_vector_type(po)));
+#pragma unroll
+      for (int h = 0; h < kRowsPanel / po; h++) {
+        ov16 x;
+#pragma unroll
+        for (int i = 0; i < po; i++) x[i] = v[po * h + i];
+        *(ov16*)(orow + po * h) = x;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kRowsPanel; u++)
+        if (q0 + u < n) orow[u] = v[u];
+    }
   }
 }
 
