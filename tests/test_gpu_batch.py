@@ -143,3 +143,49 @@ def test_step_mode_members_join_and_leave(prec, tmp_path_factory):
     _say("step mode", prec, "steps", steps)
     for k in range(len(picks)):
         assert np.array_equal(out[k], alone[k]), (picks[k], out[k][:, 0], alone[k][:, 0])
+
+
+def test_mixed_batch_recomputes_each_system_at_its_own_interval(tmp_path_factory):
+    """Mixed precision: engines with different FP64 recompute intervals share a batch.  The
+    launch interval is the common divisor of the intervals that fall inside a sweep (here 64,
+    from the 192-electron molecule's explicit 64; the smaller molecules recompute once per
+    sweep), each system is recomputed at its own interval, and every engine equals its run
+    alone bit for bit.  An open batch refuses an engine whose interval is not a multiple of
+    its launch interval."""
+    mols = _molecules(tmp_path_factory)
+    picks = [n for n, (_, mol, _) in mols.items() if mol.nelec in (16, 64, 192)]
+    m = E.method("DMC", 0, nblock=1, nstep=3, timestep=0.01)
+    solo, batched, starts = [], [], []
+    for name in picks:
+        deck, mol, nw = mols[name]
+        nw = max(16, nw // 16)
+        rf = 64 if mol.nelec == 192 else 0
+        a = E.Engine(deck, "mixed_fp32", seed=1702, refresh_interval=rf)
+        pos = synth.nonsingular_configs(mol, nw, lambda p: a.eval(p)["logpsi"], seed=64)
+        a.set_walkers(pos)
+        b = E.Engine(deck, "mixed_fp32", seed=1702, refresh_interval=rf)
+        b.set_walkers(pos)
+        solo.append(a)
+        batched.append(b)
+        starts.append((name, mol, nw))
+    _say("own intervals", [(n, mol.nelec, nw) for n, mol, nw in starts])
+    ref = [e.run(m)["eblock"] for e in solo]
+    B = E.Batch(batched)
+    res = B.run(m)
+    times, _ = B.kernel_times()
+    for k, (name, mol, nw) in enumerate(starts):
+        pa, wa = solo[k].get_walkers(nw)
+        pb, wb = batched[k].get_walkers(nw)
+        assert np.array_equal(pa, pb), name
+        assert np.array_equal(wa, wb), name
+        assert np.array_equal(ref[k], res["eblock"][k]), name
+    # 96 electrons of a spin in launches of 64 moves: two sweep launches per spin channel
+    assert times["sweep"][1] == 2 * 2 * m.nstep * m.nblock
+    B.close()
+    deck, mol, _ = mols[picks[-1]]
+    odd = E.Engine(deck, "mixed_fp32", refresh_interval=48)
+    odd.set_walkers(synth.random_configs(mol, 16, seed=3))
+    Bo = E.Batch(precision="mixed_fp32")
+    with pytest.raises(E.QwError):
+        Bo.join(odd, [m])
+    Bo.close()
