@@ -15,7 +15,10 @@ SYNTH = {"h4": synth.h4, "pp16": synth.pp16, "ae32": synth.ae32, "fg3": synth.fg
          # 80 + 80 electrons: the block-per-walker path for more than 64 electrons of a spin
          "ae40": lambda: synth.z4_molecule(40, 20261240, name="ae40"),
          # 144 + 144 electrons: above 128 of a spin the recompute is the panel Gauss-Jordan
-         "ae72": lambda: synth.z4_molecule(72, 20261272, name="ae72")}
+         "ae72": lambda: synth.z4_molecule(72, 20261272, name="ae72"),
+         # 130 + 130 electrons: a panel size that leaves a partial last panel and row group, and
+         # MO rows not 16-byte aligned (the recompute's scalar-access paths)
+         "ae65": lambda: synth.z4_molecule(65, 20261265, name="ae65")}
 
 
 def synth_deck(name):

@@ -85,6 +85,26 @@ def test_above_128_per_spin_eval_matches_oracle(prec, tol):
         assert v < tol, (k, v)
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("mixed_fp32", 2e-3)])
+def test_panel_recompute_partial_panel_matches_oracle(prec, tol):
+    """130 electrons of each spin: the last panel holds 2 columns, the last row group of the
+    whole-matrix passes 2 rows, and the MO rows (5 x 130 values) and output rows are not
+    16-byte aligned, so the recompute takes its scalar-access paths."""
+    deck, mol = synth_deck("ae65")
+    _say("ae65 eval", prec, mol.nelec, "electrons")
+    eng = E.Engine(deck, prec)
+    pos = synth.random_configs(mol, 2, seed=65)
+    a = eng.eval(pos)
+    b = O.Oracle(deck).eval(pos)
+    err = dict(logpsi=np.max(np.abs(a["logpsi"] - b["logpsi"]) / np.abs(b["logpsi"])),
+               grad=max(scaled_err(a["grad"][k], b["grad"][k]) for k in range(len(pos))),
+               kin=relerr(a["eparts"][:, 0], b["eparts"][:, 0]))
+    _say("ae65", prec, {k: "%.2e" % v for k, v in err.items()})
+    assert np.array_equal(a["sign"], b["sign"])
+    for k, v in err.items():
+        assert v < tol, (k, v)
+
+
 def test_above_128_per_spin_trajectory_matches_oracle():
     deck, mol = synth_deck("ae72")
     nw = 4
