@@ -12,6 +12,8 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 _DECK_DIR = os.path.join(tempfile.gettempdir(), "qw_test_decks_%d" % os.getuid())
 
 SYNTH = {"h4": synth.h4, "pp16": synth.pp16, "ae32": synth.ae32, "fg3": synth.fg_molecule,
+         # 34 + 34 electrons: the 8x8-block recompute with FP32 MO rows not 16-byte aligned
+         "ae17": lambda: synth.z4_molecule(17, 20261217, name="ae17"),
          # 80 + 80 electrons: the block-per-walker path for more than 64 electrons of a spin
          "ae40": lambda: synth.z4_molecule(40, 20261240, name="ae40"),
          # 144 + 144 electrons: above 128 of a spin the recompute is the panel Gauss-Jordan

@@ -30,7 +30,7 @@ def _say(*a):
 
 
 @pytest.mark.parametrize("prec", ["mixed_fp32", "mixed_bf16", "mixed_bf16x6"])
-@pytest.mark.parametrize("name", ["h4", "pp16", "ae32"])
+@pytest.mark.parametrize("name", ["h4", "pp16", "ae32", "ae17"])
 def test_mixed_eval_errors_against_fp64_oracle(name, prec):
     deck, mol = synth_deck(name)
     eng = E.Engine(deck, prec)
