@@ -604,11 +604,27 @@ __device__ inline void write_det_parts(const DevSys& S, const DevWalkers& W, int
   const int e = off + lane;
   const TM* mr = mog + (size_t)e * 5 * nsp;
   double t[4] = {0, 0, 0, 0};
+  // a lane reads its own electron's rows: 16-byte loads when the rows are full and aligned
+  constexpr int per = 16 / sizeof(TM);
+  if (n == NS && nsp % per == 0) {
+    typedef TM tv16 __attribute__((ext_vector_type(per)));
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const TM* row = mr + (size_t)(c + 1) * nsp;
+    for (int c = 0; c < 4; c++) {
+      const TM* row = mr + (size_t)(c + 1) * nsp;
 #pragma unroll
-    for (int j = 0; j < NS; j++) t[c] += (double)row[min(j, n - 1)] * (double)inv[j];
+      for (int j0 = 0; j0 < NS; j0 += per) {
+        const tv16 u = *(const tv16*)(row + j0);
+#pragma unroll
+        for (int i = 0; i < per; i++) t[c] += (double)u[i] * (double)inv[j0 + i];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const TM* row = mr + (size_t)(c + 1) * nsp;
+#pragma unroll
+      for (int j = 0; j < NS; j++) t[c] += (double)row[min(j, n - 1)] * (double)inv[j];
+    }
   }
   double* o = W.dgl + ((size_t)w * S.ne + e) * 4;
   o[0] = t[0];
