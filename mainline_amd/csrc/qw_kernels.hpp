@@ -596,6 +596,46 @@ static __global__ void k_step_advance(unsigned* stepp) {
 // in registers: dgl[c] = sum_j M_c[e_k][j] inv(k, j), c = d/dx, d/dy, d/dz, laplacian
 // (src/wavefunction/Slat_wf.h:1021-1080, getDetLap).  inv[j] is zero for j >= n, so the row index is
 // clamped instead of branched on.
+// Row `lane` of a spin's transposed inverse into registers (entries >= n zero) and back: with
+// full rows (n = NS) that start 16-byte aligned, 16-byte accesses instead of one value each.
+template <int NS, typename T>
+__device__ inline void inv_row_load(const T* ig, int lane, int n, int nsp, T (&inv)[NS]) {
+  constexpr int per = 16 / sizeof(T);
+  if (n == NS && nsp % per == 0) {
+    typedef T tv16 __attribute__((ext_vector_type(per)));
+    const T* r = ig + (size_t)(lane < n ? lane : 0) * nsp;
+#pragma unroll
+    for (int j0 = 0; j0 < NS; j0 += per) {
+      const tv16 u = lane < n ? *(const tv16*)(r + j0) : tv16(0);
+#pragma unroll
+      for (int i = 0; i < per; i++) inv[j0 + i] = u[i];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NS; j++) inv[j] = (lane < n && j < n) ? ig[(size_t)lane * nsp + j] : (T)0;
+  }
+}
+template <int NS, typename T>
+__device__ inline void inv_row_store(T* ig, int lane, int n, int nsp, const T (&inv)[NS]) {
+  if (lane >= n) return;
+  constexpr int per = 16 / sizeof(T);
+  if (n == NS && nsp % per == 0) {
+    typedef T tv16 __attribute__((ext_vector_type(per)));
+    T* r = ig + (size_t)lane * nsp;
+#pragma unroll
+    for (int j0 = 0; j0 < NS; j0 += per) {
+      tv16 u;
+#pragma unroll
+      for (int i = 0; i < per; i++) u[i] = inv[j0 + i];
+      *(tv16*)(r + j0) = u;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NS; j++)
+      if (j < n) ig[(size_t)lane * nsp + j] = inv[j];
+  }
+}
+
 template <int NS, typename TI, typename TM>
 __device__ inline void write_det_parts(const DevSys& S, const DevWalkers& W, int w, int s, const TI* inv, const TM* mog,
                                        int lane) {
@@ -668,8 +708,7 @@ __device__ __forceinline__ void sweep_fp64_body(const DevSys& S, const DevWalker
   sweep_randoms<DEPTH>(A, (unsigned)w, off, rnd, lane);
   double inv[NS];
   const double* ig = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
-#pragma unroll
-  for (int j = 0; j < NS; j++) inv[j] = (lane < n && j < n) ? ig[(size_t)lane * nsp + j] : 0.0;
+  inv_row_load<NS>(ig, lane, n, nsp, inv);
   double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
   int nacc = 0;
   double dprod = 1.0;
@@ -792,11 +831,7 @@ __device__ __forceinline__ void sweep_fp64_body(const DevSys& S, const DevWalker
     walker_sync<WPB>();
   }
   double* igw = W.inv + ((size_t)w * 2 + s) * nsp * nsp;
-  if (lane < n) {
-#pragma unroll
-    for (int j = 0; j < NS; j++)
-      if (j < n) igw[(size_t)lane * nsp + j] = inv[j];
-  }
+  inv_row_store<NS>(igw, lane, n, nsp, inv);
   if (A.dets) write_det_parts<NS>(S, W, w, s, inv, (const double*)mog, lane);
   for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
   logdet += log(fabs(dprod));

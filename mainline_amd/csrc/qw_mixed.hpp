@@ -371,8 +371,7 @@ __device__ __forceinline__ void sweep_mx_body(const DevSys& S, const MixedTables
     }
   float inv[NS];
   const float* ig = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
-#pragma unroll
-  for (int j = 0; j < NS; j++) inv[j] = (lane < n && j < n) ? ig[(size_t)lane * nsp + j] : 0.f;
+  inv_row_load<NS>(ig, lane, n, nsp, inv);
   double logdet = W.logdet[2 * w + s], sgn = W.sgn[2 * w + s];
   int nacc = 0;
   double dprod = 1.0;
@@ -500,11 +499,7 @@ __device__ __forceinline__ void sweep_mx_body(const DevSys& S, const MixedTables
     walker_sync<WPB>();
   }
   float* igw = W.invf + ((size_t)w * 2 + s) * nsp * nsp;
-  if (lane < n) {
-#pragma unroll
-    for (int j = 0; j < NS; j++)
-      if (j < n) igw[(size_t)lane * nsp + j] = inv[j];
-  }
+  inv_row_store<NS>(igw, lane, n, nsp, inv);
   if (A.dets) write_det_parts<NS>(S, W, w, s, inv, (const float*)mog, lane);
   for (int i = lane; i < 3 * ne; i += kWave) W.pos[(size_t)w * ne * 3 + i] = pos[i];
   logdet += log(fabs(dprod));
