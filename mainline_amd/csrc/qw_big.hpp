@@ -77,6 +77,11 @@ __device__ __forceinline__ void sweep_w_body(const DevSys& S, const DevWalkers& 
                                              const SweepArgs& A, int cap, int w, int lane, char* base) {
   if (w >= W.nw) return;
   const int ne = S.ne, s = A.s, n = s ? S.ndn : S.nup, nsp = S.nsp, off = s ? S.nup : 0;
+  // rows of the inverse and of the MO cache start 16-byte aligned when nsp is a multiple of
+  // the values per 16 bytes
+  constexpr int per = 16 / sizeof(TI);
+  typedef TI tvr __attribute__((ext_vector_type(per)));
+  const bool vrow = nsp % per == 0;
   double* pos = (double*)base;
   SysLds V;
   char* q = sys_lds_carve(S, base + (((size_t)3 * ne * sizeof(double) + 15) & ~(size_t)15), V);
@@ -195,14 +200,34 @@ __device__ __forceinline__ void sweep_w_body(const DevSys& S, const DevWalkers& 
         const int i = lane + kWave * r;
         if (i >= n) continue;
         TI* row = ig + (size_t)i * nsp;
+        // 16-byte accesses to the lane's row where the rows start 16-byte aligned (the same
+        // arithmetic in the same order as the one-value loop that finishes the row)
+        const int jv = vrow ? n / per * per : 0;
         if (i == k) {
           const double f = 1.0 / Rt;
-          for (int j = 0; j < n; j++) row[j] = (TI)((double)rowk[j] * f);
+          for (int j = 0; j < jv; j += per) {
+            tvr u;
+#pragma unroll
+            for (int q = 0; q < per; q++) u[q] = (TI)((double)rowk[j + q] * f);
+            *(tvr*)(row + j) = u;
+          }
+          for (int j = jv; j < n; j++) row[j] = (TI)((double)rowk[j] * f);
         } else {
           double tk = 0;
-          for (int j = 0; j < n; j++) tk += (double)row[j] * (double)phi0[j];
+          for (int j = 0; j < jv; j += per) {
+            const tvr u = *(const tvr*)(row + j);
+#pragma unroll
+            for (int q = 0; q < per; q++) tk += (double)u[q] * (double)phi0[j + q];
+          }
+          for (int j = jv; j < n; j++) tk += (double)row[j] * (double)phi0[j];
           const double pk = tk * (-1.0 / Rt);
-          for (int j = 0; j < n; j++) row[j] = (TI)((double)row[j] + (double)rowk[j] * pk);
+          for (int j = 0; j < jv; j += per) {
+            tvr u = *(const tvr*)(row + j);
+#pragma unroll
+            for (int q = 0; q < per; q++) u[q] = (TI)((double)u[q] + (double)rowk[j + q] * pk);
+            *(tvr*)(row + j) = u;
+          }
+          for (int j = jv; j < n; j++) row[j] = (TI)((double)row[j] + (double)rowk[j] * pk);
         }
         for (int c = 0; c < 5; c++) mog[((size_t)e * 5 + c) * nsp + i] = acc[r][c];
       }
@@ -230,7 +255,20 @@ __device__ __forceinline__ void sweep_w_body(const DevSys& S, const DevWalkers& 
       const TI* row = ig + (size_t)t * nsp;
       const TI* mr = mog + (size_t)(off + t) * 5 * nsp;
       double dv[4] = {0, 0, 0, 0};
-      for (int j = 0; j < n; j++) {
+      const int jv = vrow ? n / per * per : 0;
+      for (int j = 0; j < jv; j += per) {
+        const tvr u = *(const tvr*)(row + j);
+        tvr m[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) m[c] = *(const tvr*)(mr + (size_t)(c + 1) * nsp + j);
+#pragma unroll
+        for (int q = 0; q < per; q++) {
+          const double iv = (double)u[q];
+#pragma unroll
+          for (int c = 0; c < 4; c++) dv[c] += (double)m[c][q] * iv;
+        }
+      }
+      for (int j = jv; j < n; j++) {
         const double iv = (double)row[j];
         for (int c = 0; c < 4; c++) dv[c] += (double)mr[(size_t)(c + 1) * nsp + j] * iv;
       }
