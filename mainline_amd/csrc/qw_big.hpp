@@ -341,9 +341,9 @@ __global__ __launch_bounds__(256) void k_mo_rows_big(DevSys S, const double* wpo
 // its block with one FMA per element.  Every element sees the operations of k_inv_blk in the
 // same order, so the two agree bit for bit on matrices both handle; G = 16 (256 threads)
 // inverts up to 128 electrons of a spin in registers.
-template <int G>
 // The pivot row and the multipliers are stored interleaved (entry 8i + b at index G*b + i), so
 // the eight values a thread reads in a step are spread over the LDS banks.
+template <int G>
 struct InvGridLds {
   double prowL[2][8 * G];
   double gL[2][8 * G];
@@ -386,15 +386,32 @@ __device__ __forceinline__ void k_inv_grid_body(const DevSys& S, const TIN* mo, 
   double vv[8];
   bool zero_row = false;
   int pivm = 0;  // bit r: row 8bi + r already a pivot (rows >= n count as such)
+  // a thread's 8 entries of a row are contiguous: 16-byte loads when every MO row starts on a
+  // 16-byte boundary (entries past n are read, then replaced)
+  const bool vec16 = sizeof(TIN) == 4 ? (nsp & 3) == 0 : (nsp & 1) == 0;
 #pragma unroll
   for (int r = 0; r < 8; r++) {
     const int row = 8 * bi + r;
-    const TIN* rp = mo + ((size_t)w * S.ne + off + (row < n ? row : 0)) * 5 * nsp;
+    const TIN* rp = mo + ((size_t)w * S.ne + off + (row < n ? row : 0)) * 5 * nsp + 8 * bj;
+    TIN vin[8];
+    if (vec16) {
+      constexpr int per = 16 / sizeof(TIN);
+      typedef TIN tv16 __attribute__((ext_vector_type(per)));
+#pragma unroll
+      for (int h = 0; h < 8 / per; h++) {
+        const tv16 u = *(const tv16*)(rp + per * h);
+#pragma unroll
+        for (int i = 0; i < per; i++) vin[per * h + i] = u[i];
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; c++) vin[c] = 8 * bj + c < n ? rp[c] : (TIN)0;
+    }
     double mx = 0;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
       const int col = 8 * bj + c;
-      const double v = (row < n && col < n) ? (double)rp[col] : (row == col ? 1.0 : 0.0);
+      const double v = (row < n && col < n) ? (double)vin[c] : (row == col ? 1.0 : 0.0);
       a[c][r] = v;
       mx = fmax(mx, fabs(v));
     }
