@@ -1586,11 +1586,51 @@ inline size_t tmove_lds_bytes(int ne, int chunk_nf, int ndet = 0, int nsig = 0, 
 // fixed-population pair comb: sort by weight (ties by walker index), pair the p-th largest
 // weight >= 1.8 with the p-th smallest, keep one of the two with probability
 // w_big / (w_big + w_small), both slots get the mean weight.
+// Populations of up to kBranchLdsMax walkers are sorted in LDS (the same network on the same
+// keys as the device-memory sort used above that size, whose every stage waits on L2).
+constexpr int kBranchLdsMax = 8192;
 struct BranchLds {
   double red[1024];
   long long redi[1024];
   int npair;
+  unsigned long long skey[kBranchLdsMax];
+  int sval[kBranchLdsMax];
 };
+// Bitonic sort of the walkers by weight (ties by walker index) into keys/vals[0..P): P a power
+// of two >= nw, padded with the largest key.  Called with LDS or device-memory arrays.
+__device__ __forceinline__ void branch_sort(const DevWalkers& W, unsigned long long* keys, int* vals, int nw, int P,
+                                            int t) {
+  for (int i = t; i < P; i += 1024) {
+    if (i < nw) {
+      keys[i] = (unsigned long long)__double_as_longlong(W.weight[i]);
+      vals[i] = i;
+    } else {
+      keys[i] = ~0ull;
+      vals[i] = i;
+    }
+  }
+  __syncthreads();
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P; i += 1024) {
+        int l = i ^ j;
+        if (l > i) {
+          bool up = (i & k) == 0;
+          unsigned long long ki = keys[i], kl = keys[l];
+          int vi = vals[i], vl = vals[l];
+          bool gt = ki > kl || (ki == kl && vi > vl);
+          if (gt == up) {
+            keys[i] = kl;
+            keys[l] = ki;
+            vals[i] = vl;
+            vals[l] = vi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
 __device__ __forceinline__ void branch_body(const DevWalkers& W, DmcCtl* ctl, unsigned long long* keys, int* vals,
                                             int* pair_src, int* pair_dst, int nmoves, int do_branch,
                                             unsigned long long seed, unsigned event_host, const unsigned* eventp,
@@ -1628,35 +1668,12 @@ __device__ __forceinline__ void branch_body(const DevWalkers& W, DmcCtl* ctl, un
   if (!do_branch) return;
   int P = 1;
   while (P < nw) P <<= 1;
-  for (int i = t; i < P; i += 1024) {
-    if (i < nw) {
-      keys[i] = (unsigned long long)__double_as_longlong(W.weight[i]);
-      vals[i] = i;
-    } else {
-      keys[i] = ~0ull;
-      vals[i] = i;
-    }
-  }
-  __syncthreads();
-  for (int k = 2; k <= P; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = t; i < P; i += 1024) {
-        int l = i ^ j;
-        if (l > i) {
-          bool up = (i & k) == 0;
-          unsigned long long ki = keys[i], kl = keys[l];
-          int vi = vals[i], vl = vals[l];
-          bool gt = ki > kl || (ki == kl && vi > vl);
-          if (gt == up) {
-            keys[i] = kl;
-            keys[l] = ki;
-            vals[i] = vl;
-            vals[l] = vi;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  const bool in_lds = P <= kBranchLdsMax;
+  if (in_lds) {
+    branch_sort(W, L.skey, L.sval, nw, P, t);
+    vals = L.sval;
+  } else {
+    branch_sort(W, keys, vals, nw, P, t);
   }
   // pairs: p-th from the top with p-th from the bottom while the top weight >= 1.8
   if (t == 0) {
